@@ -1,0 +1,191 @@
+/*
+ * tpsph.h -- C ABI of the MI355X-native WCSPH right-hand side (HIP, gfx950).
+ *
+ * The library replaces the per-stage work behind TrixiParticles.jl's ODE surface for one
+ * fluid system (WeaklyCompressibleSPHSystem, ContinuityDensity, StateEquationCole,
+ * ArtificialViscosityMonaghan, DensityDiffusionMolteniColagrossi, WendlandC2Kernel) and one
+ * static wall system (WallBoundarySystem + BoundaryModelDummyParticles +
+ * AdamiPressureExtrapolation):
+ *
+ *   kick!(dv_ode, v_ode, u_ode, p, t)   src/general/semidiscretization.jl:589-612
+ *   drift!(du_ode, v_ode, u_ode, p, t)  src/general/semidiscretization.jl:522-536
+ *
+ * State layout is the reference's (column-major, one column per particle, FP64):
+ *   v_ode : (ndims+1) x n_fluid  = [v_1 .. v_ndims, rho] per particle
+ *           (wcsph/system.jl:240-242, 258-279; the wall block of v_ode is empty)
+ *   u_ode :  ndims    x n_fluid  = [x_1 .. x_ndims]       per particle
+ *   dv_ode, du_ode: same shapes as v_ode, u_ode.
+ * Results are always in the caller's particle order; the library's cell-sorted internal
+ * order is invisible at this boundary.
+ *
+ * Pointers are device pointers when params.memory == TPSPH_MEM_DEVICE and host pointers when
+ * params.memory == TPSPH_MEM_HOST (the library then stages through its own device buffers).
+ * Every call is asynchronous on the handle's stream in device mode; host mode returns after
+ * the results are in host memory.
+ *
+ * Errors: every entry point returns an int status (TPSPH_OK == 0).  The message of the last
+ * failure is available from tpsph_last_error(handle) (or tpsph_last_error(NULL) for a
+ * failed tpsph_create).  Kernels never abort; argument validation mirrors the reference's
+ * ArgumentError checks at construction time.
+ *
+ * A handle is bound to one stream and is not re-entrant; distinct handles may run
+ * concurrently on distinct streams.
+ */
+#ifndef TPSPH_H
+#define TPSPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TPSPH_ABI_VERSION 1
+
+enum tpsph_status {
+    TPSPH_OK = 0,
+    TPSPH_ERR_ARGUMENT = 1, /* invalid argument (ArgumentError in the reference) */
+    TPSPH_ERR_HIP = 2,      /* HIP runtime failure */
+    TPSPH_ERR_CAPACITY = 3, /* output buffer too small (see tpsph_neighbors_csr) */
+    TPSPH_ERR_STATE = 4     /* call not valid in the handle's state */
+};
+
+enum tpsph_memory { TPSPH_MEM_DEVICE = 0, TPSPH_MEM_HOST = 1 };
+
+enum tpsph_kernel_kind { TPSPH_KERNEL_WENDLAND_C2 = 0 };
+
+/* Which neighbour relation tpsph_neighbors_csr reports. */
+enum tpsph_pair {
+    TPSPH_PAIR_FLUID_FLUID = 0, /* rows: fluid particles, columns: fluid particles */
+    TPSPH_PAIR_FLUID_WALL = 1,  /* rows: fluid particles, columns: wall particles  */
+    TPSPH_PAIR_WALL_FLUID = 2   /* rows: wall particles,  columns: fluid particles */
+};
+
+/* Kernel identifiers for tpsph_kernel_time. */
+enum tpsph_timed_kernel {
+    TPSPH_TIME_CELL_LIST = 0,   /* counting sort of the fluid cell list (all its launches) */
+    TPSPH_TIME_GATHER = 1,      /* reorder into cell-sorted SoA + fluid state equation   */
+    TPSPH_TIME_WALL = 2,        /* Adami pressure extrapolation + inverse state equation  */
+    TPSPH_TIME_FLUID = 3,       /* fused fluid-fluid + fluid-wall interaction + gravity   */
+    TPSPH_TIME_STAGE = 4,       /* CK2N54 stage update                                    */
+    TPSPH_TIME_COUNT = 5
+};
+
+/*
+ * Physical and numerical parameters.  Field meanings follow the reference constructors:
+ *   smoothing_length, kernel           WeaklyCompressibleSPHSystem(; smoothing_kernel,
+ *                                      smoothing_length) (wcsph/system.jl:90-174); the wall
+ *                                      model uses the same kernel and length
+ *   reference_density, sound_speed,    StateEquationCole(...) (state_equations.jl:100-139)
+ *   exponent, background_pressure,
+ *   clip_fluid_eos
+ *   use_viscosity, alpha, beta,        ArtificialViscosityMonaghan (viscosity.jl:68-83)
+ *   epsilon
+ *   use_density_diffusion, delta       DensityDiffusionMolteniColagrossi (density_diffusion.jl:41-47)
+ *   acceleration                       `acceleration` of the fluid system (gravity)
+ *   clip_wall_pressure                 BoundaryModelDummyParticles(; clip_negative_pressure)
+ *   pressure_offset                    AdamiPressureExtrapolation(; pressure_offset)
+ *   domain_min, domain_max             bounding box of the cell grid (the reference's
+ *                                      FullGridCellList box); particles outside it are
+ *                                      clamped into the border cells, which keeps the
+ *                                      neighbour search exact but slower there
+ */
+typedef struct tpsph_params {
+    int32_t ndims;                 /* 2 or 3 */
+    int32_t kernel;                /* TPSPH_KERNEL_WENDLAND_C2 */
+    int32_t memory;                /* TPSPH_MEM_DEVICE or TPSPH_MEM_HOST */
+    int32_t clip_fluid_eos;        /* 0/1 */
+    int32_t clip_wall_pressure;    /* 0/1 */
+    int32_t use_viscosity;         /* 0/1 */
+    int32_t use_density_diffusion; /* 0/1 */
+    int32_t reserved0;
+    double smoothing_length;
+    double reference_density;
+    double sound_speed;
+    double exponent;
+    double background_pressure;
+    double alpha;
+    double beta;
+    double epsilon;
+    double delta;
+    double acceleration[3];
+    double pressure_offset;
+    double domain_min[3];
+    double domain_max[3];
+} tpsph_params;
+
+typedef struct tpsph_handle tpsph_handle;
+
+/* Library ABI version (TPSPH_ABI_VERSION of the build). */
+int tpsph_abi_version(void);
+
+/* Create a handle for one fluid + one static wall system.
+ *   fluid_mass:       n_fluid masses            (hydrodynamic_mass, fluid.jl:44-46)
+ *   wall_coordinates: ndims x n_wall            (WallBoundarySystem.coordinates)
+ *   wall_mass:        n_wall masses             (boundary_model.hydrodynamic_mass)
+ *   stream:           hipStream_t to run on (NULL = the null stream)
+ * Replaces semidiscretize(semi, tspan) (semidiscretization.jl:293-396) for this pair: the
+ * static data is copied to the device once and the static wall cell list is built here. */
+int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
+                 const double *fluid_mass, const double *wall_coordinates,
+                 const double *wall_mass, void *stream, tpsph_handle **out);
+
+/* kick!(dv_ode, v_ode, u_ode, p, t) (semidiscretization.jl:589-612): zero dv, rebuild the
+ * fluid cell list (update_nhs!, neighborhood_search.jl:479-519), fluid pressure
+ * (wcsph/system.jl:426-438), Adami wall pressure/density (dummy_particles.jl:489-569),
+ * fluid-fluid and fluid-wall interact! (wcsph/rhs.jl:5-127), gravity
+ * (semidiscretization.jl:668-731). */
+int tpsph_kick(tpsph_handle *h, double *dv_ode, const double *v_ode, const double *u_ode,
+               double t);
+
+/* drift!(du_ode, v_ode, u_ode, p, t) (semidiscretization.jl:522-571): du = v. */
+int tpsph_drift(tpsph_handle *h, double *du_ode, const double *v_ode, const double *u_ode,
+                double t);
+
+/* Quantities of the last kick, in the caller's particle order (any pointer may be NULL):
+ * system.pressure (n_fluid), boundary_model.pressure (n_wall),
+ * boundary_model.cache.density (n_wall). */
+int tpsph_get_pressures(tpsph_handle *h, double *fluid_pressure, double *wall_pressure,
+                        double *wall_density);
+
+/* Neighbour sets of the current positions as CSR (PointNeighbors' foreach_point_neighbor
+ * relation, neighborhood_search.jl:5-13: all b with |x_a - x_b|^2 <= (2h)^2, self
+ * included), rows and column indices in the caller's order, columns ascending per row.
+ * offsets has n_rows+1 entries.  With idx == NULL (or capacity too small) only offsets are
+ * written and the call returns TPSPH_OK (resp. TPSPH_ERR_CAPACITY); offsets[n_rows] is the
+ * total.  u_ode is the fluid position array as in tpsph_kick. */
+int tpsph_neighbors_csr(tpsph_handle *h, const double *u_ode, int32_t pair, int64_t *offsets,
+                        int32_t *idx, int64_t capacity);
+
+/* One CarpenterKennedy2N54 stage over y (n values) with register tmp and slope k:
+ *   first stage: tmp = dt*k;              y = y + b*tmp
+ *   otherwise:   tmp = a*tmp + dt*k;      y = y + b*tmp      (2N-storage form)
+ * The building block a device-array type needs to run OrdinaryDiffEq's low-storage
+ * update without leaving the device. */
+int tpsph_rk_stage(tpsph_handle *h, double *y, double *tmp, const double *k, double a,
+                   double b, double dt, int64_t n, int32_t first_stage);
+
+/* nsteps fixed steps of CarpenterKennedy2N54(williamson_condition=false) from t0, fully on
+ * the device: per stage kick! + drift! + the fused stage update of (v_ode, u_ode). */
+int tpsph_solve_ck2n54(tpsph_handle *h, double *v_ode, double *u_ode, double t0, double dt,
+                       int64_t nsteps);
+
+/* Wait for all work queued on the handle's stream. */
+int tpsph_synchronize(tpsph_handle *h);
+
+/* Per-kernel HIP-event timing (events recorded on the handle's stream).  enable != 0
+ * starts (and resets) timing; tpsph_kernel_time synchronises and reports the number of
+ * launches and their summed duration in milliseconds since timing was enabled. */
+int tpsph_enable_timing(tpsph_handle *h, int32_t enable);
+int tpsph_kernel_time(tpsph_handle *h, int32_t which, int64_t *launches, double *total_ms);
+
+/* Message of the last failure on h (or of the last failed tpsph_create when h == NULL). */
+const char *tpsph_last_error(const tpsph_handle *h);
+
+void tpsph_destroy(tpsph_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TPSPH_H */

@@ -1,0 +1,92 @@
+"""The C-ABI library loads, exports every symbol include/tpsph.h declares, and rejects
+invalid parameters at create time (validation runs before any HIP call, so this needs no GPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import REPO
+
+
+def declared_symbols():
+    text = open(os.path.join(REPO, "include", "tpsph.h")).read()
+    return sorted(set(re.findall(r"\b(tpsph_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(tp):
+    lib = tp.native.load()
+    syms = declared_symbols()
+    assert "tpsph_kick" in syms and "tpsph_create" in syms
+    for s in syms:
+        assert hasattr(lib, s), f"{s} declared in include/tpsph.h but not exported"
+    assert set(tp.native.EXPORTED_SYMBOLS) == set(syms)
+    assert lib.tpsph_abi_version() == 1
+
+
+def test_params_struct_layout_matches_header(tp):
+    # 8 int32 + 9 doubles + 3 + 1 + 3 + 3 doubles
+    assert C.sizeof(tp.native.TpsphParams) == 8 * 4 + (9 + 3 + 1 + 3 + 3) * 8
+
+
+def _create(tp, **overrides):
+    lib = tp.native.load()
+    p = tp.native.TpsphParams()
+    p.ndims = 2
+    p.memory = tp.native.TPSPH_MEM_HOST
+    p.smoothing_length = 0.012
+    p.reference_density = 1000.0
+    p.sound_speed = 48.5
+    p.exponent = 7.0
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    out = C.c_void_p()
+    mass = (C.c_double * 1)(1.0)
+    rc = lib.tpsph_create(C.byref(p), 1, 0, C.cast(mass, C.c_void_p), None, None, None,
+                          C.byref(out))
+    return rc, lib.tpsph_last_error(None).decode()
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("ndims", 4, "ndims"),
+    ("smoothing_length", 0.0, "smoothing_length"),
+    ("smoothing_length", float("nan"), "smoothing_length"),
+    ("reference_density", 0.0, "reference_density"),
+    ("sound_speed", -1.0, "sound_speed"),
+    ("exponent", 0.0, "exponent"),
+    ("kernel", 3, "WendlandC2"),
+    ("memory", 7, "memory"),
+])
+def test_create_rejects_invalid_parameters(tp, field, value, msg):
+    rc, err = _create(tp, **{field: value})
+    assert rc == tp.native.TPSPH_ERR_ARGUMENT
+    assert msg in err
+
+
+def test_create_rejects_null_pointers(tp):
+    lib = tp.native.load()
+    p = tp.native.TpsphParams()
+    p.ndims = 3
+    p.memory = tp.native.TPSPH_MEM_HOST
+    p.smoothing_length = p.reference_density = p.sound_speed = p.exponent = 1.0
+    out = C.c_void_p()
+    assert lib.tpsph_create(C.byref(p), 10, 0, None, None, None, None, C.byref(out)) == \
+        tp.native.TPSPH_ERR_ARGUMENT
+    assert lib.tpsph_create(None, 10, 0, None, None, None, None, C.byref(out)) == \
+        tp.native.TPSPH_ERR_ARGUMENT
+    assert lib.tpsph_kick(None, None, None, None, 0.0) == tp.native.TPSPH_ERR_ARGUMENT
+
+
+def test_system_constructors_reject_out_of_scope_options(tp):
+    ic = tp.RectangularShape(0.1, (3, 3), (0.0, 0.0), density=1000.0)
+    se = tp.StateEquationCole(sound_speed=10.0, reference_density=1000.0, exponent=7)
+    k = tp.WendlandC2Kernel(2)
+    with pytest.raises(NotImplementedError):
+        tp.WeaklyCompressibleSPHSystem(ic, tp.ContinuityDensity(), se, k, 0.12,
+                                       surface_tension=object())
+    with pytest.raises(ValueError):
+        tp.WeaklyCompressibleSPHSystem(ic, tp.ContinuityDensity(), se, tp.WendlandC2Kernel(3),
+                                       0.12)
+    with pytest.raises(NotImplementedError):
+        tp.BoundaryModelDummyParticles([1000.0], [1.0], tp.AdamiPressureExtrapolation(), k,
+                                       0.12, state_equation=se, viscosity=object())

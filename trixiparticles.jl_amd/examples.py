@@ -1,0 +1,68 @@
+"""Dam-break setups of the benchmark workloads, assembled like
+``examples/fluid/dam_break_2d.jl`` / ``dam_break_3d.jl``:
+WendlandC2 kernel with h = 1.2 dx, StateEquationCole(rho0 = 1000, c = 20 sqrt(g H), exponent
+7), ArtificialViscosityMonaghan(alpha = 0.02), DensityDiffusionMolteniColagrossi(delta = 0.1),
+ContinuityDensity, AdamiPressureExtrapolation walls with clip_negative_pressure = true,
+gravity -9.81 e_y, a RectangularTank with 3 dummy-particle layers and an open top, and a
+lattice fluid block with hydrostatic density plus seeded uniform jitter of +-0.01 dx."""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+from .schemes import (AdamiPressureExtrapolation, ArtificialViscosityMonaghan,
+                      BoundaryModelDummyParticles, ContinuityDensity,
+                      DensityDiffusionMolteniColagrossi, StateEquationCole, WallBoundarySystem,
+                      WeaklyCompressibleSPHSystem, WendlandC2Kernel)
+from .setups import RectangularTank
+
+__all__ = ["DamBreak", "dam_break"]
+
+
+@dataclass
+class DamBreak:
+    tank: RectangularTank
+    fluid_system: WeaklyCompressibleSPHSystem
+    boundary_system: WallBoundarySystem
+    state_equation: StateEquationCole
+    particle_spacing: float
+
+    @property
+    def n_fluid(self):
+        return self.fluid_system.n_particles
+
+    @property
+    def n_wall(self):
+        return self.boundary_system.n_particles
+
+
+def dam_break(ndims=2, particle_spacing=0.01, fluid_size=None, tank_size=None, *,
+              alpha=0.02, delta=0.1, gravity=9.81, n_layers=3, open_top=True,
+              jitter=0.01, seed=0, smoothing_factor=1.2, exponent=7.0, water_height=0.6,
+              clip_wall_pressure=True):
+    """Dam break with the workload defaults: 2D 1.2 x 0.6 m water in a 3.2 x 1.0 m tank, 3D
+    1.0 x 0.5 x 0.5 m water in a 3.2 x 1.0 x 1.0 m tank."""
+    if fluid_size is None:
+        fluid_size = (1.2, 0.6) if ndims == 2 else (1.0, 0.5, 0.5)
+    if tank_size is None:
+        tank_size = (3.2, 1.0) if ndims == 2 else (3.2, 1.0, 1.0)
+    sound_speed = 20 * math.sqrt(gravity * water_height)
+    state_equation = StateEquationCole(sound_speed=sound_speed, reference_density=1000.0,
+                                       exponent=exponent, clip_negative_pressure=False)
+    acceleration = (0.0, -gravity) if ndims == 2 else (0.0, -gravity, 0.0)
+    faces = (True, True, True, not open_top) if ndims == 2 else \
+        (True, True, True, not open_top, True, True)
+    tank = RectangularTank(particle_spacing, fluid_size, tank_size, 1000.0, n_layers=n_layers,
+                           acceleration=acceleration, state_equation=state_equation,
+                           faces=faces, coordinates_perturbation=jitter, seed=seed)
+    h = smoothing_factor * particle_spacing
+    kernel = WendlandC2Kernel(ndims)
+    fluid_system = WeaklyCompressibleSPHSystem(
+        tank.fluid, ContinuityDensity(), state_equation, kernel, h,
+        acceleration=acceleration, viscosity=ArtificialViscosityMonaghan(alpha=alpha, beta=0.0),
+        density_diffusion=DensityDiffusionMolteniColagrossi(delta=delta))
+    boundary_model = BoundaryModelDummyParticles(
+        tank.boundary.density, tank.boundary.mass, AdamiPressureExtrapolation(), kernel, h,
+        state_equation=state_equation, clip_negative_pressure=clip_wall_pressure)
+    boundary_system = WallBoundarySystem(tank.boundary, boundary_model)
+    return DamBreak(tank, fluid_system, boundary_system, state_equation, particle_spacing)

@@ -1,0 +1,331 @@
+"""ODE surface: ``Semidiscretization``, ``semidiscretize``, ``kick!``/``drift!`` and a
+``CarpenterKennedy2N54`` driver, mirroring ``src/general/semidiscretization.jl``.
+
+The ODE state keeps the reference's layout (``semidiscretization.jl:128-135``):
+``v_ode`` is ``(ndims+1) x n_fluid`` (velocity + density, ``wcsph/system.jl:240-242``) and
+``u_ode`` is ``ndims x n_fluid``; the wall block of both is empty.  Arrays are stored
+column-major like Julia's, i.e. as C-contiguous arrays of shape ``(n_fluid, ndims+1)`` /
+``(n_fluid, ndims)`` whose rows are the reference's columns.
+
+Two backends:
+
+* ``"hip"`` (default): state in device memory (torch tensors on ``cuda:0`` as plain device
+  buffers); ``kick``/``drift`` launch the HIP kernels through the C ABI, asynchronously.
+* ``"hip_host"``: state in host numpy arrays; the library stages through device buffers.
+
+There is no CPU compute path here: both backends run the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import native
+from .schemes import WallBoundarySystem, WeaklyCompressibleSPHSystem
+
+__all__ = ["Semidiscretization", "semidiscretize", "DynamicalODEProblem", "kick", "drift",
+           "CarpenterKennedy2N54", "solve", "CK2N54_A", "CK2N54_B", "CK2N54_C"]
+
+# Carpenter & Kennedy (1994), NASA TM-109112, five-stage fourth-order 2N-storage scheme.
+CK2N54_A = (0.0, -567301805773 / 1357537059087, -2404267990393 / 2016746695238,
+            -3550918686646 / 2091501179385, -1275806237668 / 842570457699)
+CK2N54_B = (1432997174477 / 9575080441755, 5161836677717 / 13612068292357,
+            1720146321549 / 2090206949498, 3134564353537 / 4481467310338,
+            2277821191437 / 14882151754819)
+CK2N54_C = (0.0, 1432997174477 / 9575080441755, 2526269341429 / 6820363962896,
+            2006345519317 / 3224310063776, 2802321613138 / 2924317926251)
+
+
+def _ptr(a):
+    """Raw address of a numpy array or torch tensor (None for None)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags.c_contiguous or a.dtype != np.float64:
+            raise ValueError("arrays must be C-contiguous float64")
+        return a.ctypes.data
+    if not a.is_contiguous():
+        raise ValueError("tensors must be contiguous")
+    return a.data_ptr()
+
+
+class Semidiscretization:
+    """``Semidiscretization(systems...; neighborhood_search, parallelization_backend)``
+    (semidiscretization.jl:106-155) for one ``WeaklyCompressibleSPHSystem`` and at most one
+    static ``WallBoundarySystem``.
+
+    ``domain``: optional ``(min_corner, max_corner)`` of the cell grid (the reference's
+    ``FullGridCellList`` box).  By default it is the bounding box of both systems, grown
+    by ``domain_margin`` times its extent on every side."""
+
+    def __init__(self, *systems, neighborhood_search=None, parallelization_backend="hip",
+                 domain=None, domain_margin=0.25, device=0):
+        fluids = [s for s in systems if isinstance(s, WeaklyCompressibleSPHSystem)]
+        walls = [s for s in systems if isinstance(s, WallBoundarySystem)]
+        if len(fluids) != 1 or len(walls) > 1 or len(fluids) + len(walls) != len(systems):
+            raise NotImplementedError("the HIP right-hand side handles one "
+                                      "WeaklyCompressibleSPHSystem and at most one "
+                                      "WallBoundarySystem")
+        if parallelization_backend not in ("hip", "hip_host"):
+            raise ValueError("parallelization_backend must be 'hip' or 'hip_host'")
+        self.systems = tuple(systems)
+        self.fluid = fluids[0]
+        self.wall = walls[0] if walls else None
+        self.backend = parallelization_backend
+        self.device = device
+        nd = self.fluid.ndims
+        if self.wall is not None:
+            if self.wall.ndims != nd:
+                raise ValueError("all systems must have the same number of dimensions")
+            bm = self.wall.boundary_model
+            if type(bm.smoothing_kernel) is not type(self.fluid.smoothing_kernel) or \
+                    not math.isclose(bm.smoothing_length, self.fluid.smoothing_length,
+                                     rel_tol=0, abs_tol=0):
+                raise NotImplementedError("the wall model must use the fluid's smoothing "
+                                          "kernel and smoothing length")
+        if domain is None:
+            pts = [self.fluid.initial_condition.coordinates]
+            if self.wall is not None:
+                pts.append(self.wall.coordinates)
+            allp = np.concatenate(pts, axis=1)
+            lo, hi = allp.min(axis=1), allp.max(axis=1)
+            ext = hi - lo
+            domain = (lo - domain_margin * ext, hi + domain_margin * ext)
+        self.domain = (np.asarray(domain[0], dtype=np.float64),
+                       np.asarray(domain[1], dtype=np.float64))
+        self.neighborhood_search = neighborhood_search
+        self._handle = None
+
+    # -- parameters of the C ABI
+    def params(self) -> native.TpsphParams:
+        f = self.fluid
+        se = f.state_equation
+        p = native.TpsphParams()
+        p.ndims = f.ndims
+        p.kernel = 0
+        p.memory = native.TPSPH_MEM_DEVICE if self.backend == "hip" else native.TPSPH_MEM_HOST
+        p.clip_fluid_eos = int(se.clip_negative_pressure)
+        p.smoothing_length = f.smoothing_length
+        p.reference_density = se.reference_density
+        p.sound_speed = se.sound_speed
+        p.exponent = se.exponent
+        p.background_pressure = se.background_pressure
+        if f.viscosity is not None:
+            p.use_viscosity = 1
+            p.alpha, p.beta, p.epsilon = f.viscosity.alpha, f.viscosity.beta, f.viscosity.epsilon
+        if f.density_diffusion is not None:
+            p.use_density_diffusion = 1
+            p.delta = f.density_diffusion.delta
+        for d in range(f.ndims):
+            p.acceleration[d] = f.acceleration[d]
+            p.domain_min[d] = self.domain[0][d]
+            p.domain_max[d] = self.domain[1][d]
+        if self.wall is not None:
+            bm = self.wall.boundary_model
+            p.clip_wall_pressure = int(bm.clip_negative_pressure)
+            p.pressure_offset = bm.density_calculator.pressure_offset
+            wse = bm.state_equation
+            if (wse.sound_speed, wse.reference_density, wse.exponent,
+                    wse.background_pressure) != (se.sound_speed, se.reference_density,
+                                                 se.exponent, se.background_pressure):
+                raise NotImplementedError("the wall model must use the fluid's state equation")
+        return p
+
+    # -- array helpers for the configured backend
+    def _zeros(self, *shape):
+        if self.backend == "hip":
+            import torch
+            return torch.zeros(shape, dtype=torch.float64, device=f"cuda:{self.device}")
+        return np.zeros(shape, dtype=np.float64)
+
+    def _array(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if self.backend == "hip":
+            import torch
+            return torch.from_numpy(a).to(f"cuda:{self.device}")
+        return a
+
+    @property
+    def handle(self):
+        if self._handle is None:
+            lib = native.load()
+            p = self.params()
+            nf = self.fluid.n_particles
+            nw = self.wall.n_particles if self.wall is not None else 0
+            fm = self._array(self.fluid.mass)
+            if self.wall is not None:
+                wx = self._array(self.wall.coordinates.T)
+                wm = self._array(self.wall.boundary_model.hydrodynamic_mass)
+            else:
+                wx = wm = None
+            out = C.c_void_p()
+            rc = lib.tpsph_create(C.byref(p), nf, nw, _ptr(fm), _ptr(wx), _ptr(wm), None,
+                                  C.byref(out))
+            native.check(rc, None)
+            self._handle = out
+            self._keep = (fm, wx, wm)
+        return self._handle
+
+    def close(self):
+        if self._handle is not None:
+            native.load().tpsph_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state
+    def initial_state(self):
+        """``write_v0!``/``write_u0!`` (fluid.jl:79-97, wcsph/system.jl:440-445)."""
+        ic = self.fluid.initial_condition
+        v0 = np.concatenate([ic.velocity, ic.density.reshape(1, -1)], axis=0).T
+        u0 = ic.coordinates.T
+        return self._array(v0), self._array(u0)
+
+    def synchronize(self):
+        native.check(native.load().tpsph_synchronize(self.handle), self.handle)
+
+    # -- quantities read by callbacks
+    def update_pressures(self):
+        """Copy ``system.pressure``, ``boundary_model.pressure`` and
+        ``boundary_model.cache.density`` of the last kick to the host objects."""
+        f, w = self.fluid, self.wall
+        pw = w.boundary_model.pressure if w is not None else None
+        rw = w.boundary_model.density if w is not None else None
+        if self.backend == "hip_host":
+            native.check(native.load().tpsph_get_pressures(
+                self.handle, f.pressure.ctypes.data,
+                pw.ctypes.data if pw is not None else None,
+                rw.ctypes.data if rw is not None else None), self.handle)
+        else:
+            self._get_pressures_device()
+        return f.pressure, pw, rw
+
+    def _get_pressures_device(self):
+        import torch
+        f, w = self.fluid, self.wall
+        dev = f"cuda:{self.device}"
+        pf = torch.empty(f.n_particles, dtype=torch.float64, device=dev)
+        nw = w.n_particles if w is not None else 0
+        pw = torch.empty(max(nw, 1), dtype=torch.float64, device=dev)
+        rw = torch.empty(max(nw, 1), dtype=torch.float64, device=dev)
+        native.check(native.load().tpsph_get_pressures(
+            self.handle, pf.data_ptr(), pw.data_ptr() if nw else None,
+            rw.data_ptr() if nw else None), self.handle)
+        f.pressure[:] = pf.cpu().numpy()
+        if nw:
+            w.boundary_model.pressure[:] = pw.cpu().numpy()
+            w.boundary_model.density[:] = rw.cpu().numpy()
+
+    def neighbors(self, u_ode, pair=native.PAIR_FLUID_FLUID):
+        """Neighbour CSR ``(offsets, indices)`` of the current positions (host arrays)."""
+        lib = native.load()
+        nf = self.fluid.n_particles
+        nw = self.wall.n_particles if self.wall is not None else 0
+        nrows = nw if pair == native.PAIR_WALL_FLUID else nf
+        offsets = np.zeros(nrows + 1, dtype=np.int64)
+        if self.backend == "hip":
+            import torch
+            off_d = torch.zeros(nrows + 1, dtype=torch.int64, device=f"cuda:{self.device}")
+            native.check(lib.tpsph_neighbors_csr(self.handle, _ptr(u_ode), pair,
+                                                 off_d.data_ptr(), None, 0), self.handle)
+            offsets = off_d.cpu().numpy()
+            idx_d = torch.zeros(max(int(offsets[-1]), 1), dtype=torch.int32,
+                                device=f"cuda:{self.device}")
+            native.check(lib.tpsph_neighbors_csr(self.handle, _ptr(u_ode), pair,
+                                                 off_d.data_ptr(), idx_d.data_ptr(),
+                                                 idx_d.numel()), self.handle)
+            return offsets, idx_d.cpu().numpy()[: int(offsets[-1])]
+        native.check(lib.tpsph_neighbors_csr(self.handle, _ptr(u_ode), pair,
+                                             offsets.ctypes.data, None, 0), self.handle)
+        idx = np.zeros(max(int(offsets[-1]), 1), dtype=np.int32)
+        native.check(lib.tpsph_neighbors_csr(self.handle, _ptr(u_ode), pair,
+                                             offsets.ctypes.data, idx.ctypes.data, idx.size),
+                     self.handle)
+        return offsets, idx[: int(offsets[-1])]
+
+
+def kick(dv_ode, v_ode, u_ode, p, t):
+    """``kick!(dv_ode, v_ode, u_ode, p, t)`` (semidiscretization.jl:589-612)."""
+    semi = p["semi"]
+    native.check(native.load().tpsph_kick(semi.handle, _ptr(dv_ode), _ptr(v_ode), _ptr(u_ode),
+                                          float(t)), semi.handle)
+    return dv_ode
+
+
+def drift(du_ode, v_ode, u_ode, p, t):
+    """``drift!(du_ode, v_ode, u_ode, p, t)`` (semidiscretization.jl:522-536)."""
+    semi = p["semi"]
+    native.check(native.load().tpsph_drift(semi.handle, _ptr(du_ode), _ptr(v_ode),
+                                           _ptr(u_ode), float(t)), semi.handle)
+    return du_ode
+
+
+@dataclass
+class DynamicalODEProblem:
+    """``DynamicalODEProblem(kick!, drift!, v0, u0, tspan, p)`` (semidiscretization.jl:394-395)."""
+
+    f1: object
+    f2: object
+    v0: object
+    u0: object
+    tspan: tuple
+    p: dict
+
+
+def semidiscretize(semi: Semidiscretization, tspan) -> DynamicalODEProblem:
+    """``semidiscretize(semi, tspan)`` (semidiscretization.jl:293-396): initial state in the
+    backend's memory, static data copied to the device once (handle creation)."""
+    v0, u0 = semi.initial_state()
+    _ = semi.handle
+    return DynamicalODEProblem(kick, drift, v0, u0, tuple(tspan),
+                               {"semi": semi, "split_integration_data": None})
+
+
+@dataclass
+class CarpenterKennedy2N54:
+    """``CarpenterKennedy2N54(williamson_condition=false)`` (OrdinaryDiffEqLowStorageRK)."""
+
+    williamson_condition: bool = False
+
+
+def solve(ode: DynamicalODEProblem, alg=None, *, dt, nsteps=None, callback=None):
+    """Fixed-step ``solve(ode, CarpenterKennedy2N54(); dt, adaptive=false)``.
+
+    Without ``callback`` the steps run entirely on the device in one library call
+    (``tpsph_solve_ck2n54``).  With ``callback(v, u, t, step)`` every step is driven from
+    here through ``kick``/``drift`` and ``tpsph_rk_stage``, the same sequence OrdinaryDiffEq
+    runs.  Returns the final ``(v, u)`` (copies of the initial state, advanced)."""
+    semi = ode.p["semi"]
+    t0, t1 = ode.tspan
+    if nsteps is None:
+        nsteps = int(round((t1 - t0) / dt))
+    v = ode.v0.clone() if hasattr(ode.v0, "clone") else ode.v0.copy()
+    u = ode.u0.clone() if hasattr(ode.u0, "clone") else ode.u0.copy()
+    lib = native.load()
+    if callback is None:
+        native.check(lib.tpsph_solve_ck2n54(semi.handle, _ptr(v), _ptr(u), float(t0),
+                                            float(dt), int(nsteps)), semi.handle)
+        return v, u
+    dv, du = semi._zeros(*v.shape), semi._zeros(*u.shape)
+    tv, tu = semi._zeros(*v.shape), semi._zeros(*u.shape)
+    t = t0
+    for step in range(int(nsteps)):
+        for i in range(5):
+            ts = t + CK2N54_C[i] * dt
+            ode.f1(dv, v, u, ode.p, ts)
+            ode.f2(du, v, u, ode.p, ts)
+            for y, tmp, k in ((v, tv, dv), (u, tu, du)):
+                native.check(lib.tpsph_rk_stage(semi.handle, _ptr(y), _ptr(tmp), _ptr(k),
+                                                CK2N54_A[i], CK2N54_B[i], float(dt),
+                                                int(np.prod(y.shape)), int(i == 0)),
+                             semi.handle)
+        t = t0 + (step + 1) * dt
+        callback(v, u, t, step + 1)
+    return v, u
