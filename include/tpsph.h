@@ -89,6 +89,16 @@ enum tpsph_timed_kernel {
  *                                      FullGridCellList box); particles outside it are
  *                                      clamped into the border cells, which keeps the
  *                                      neighbour search exact but slower there
+ *   cell_divisions                     cells per compact support along y and z (0 = 2)
+ *   x_cell_divisions                   cells per compact support along x (0 = 2 cell_divisions);
+ *                                      x is the fastest-varying cell index
+ *   skin                               Verlet skin of the fluid cell list as a fraction of
+ *                                      the compact support (< 0 = default 0.04): the
+ *                                      cell-sorted order is rebuilt only once a particle has
+ *                                      moved by more than skin/2 since the last rebuild.
+ *                                      Every kick still tests all pairs exactly against the
+ *                                      current positions, so the neighbour relation is the
+ *                                      reference's one whatever the skin.
  */
 typedef struct tpsph_params {
     int32_t ndims;                 /* 2 or 3 */
@@ -98,6 +108,8 @@ typedef struct tpsph_params {
     int32_t clip_wall_pressure;    /* 0/1 */
     int32_t use_viscosity;         /* 0/1 */
     int32_t use_density_diffusion; /* 0/1 */
+    int32_t cell_divisions;        /* 0 = default (2) */
+    int32_t x_cell_divisions;      /* 0 = default (2 * cell_divisions) */
     int32_t reserved0;
     double smoothing_length;
     double reference_density;
@@ -112,6 +124,7 @@ typedef struct tpsph_params {
     double pressure_offset;
     double domain_min[3];
     double domain_max[3];
+    double skin;                   /* < 0 = default (0.04) */
 } tpsph_params;
 
 typedef struct tpsph_handle tpsph_handle;

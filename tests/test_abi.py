@@ -25,8 +25,8 @@ def test_library_exports_every_declared_symbol(tp):
 
 
 def test_params_struct_layout_matches_header(tp):
-    # 8 int32 + 9 doubles + 3 + 1 + 3 + 3 doubles
-    assert C.sizeof(tp.native.TpsphParams) == 8 * 4 + (9 + 3 + 1 + 3 + 3) * 8
+    # 10 int32 + 9 doubles + acceleration[3] + offset + domain_min[3] + domain_max[3] + skin
+    assert C.sizeof(tp.native.TpsphParams) == 10 * 4 + (9 + 3 + 1 + 3 + 3 + 1) * 8
 
 
 def _create(tp, **overrides):

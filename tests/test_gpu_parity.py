@@ -19,10 +19,10 @@ def rel_err(got, ref):
     return np.abs(got - ref).max() / (scale if scale > 0 else 1.0)
 
 
-def setup_case(tp, nd, dx, backend="hip", **kw):
+def setup_case(tp, nd, dx, backend="hip", grid=None, **kw):
     db = tp.dam_break(nd, dx, **kw)
     semi = tp.Semidiscretization(db.fluid_system, db.boundary_system,
-                                 parallelization_backend=backend)
+                                 parallelization_backend=backend, **(grid or {}))
     return db, semi
 
 
@@ -48,9 +48,18 @@ def check_columns(got, ref, rtol=RTOL, what="dv"):
         assert err <= rtol, f"{what} column {col}: norm-wise relative error {err:.3e}"
 
 
+GRIDS = [
+    {},
+    {"cell_divisions": 1, "x_cell_divisions": 1, "skin": 0.0},
+    {"cell_divisions": 3, "x_cell_divisions": 5, "skin": 0.3},
+    {"cell_divisions": 2, "x_cell_divisions": 8, "skin": 0.1, "domain_margin": 0.0},
+]
+
+
+@pytest.mark.parametrize("grid", GRIDS)
 @pytest.mark.parametrize("nd,dx", [(2, 0.01), (3, 0.02)])
-def test_neighbor_sets_match_bruteforce(tp, orc, nd, dx):
-    db, semi = setup_case(tp, nd, dx)
+def test_neighbor_sets_match_bruteforce(tp, orc, nd, dx, grid):
+    db, semi = setup_case(tp, nd, dx, grid=grid)
     v0, u0 = semi.initial_state()
     u = u0.cpu().numpy()
     xw = db.boundary_system.coordinates.T.copy()
@@ -64,9 +73,10 @@ def test_neighbor_sets_match_bruteforce(tp, orc, nd, dx):
         assert np.array_equal(idx.astype(np.int64), ridx), f"pair {pair}: neighbour sets differ"
 
 
-@pytest.mark.parametrize("nd,dx", [(2, 0.01), (2, 0.02), (3, 0.025), (3, 0.0125)])
-def test_kick_matches_oracle(tp, orc, nd, dx):
-    db, semi = setup_case(tp, nd, dx)
+@pytest.mark.parametrize("nd,dx,grid", [(2, 0.01, {}), (2, 0.02, GRIDS[1]), (3, 0.025, {}),
+                                         (3, 0.0125, {}), (3, 0.025, GRIDS[2])])
+def test_kick_matches_oracle(tp, orc, nd, dx, grid):
+    db, semi = setup_case(tp, nd, dx, grid=grid)
     v0, u0 = semi.initial_state()
     v, u = v0.cpu().numpy(), u0.cpu().numpy()
     # a non-trivial velocity field so viscosity, continuity and diffusion all contribute
@@ -143,16 +153,41 @@ def test_callback_driver_matches_native_solver(tp):
     assert np.array_equal(u1.cpu().numpy(), u2.cpu().numpy())
 
 
-def test_particles_outside_domain_are_clamped(tp, orc):
-    """Particles leaving the grid box are clamped into border cells; the search stays exact."""
-    db, semi = setup_case(tp, 2, 0.02)
+@pytest.mark.parametrize("nd", [2, 3])
+def test_particles_outside_domain_are_clamped(tp, orc, nd):
+    """Particles leaving the grid box are clamped into border cells; the search stays exact,
+    also for pairs that are both outside and close to each other."""
+    db, semi = setup_case(tp, nd, 0.02 if nd == 2 else 0.04)
     v0, u0 = semi.initial_state()
     v, u = v0.cpu().numpy(), u0.cpu().numpy().copy()
+    h = db.fluid_system.smoothing_length
     u[:5, 1] += 50.0  # far above the tank and the domain box
     u[5:8, 0] -= 30.0
+    u[8, :] = u[5, :] + 0.7 * h  # a close pair outside the box
     dv_ref, *_ = oracle_kick(orc, semi, db, v, u)
     got = gpu_kick(tp, semi, v, u)
     check_columns(got, dv_ref)
+    R = 2 * h
+    import torch
+    off, idx = semi.neighbors(torch.from_numpy(u).cuda())
+    roff, ridx = orc.neighbors(nd, R, u, u, brute=True)
+    assert np.array_equal(off, roff) and np.array_equal(idx.astype(np.int64), ridx)
+
+
+def test_skin_rebuild_keeps_results_exact(tp, orc):
+    """Kicks at positions that moved less and more than half the skin after the last
+    rebuild both match the oracle (the cell order is reused, then rebuilt)."""
+    db, semi = setup_case(tp, 3, 0.025, grid={"skin": 0.2})
+    v0, u0 = semi.initial_state()
+    v, u = v0.cpu().numpy(), u0.cpu().numpy()
+    R = 2 * db.fluid_system.smoothing_length
+    rng = np.random.default_rng(3)
+    gpu_kick(tp, semi, v, u)  # builds the cell list
+    for amp in (0.04 * R, 0.09 * R, 0.4 * R, 0.01 * R):
+        u = u + rng.uniform(-amp, amp, size=u.shape)
+        dv_ref, *_ = oracle_kick(orc, semi, db, v, u)
+        got = gpu_kick(tp, semi, v, u)
+        check_columns(got, dv_ref)
 
 
 def test_full_size_3d_kick_matches_oracle(tp, orc):

@@ -1,27 +1,37 @@
 // tpsph.hip -- MI355X (gfx950) WCSPH right-hand side behind the C ABI of include/tpsph.h.
 //
-// Pipeline of one kick! (semidiscretization.jl:589-612), all on the handle's stream:
-//   1. fluid cell list: linear cell id + per-cell histogram (atomics)      [k_fluid_cells]
-//      exclusive scan of the histogram                                     [k_scan_*]
-//      scatter + per-cell index sort (a stable counting sort)              [k_scatter, k_sort_cells]
-//   2. gather the fluid state into cell-sorted SoA, fluid state equation   [k_gather]
-//   3. Adami pressure extrapolation + inverse state equation per wall      [k_wall]
-//   4. fused fluid-fluid + fluid-wall interaction + gravity, written back
-//      to the caller's particle order                                      [k_fluid]
-// The static wall cell list is built once at create.
+// One kick! (semidiscretization.jl:589-612) runs, on the handle's stream:
+//   1. k_gather<0>   gather the fluid state into cell-sorted order with the current
+//                    permutation, fluid state equation, and flag a rebuild if any particle
+//                    moved by more than half the skin since the last cell-list build;
+//   2. rebuild, only when flagged (every kernel reads the flag and returns otherwise):
+//      k_cells + k_scan_* + k_scatter + k_sort_cells (a stable counting sort on the linear
+//      cell index) and k_gather<1> (gather again, snapshot of the build positions);
+//   3. k_wall        Adami pressure extrapolation + inverse state equation per wall particle;
+//   4. k_fluid       fused fluid-fluid + fluid-wall interaction + gravity, results scattered
+//                    back to the caller's particle order.
+// The wall cell list is static and built once at create.
 //
-// Cells have edge R (1 + 2^-30), R = compact support 2h, so every pair with d^2 <= R^2 lies
-// in adjacent cells even after rounding of the cell coordinates.  Cell coordinates are
-// clamped into the grid, which keeps the search exact for particles outside the box.
+// Neighbour search.  Cells have edge R/k (k = cell_divisions, R = compact support 2h).  For
+// a target at x_a the candidate cells are enumerated as x-rows ("pencils") of cells: for
+// every (y, z) cell pair the transverse distance d_t from x_a to the pencil (cell interval
+// grown by half the skin) prunes the pencil when d_t > R, and otherwise only the x-cells
+// within sqrt(R^2 - d_t^2) + skin/2 of x_a are visited -- one contiguous range of the
+// cell-sorted arrays.  Border cells extend to infinity outwards, so clamped particles
+// (outside the grid box) are always visited.  Every candidate then passes the exact test
+// d^2 = (dx*dx + dy*dy) + dz*dz <= R^2 (FP contraction off, the oracle's evaluation order),
+// so the neighbour relation is exactly the reference's.
 //
-// The radius test evaluates d^2 = (dx*dx + dy*dy) + dz*dz with FP contraction off, the
-// order the oracle uses, so neighbour sets agree bit for bit.
+// Pair evaluation.  k_fluid is two-phase per wavefront: phase 1 runs the exact radius test
+// over the candidates and appends hits to a per-lane list in LDS (kCap entries per lane);
+// phase 2 evaluates the pair terms on the dense lists.  Phases alternate until every lane
+// has exhausted its pencils, so any neighbour count is handled.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
-#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
@@ -31,9 +41,12 @@
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;  // 4 wavefronts of 64
+constexpr int kCap = 32;     // neighbour-list entries per lane and phase
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
+constexpr int kMaxGridBlocks = 4096;  // grid-stride kernels over the cell grid
+constexpr int kPad = 4;               // NaN entries after the SoA coordinate arrays
 
 thread_local std::string g_create_error;
 
@@ -41,49 +54,63 @@ struct DevParams {
     int nd;
     int nf, nw;
     int use_visc, use_dd, clip_fluid, clip_wall;
-    double h, h_inv, R2, two_h, almostzero;
+    double h, h_inv, R, R2, two_h, az2;
     double kval_c;   // kernel normalisation sigma * h^-d
-    double kgrad_c;  // -5 * sigma * h^-d * h^-2
+    double kgrad_c;  // -5 sigma h^-d h^-2: grad W = kgrad_c (1 - q/2)^3 r_ab
     double rho0, gamma, inv_gamma, p_bg, B, c;
-    double alpha, beta, eps_h2;  // epsilon * h^2
-    double dd_coef;              // delta * h * c
+    double alpha, beta, eps_h2;  // epsilon h^2
+    double av_c;                 // alpha c h
+    double dd2;                  // 2 delta h c (0 without density diffusion)
     double g[3];
     double p_offset;
+    // grid
     double lo[3];
-    double inv_cell;
+    double cs[3], inv_cs[3];  // cell edge per axis
     int n[3];
+    int ncells;
+    double hs_fluid;  // half skin + rounding margin (fluid cells)
+    double hs_wall;   // rounding margin (static wall cells)
+    double half_skin;
+};
+
+// Cell-sorted fluid data.  rec[2s], rec[2s+1] = {x, y, z, rho}, {vx, vy, vz, p} (one 64-byte
+// line per particle, read by the pair phase), vol[s] = {m/rho, m}; x/y/z SoA for the radius
+// test.
+struct FluidView {
+    const int *start;
+    const int *perm;
+    const double *x, *y, *z;
+    const double4 *rec;
+    const double2 *vol;
+};
+
+struct WallView {
+    const int *start;
+    const double *x, *y, *z;
+    const double4 *pos;  // {x, y, z, m}
+    const double2 *pv;   // {p_w, m/rho_w} of the current kick
 };
 
 // ------------------------------------------------------------------------------------------
 // device helpers
 // ------------------------------------------------------------------------------------------
-
 // Julia's max(x, 0) for Float64: NaN propagates, -0.0 becomes +0.0.
 __device__ __forceinline__ double julia_max0(double x) { return (x != x) ? x : (x > 0.0 ? x : 0.0); }
 
-// Squared distance in the shared evaluation order, no contraction.
 template <int ND>
-__device__ __forceinline__ double dist2(const double *pd)
+__device__ __forceinline__ double dist2(double a, double b, double c)
 {
 #pragma clang fp contract(off)
-    double s = pd[0] * pd[0] + pd[1] * pd[1];
-    if constexpr (ND == 3) s = s + pd[2] * pd[2];
+    double s = a * a + b * b;
+    if constexpr (ND == 3) s = s + c * c;
     return s;
 }
 
 __device__ __forceinline__ int cell_1d(double x, double lo, double inv, int n)
 {
     double f = floor((x - lo) * inv);
-    f = fmin(fmax(f, 0.0), (double)(n - 1));  // fmax/fmin also map NaN into the grid
+    f = fmin(fmax(f, 0.0), (double)(n - 1));  // also maps NaN into the grid
     return (int)f;
-}
-
-template <int ND>
-__device__ __forceinline__ void cell_of(const DevParams &P, const double *x, int *c)
-{
-    c[0] = cell_1d(x[0], P.lo[0], P.inv_cell, P.n[0]);
-    c[1] = cell_1d(x[1], P.lo[1], P.inv_cell, P.n[1]);
-    c[2] = (ND == 3) ? cell_1d(x[2], P.lo[2], P.inv_cell, P.n[2]) : 0;
 }
 
 __device__ __forceinline__ double state_equation(const DevParams &P, double rho)
@@ -101,43 +128,114 @@ __device__ __forceinline__ double wendland_c2(const DevParams &P, double r)
     return r < P.two_h ? P.kval_c * (w2 * w2) * (2.0 * q + 1.0) : 0.0;
 }
 
-// Iterate over the candidate ranges of the 3^ND cells around cell c: one contiguous range
-// of the cell-sorted arrays per (y, z) row of 3 cells.
-template <int ND, typename F>
-__device__ __forceinline__ void foreach_row(const DevParams &P, const int *c, const int *start,
-                                            F &&f)
+// Distance from coordinate y to the interval of cell c along axis d, grown by hs; border
+// cells reach to infinity outwards.
+__device__ __forceinline__ double transverse(const DevParams &P, int d, int c, double y, double hs)
 {
-    const int x0 = max(c[0] - 1, 0), x1 = min(c[0] + 1, P.n[0] - 1);
-    const int zlo = (ND == 3) ? max(c[2] - 1, 0) : 0;
-    const int zhi = (ND == 3) ? min(c[2] + 1, P.n[2] - 1) : 0;
-    const int ylo = max(c[1] - 1, 0), yhi = min(c[1] + 1, P.n[1] - 1);
-    for (int z = zlo; z <= zhi; z++)
-        for (int y = ylo; y <= yhi; y++) {
-            const int row = P.n[0] * (y + P.n[1] * z);
-            f(start[row + x0], start[row + x1 + 1]);
+    double lower = (c == 0) ? -INFINITY : P.lo[d] + c * P.cs[d] - hs;
+    double upper = (c == P.n[d] - 1) ? INFINITY : P.lo[d] + (c + 1) * P.cs[d] + hs;
+    return fmax(0.0, fmax(lower - y, y - upper));
+}
+
+// Byte-offset loads: base pointer in SGPRs, 32-bit offset in a VGPR (global_load saddr form).
+template <typename T>
+__device__ __forceinline__ T ldo(const T *__restrict__ base, uint32_t byte_off)
+{
+    return *(const T *)((const char *)base + byte_off);
+}
+
+// Enumerates the candidate ranges (pencils) around one point, z-outer, y-inner.
+template <int ND>
+struct Pencils {
+    double xa, ya, za, hs, dz2;
+    int y0, y1, z1, iy, iz, rowz;
+
+    __device__ __forceinline__ void init(const DevParams &P, double x, double y, double z,
+                                         double hs_)
+    {
+        xa = x;
+        ya = y;
+        za = z;
+        hs = hs_;
+        dz2 = 0.0;
+        rowz = 0;
+        const double reach = P.R + hs;
+        y0 = cell_1d(ya - reach, P.lo[1], P.inv_cs[1], P.n[1]);
+        y1 = cell_1d(ya + reach, P.lo[1], P.inv_cs[1], P.n[1]);
+        if constexpr (ND == 3) {
+            iz = cell_1d(za - reach, P.lo[2], P.inv_cs[2], P.n[2]) - 1;
+            z1 = cell_1d(za + reach, P.lo[2], P.inv_cs[2], P.n[2]);
+        } else {
+            iz = -1;
+            z1 = 0;
         }
+        iy = y1 + 1;  // the first call advances to the first z plane
+    }
+
+    // next non-empty candidate range [jb, je); false when exhausted
+    __device__ __forceinline__ bool next(const DevParams &P, const int *__restrict__ start,
+                                         int &jb, int &je)
+    {
+        while (true) {
+            if (iy > y1) {
+                if (++iz > z1) return false;
+                iy = y0;
+                if constexpr (ND == 3) {
+                    const double dz = transverse(P, 2, iz, za, hs);
+                    dz2 = dz * dz;
+                }
+                rowz = P.n[0] * P.n[1] * iz;
+            }
+            const int cy = iy++;
+            const double dy = transverse(P, 1, cy, ya, hs);
+            const double r2 = P.R2 - fma(dy, dy, dz2);
+            if (r2 < 0.0) continue;
+            // conservative half-width: FP32 sqrt, enlarged well past its rounding error
+            const double rx = (double)(__builtin_amdgcn_sqrtf((float)r2) * 1.000001f) + hs;
+            const int x0 = cell_1d(xa - rx, P.lo[0], P.inv_cs[0], P.n[0]);
+            const int x1 = cell_1d(xa + rx, P.lo[0], P.inv_cs[0], P.n[0]);
+            const uint32_t row = (uint32_t)(rowz + P.n[0] * cy);
+            jb = ldo(start, (row + (uint32_t)x0) * 4u);
+            je = ldo(start, (row + (uint32_t)x1 + 1u) * 4u);
+            if (jb < je) return true;
+        }
+    }
+};
+
+// Map block b to a tile so that each XCD (blocks are dealt round-robin over 8 XCDs) works
+// on one contiguous part of the cell-sorted particles and its L2 keeps the neighbours.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    constexpr int kXcd = 8;
+    const int per = nb / kXcd, extra = nb % kXcd;
+    const int x = b % kXcd, k = b / kXcd;
+    return (x < extra) ? x * (per + 1) + k : extra * (per + 1) + (x - extra) * per + k;
 }
 
 // ------------------------------------------------------------------------------------------
-// cell list (counting sort)
+// cell list (counting sort), every kernel gated by the rebuild flag (nullptr = always)
 // ------------------------------------------------------------------------------------------
-__global__ void k_fill_i32(int *p, int value, int n)
+__device__ __forceinline__ bool skip(const int *flag) { return flag && *flag == 0; }
+
+__global__ void k_fill_i32(const int *flag, int *p, int value, int n)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = value;
+    if (skip(flag)) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = value;
 }
 
 template <int ND>
-__global__ void k_cells(DevParams P, int n, const double *__restrict__ x, int stride,
+__global__ void k_cells(const int *flag, DevParams P, int n, const double *__restrict__ x,
                         int *__restrict__ cell, int *__restrict__ rank, int *__restrict__ count)
 {
+    if (skip(flag)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double xi[3] = {0.0, 0.0, 0.0};
-    for (int d = 0; d < ND; d++) xi[d] = x[(size_t)i * stride + d];
-    int c[3];
-    cell_of<ND>(P, xi, c);
-    int lin = c[0] + P.n[0] * (c[1] + P.n[1] * c[2]);
+    const double *xi = x + (size_t)i * ND;
+    int c0 = cell_1d(xi[0], P.lo[0], P.inv_cs[0], P.n[0]);
+    int c1 = cell_1d(xi[1], P.lo[1], P.inv_cs[1], P.n[1]);
+    int c2 = (ND == 3) ? cell_1d(xi[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+    int lin = c0 + P.n[0] * (c1 + P.n[1] * c2);
     cell[i] = lin;
     rank[i] = atomicAdd(&count[lin], 1);
 }
@@ -158,9 +256,10 @@ __device__ __forceinline__ int block_exclusive_scan(int value, int *s, int &tota
     return excl;
 }
 
-// Exclusive scan of one tile of kScanTile elements per block; tile totals to tile_sums.
-__global__ void __launch_bounds__(kBlock) k_scan_tiles(const int *in, int *out, int *tile_sums, int n)
+__global__ void __launch_bounds__(kBlock)
+    k_scan_tiles(const int *flag, const int *in, int *out, int *tile_sums, int n)
 {
+    if (skip(flag)) return;
     __shared__ int s[kBlock];
     int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
@@ -180,9 +279,9 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(const int *in, int *out, 
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
 
-// In-place exclusive scan of n values by a single block.
-__global__ void __launch_bounds__(kBlock) k_scan_single(int *data, int n)
+__global__ void __launch_bounds__(kBlock) k_scan_single(const int *flag, int *data, int n)
 {
+    if (skip(flag)) return;
     __shared__ int s[kBlock];
     int carry = 0;
     for (int chunk = 0; chunk < n; chunk += kScanTile) {
@@ -205,206 +304,381 @@ __global__ void __launch_bounds__(kBlock) k_scan_single(int *data, int n)
     }
 }
 
-__global__ void k_scan_add(int *out, const int *tile_sums, int n)
+__global__ void k_scan_add(const int *flag, int *out, const int *tile_sums, int n)
 {
+    if (skip(flag)) return;
     int i = blockIdx.x * kScanTile + threadIdx.x;
     int add = tile_sums[blockIdx.x];
     for (int k = 0; k < kScanItems; k++, i += kBlock)
         if (i < n) out[i] += add;
 }
 
-__global__ void k_scatter(int n, const int *__restrict__ cell, const int *__restrict__ rank,
-                          const int *__restrict__ start, int *__restrict__ perm)
+__global__ void k_scatter(const int *flag, int n, const int *__restrict__ cell,
+                          const int *__restrict__ rank, const int *__restrict__ start,
+                          int *__restrict__ perm)
 {
+    if (skip(flag)) return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     perm[start[cell[i]] + rank[i]] = i;
 }
 
-// Order the members of every cell by particle index: the result equals a stable counting
-// sort, independent of the order in which the histogram atomics were served.
-__global__ void k_sort_cells(int ncells, const int *__restrict__ start, int *__restrict__ perm)
+// Order every cell's members by particle index: the result is the stable counting sort,
+// independent of the order in which the histogram atomics were served.
+__global__ void k_sort_cells(const int *flag, int ncells, const int *__restrict__ start,
+                             int *__restrict__ perm)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    int b = start[c], e = start[c + 1];
-    for (int k = b + 1; k < e; k++) {
-        int key = perm[k];
-        int j = k - 1;
-        while (j >= b && perm[j] > key) {
-            perm[j + 1] = perm[j];
-            j--;
+    if (skip(flag)) return;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+        int b = start[c], e = start[c + 1];
+        for (int k = b + 1; k < e; k++) {
+            int key = perm[k];
+            int j = k - 1;
+            while (j >= b && perm[j] > key) {
+                perm[j + 1] = perm[j];
+                j--;
+            }
+            perm[j + 1] = key;
         }
-        perm[j + 1] = key;
     }
 }
 
+__global__ void k_set_i32(int *p, int value) { *p = value; }
+
 // ------------------------------------------------------------------------------------------
-// gather into cell-sorted SoA + fluid state equation (wcsph/system.jl:426-438)
+// gather into cell-sorted order + fluid state equation (wcsph/system.jl:426-438)
+//   MODE 0: runs when no rebuild is pending; flags one if a particle left its skin.  A block
+//           that starts after another block raised the flag returns early: the rebuild that
+//           follows gathers everything again.
+//   MODE 1: runs after a rebuild; stores the build positions.
 // ------------------------------------------------------------------------------------------
-template <int ND>
-__global__ void k_gather(DevParams P, const int *__restrict__ perm, const double *__restrict__ u,
-                         const double *__restrict__ v, const double *__restrict__ m,
-                         double *__restrict__ xs, double *__restrict__ vs,
-                         double *__restrict__ rhos, double *__restrict__ ps,
-                         double *__restrict__ ms)
+template <int ND, int MODE>
+__global__ void k_gather(int *flag, DevParams P, const int *__restrict__ perm,
+                         const double *__restrict__ u, const double *__restrict__ v,
+                         const double *__restrict__ m, double *__restrict__ xs,
+                         double *__restrict__ ys, double *__restrict__ zs,
+                         double4 *__restrict__ rec, double2 *__restrict__ vol,
+                         double *__restrict__ snap)
 {
+    const int f = *flag;
+    if ((MODE == 0) == (f != 0)) return;
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= P.nf) return;
-    int i = perm[s];
     const size_t nf = (size_t)P.nf;
+    const int i = perm[s];
+    double x[3] = {0.0, 0.0, 0.0}, vv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int d = 0; d < ND; d++) {
-        xs[d * nf + s] = u[(size_t)i * ND + d];
-        vs[d * nf + s] = v[(size_t)i * (ND + 1) + d];
+        x[d] = u[(size_t)i * ND + d];
+        vv[d] = v[(size_t)i * (ND + 1) + d];
     }
-    double rho = v[(size_t)i * (ND + 1) + ND];
-    rhos[s] = rho;
-    ps[s] = state_equation(P, rho);
-    ms[s] = m[i];
+    const double rho = v[(size_t)i * (ND + 1) + ND];
+    const double mass = m[i];
+    xs[s] = x[0];
+    ys[s] = x[1];
+    if constexpr (ND == 3) zs[s] = x[2];
+    rec[2 * (size_t)s] = make_double4(x[0], x[1], x[2], rho);
+    rec[2 * (size_t)s + 1] = make_double4(vv[0], vv[1], vv[2], state_equation(P, rho));
+    vol[s] = make_double2(mass / rho, mass);
+    if constexpr (MODE == 0) {
+        bool moved = false;
+#pragma unroll
+        for (int d = 0; d < ND; d++) moved |= !(fabs(x[d] - snap[d * nf + s]) <= P.half_skin);
+        if (moved) atomicOr(flag, 1);
+    } else {
+#pragma unroll
+        for (int d = 0; d < ND; d++) snap[d * nf + s] = x[d];
+    }
 }
 
 // ------------------------------------------------------------------------------------------
-// Adami pressure extrapolation (dummy_particles.jl:489-569), one thread per wall particle in
-// wall-cell order, over the fluid neighbours within 2h.
+// Adami pressure extrapolation (dummy_particles.jl:489-569): one thread per wall particle in
+// wall-cell order over its fluid neighbours within 2h, then the inverse state equation.
 // ------------------------------------------------------------------------------------------
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
-    k_wall(DevParams P, const double *__restrict__ wxs, const int *__restrict__ fstart,
-           const double *__restrict__ xs, const double *__restrict__ rhos,
-           const double *__restrict__ ps, double *__restrict__ pw, double *__restrict__ rhow)
+    k_wall(DevParams P, FluidView F, const double4 *__restrict__ wpos, double *__restrict__ pw,
+           double *__restrict__ rhow, double2 *__restrict__ wpv)
 {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= P.nw) return;
-    const size_t nw = (size_t)P.nw, nf = (size_t)P.nf;
-    double xw[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int d = 0; d < ND; d++) xw[d] = wxs[d * nw + w];
-    int c[3];
-    cell_of<ND>(P, xw, c);
+    const double4 xw = wpos[w];
+    Pencils<ND> it;
+    it.init(P, xw.x, xw.y, xw.z, P.hs_fluid);
     double pressure = 0.0, volume = 0.0;
-    foreach_row<ND>(P, c, fstart, [&](int jb, int je) {
+    int jb, je;
+    while (it.next(P, F.start, jb, je)) {
         for (int j = jb; j < je; j++) {
-            double pd[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int d = 0; d < ND; d++) pd[d] = xw[d] - xs[d * nf + j];
-            double d2 = dist2<ND>(pd);
+            const double p0 = xw.x - F.x[j], p1 = xw.y - F.y[j];
+            const double p2 = (ND == 3) ? xw.z - F.z[j] : 0.0;
+            const double d2 = dist2<ND>(p0, p1, p2);
             if (d2 > P.R2) continue;
-            double r = sqrt(d2);
-            double W = wendland_c2(P, r);
-            double rho_f = rhos[j];
-            double hydro = P.g[0] * (rho_f * pd[0]) + P.g[1] * (rho_f * pd[1]);
-            if constexpr (ND == 3) hydro += P.g[2] * (rho_f * pd[2]);
-            double sum_pressures = (P.p_offset + ps[j]) + hydro;
-            pressure += sum_pressures * W;
+            const double W = wendland_c2(P, sqrt(d2));
+            const double rho_f = F.rec[2 * (size_t)j].w;
+            const double p_f = F.rec[2 * (size_t)j + 1].w;
+            double hydro = P.g[0] * (rho_f * p0) + P.g[1] * (rho_f * p1);
+            if constexpr (ND == 3) hydro += P.g[2] * (rho_f * p2);
+            pressure += ((P.p_offset + p_f) + hydro) * W;
             volume += W;
         }
-    });
+    }
     if (volume > DBL_EPSILON) pressure /= volume;
     if (P.clip_wall) pressure = julia_max0(pressure);
+    const double rho = P.rho0 * pow((pressure - P.p_bg) / P.B + 1.0, P.inv_gamma);
     pw[w] = pressure;
-    rhow[w] = P.rho0 * pow((pressure - P.p_bg) / P.B + 1.0, P.inv_gamma);
+    rhow[w] = rho;
+    wpv[w] = make_double2(pressure, xw.w / rho);
 }
 
 // ------------------------------------------------------------------------------------------
-// Fused WCSPH interaction (wcsph/rhs.jl:5-127) for fluid-fluid and fluid-wall neighbours,
-// plus gravity (semidiscretization.jl:720-728).  One thread per fluid particle in
-// cell-sorted order; results go to the caller's order through perm.
+// Fused WCSPH interaction (wcsph/rhs.jl:5-127) for fluid-fluid and fluid-wall neighbours plus
+// gravity (semidiscretization.jl:720-728), one lane per fluid particle in cell-sorted order.
 //
-// Per fluid-fluid pair with grad W = kdr * r_ab, kdr = -5 sigma h^-2 (1 - q/2)^3:
-//   dv   += -m_b (p_a + p_b)/(rho_a rho_b) grad W                    (pressure, continuity form)
-//   dv   += (m_b alpha c mu + m_b beta mu^2)/rho_mean grad W, if v_ab.r_ab < 0   (Monaghan AV)
-//   drho += rho_a/rho_b m_b (v_ab . grad W)                           (continuity)
-//   drho += delta h c (m_b/rho_b) psi . grad W, psi = 2(rho_a-rho_b) r_ab/|r_ab|^2 (Molteni)
-// psi . grad W = 2 (rho_a - rho_b) kdr (|r_ab|^2/|r_ab|^2), so the diffusion term needs no
-// division.  Wall neighbours: v_b = 0, p_b / rho_b from the Adami pass, no viscosity (the
-// wall model's viscosity is `nothing`) and no density diffusion.
+// With grad W = kdr r_ab, kdr = -5 sigma h^-2 (1 - q/2)^3, V_b = m_b / rho_b:
+//   pressure   -m_b (p_a + p_b)/(rho_a rho_b) grad W = -V_b (p_a + p_b)/rho_a grad W
+//   viscosity  (m_b alpha c mu + m_b beta mu^2)/rho_mean grad W  when v_ab.r_ab < 0,
+//              mu = h v_ab.r_ab / (r^2 + eps h^2)                     (fluid neighbours)
+//   continuity rho_a/rho_b m_b v_ab.grad W = rho_a V_b kdr v_ab.r_ab
+//   diffusion  delta h c V_b psi.grad W = 2 delta h c V_b (rho_a - rho_b) kdr
+//              (psi.grad W = 2 (rho_a - rho_b) kdr r_ab.r_ab / r^2)   (fluid neighbours)
+// Wall neighbours: v_b = 0, p_b and rho_b from the Adami pass, no viscosity (the wall model's
+// viscosity is `nothing`) and no density diffusion.  Per particle the fluid sum, the wall sum
+// and gravity are added in this order, as the reference adds them into dv.
+//
+// sqrt and the viscosity division use the hardware rsq / rcp refined by Goldschmidt and
+// Newton-Raphson steps (the refinement LLVM emits for IEEE f64 sqrt and division, without
+// the range scaling: arguments here are normal and positive); results agree with the
+// correctly rounded operations to an ulp or so.
 // ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double sqrt_pos(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, hh = 0.5 * y;
+    const double r = fma(-hh, g, 0.5);
+    g = fma(g, r, g);
+    hh = fma(hh, r, hh);
+    double d = fma(-g, g, x);
+    g = fma(d, hh, g);
+    d = fma(-g, g, x);
+    return fma(d, hh, g);
+}
+
+__device__ __forceinline__ double rcp_pos(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+struct FluidNbr {
+    double4 a, b;  // {x, y, z, rho}, {vx, vy, vz, p}
+    double2 v;     // {V, m}
+};
+
+struct WallNbr {
+    double4 a;  // {x, y, z, m}
+    double2 pv; // {p, V}
+};
+
+__device__ __forceinline__ FluidNbr load_fluid(const FluidView &F, int j)
+{
+    const uint32_t o = (uint32_t)j;
+    return {ldo(F.rec, o * 64u), ldo(F.rec, o * 64u + 32u), ldo(F.vol, o * 16u)};
+}
+
+__device__ __forceinline__ WallNbr load_wall(const WallView &Wv, int j)
+{
+    const uint32_t o = (uint32_t)j;
+    return {ldo(Wv.pos, o * 32u), ldo(Wv.pv, o * 16u)};
+}
+
+template <int ND>
+__device__ __forceinline__ void pair_fluid(const DevParams &P, const FluidNbr &n,
+                                           const double *xa, const double *va, double rho_a,
+                                           double p_a, double inv_rho_a, double *acc,
+                                           double &drho)
+{
+    const double p0 = xa[0] - n.a.x, p1 = xa[1] - n.a.y, p2 = (ND == 3) ? xa[2] - n.a.z : 0.0;
+    const double d2 = dist2<ND>(p0, p1, p2);
+    const double r = sqrt_pos(d2);
+    const double w = fma(-0.5 * P.h_inv, r, 1.0);
+    const double kdr = P.kgrad_c * (w * w * w);
+    const double v0 = va[0] - n.b.x, v1 = va[1] - n.b.y, v2 = (ND == 3) ? va[2] - n.b.z : 0.0;
+    double vr = v0 * p0 + v1 * p1;
+    if constexpr (ND == 3) vr += v2 * p2;
+    const double rho_b = n.a.w;
+    double fac = -n.v.x * inv_rho_a * (p_a + n.b.w);
+    if (P.use_visc && vr < 0.0) {
+        // (m_b alpha c mu + m_b beta mu^2) / rho_mean, rho_mean = (rho_a + rho_b) / 2
+        const double q = d2 + P.eps_h2;
+        const double sum_rho = rho_a + rho_b;
+        fac = fma(2.0 * P.av_c * n.v.y * vr, rcp_pos(q * sum_rho), fac);
+        if (P.beta != 0.0) {
+            const double mu = P.h * vr * rcp_pos(q);
+            fac += 2.0 * n.v.y * P.beta * (mu * mu) * rcp_pos(sum_rho);
+        }
+    }
+    fac *= kdr;
+    acc[0] = fma(fac, p0, acc[0]);
+    acc[1] = fma(fac, p1, acc[1]);
+    if constexpr (ND == 3) acc[2] = fma(fac, p2, acc[2]);
+    drho = fma(n.v.x * kdr, fma(P.dd2, rho_a - rho_b, rho_a * vr), drho);
+}
+
+template <int ND>
+__device__ __forceinline__ void pair_wall(const DevParams &P, const WallNbr &n,
+                                          const double *xa, const double *va, double rho_a,
+                                          double p_a, double inv_rho_a, double *acc,
+                                          double &drho)
+{
+    const double p0 = xa[0] - n.a.x, p1 = xa[1] - n.a.y, p2 = (ND == 3) ? xa[2] - n.a.z : 0.0;
+    const double d2 = dist2<ND>(p0, p1, p2);
+    const double r = sqrt_pos(d2);
+    const double w = fma(-0.5 * P.h_inv, r, 1.0);
+    const double kdr = P.kgrad_c * (w * w * w);
+    double vr = va[0] * p0 + va[1] * p1;
+    if constexpr (ND == 3) vr += va[2] * p2;
+    const double fac = -n.pv.y * inv_rho_a * (p_a + n.pv.x) * kdr;
+    acc[0] = fma(fac, p0, acc[0]);
+    acc[1] = fma(fac, p1, acc[1]);
+    if constexpr (ND == 3) acc[2] = fma(fac, p2, acc[2]);
+    drho = fma(n.pv.y * kdr, rho_a * vr, drho);
+}
+
+// Two-phase sweep over the neighbours of one target (see the file header).  lst points at
+// this lane's column of the block's LDS lists (stride kBlock).  Phase 1 tests four
+// candidates per iteration without branches: the arrays cx/cy/cz carry kPad NaN entries
+// past their end, so reads beyond a range stay in bounds and never pass the test, and every
+// candidate is written to the list while the count only advances on a hit.
+template <int ND, bool WALL>
+__device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *__restrict__ start,
+                                                const double *__restrict__ cx,
+                                                const double *__restrict__ cy,
+                                                const double *__restrict__ cz, double hs,
+                                                bool active, const FluidView &F,
+                                                const WallView &Wv, int *lst, const double *xa,
+                                                const double *va, double rho_a, double p_a,
+                                                double inv_rho_a, double *acc, double &drho)
+{
+    Pencils<ND> it;
+    it.init(P, xa[0], xa[1], (ND == 3) ? xa[2] : 0.0, hs);
+    bool done = !active;
+    int j = 0, je = 0;
+    while (true) {
+        int cnt = 0;
+        // phase 1: exact radius test
+        while (!done && cnt <= kCap - 4) {
+            if (j >= je) {
+                if (!it.next(P, start, j, je)) done = true;
+                continue;
+            }
+            const uint32_t o = (uint32_t)j * 8u;
+            double d2[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double p0 = xa[0] - ldo(cx, o + 8u * k);
+                const double p1 = xa[1] - ldo(cy, o + 8u * k);
+                const double p2 = (ND == 3) ? xa[2] - ldo(cz, o + 8u * k) : 0.0;
+                d2[k] = dist2<ND>(p0, p1, p2);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool hit = (j + k < je) & (d2[k] <= P.R2) & (d2[k] >= P.az2);
+                lst[cnt * kBlock] = j + k;
+                cnt += hit ? 1 : 0;
+            }
+            j += 4;
+        }
+        // phase 2: pair terms on the dense list, loads one pair ahead
+        if (cnt > 0) {
+            if constexpr (WALL) {
+                WallNbr cur = load_wall(Wv, lst[0]);
+                for (int k = 0; k < cnt; k++) {
+                    const WallNbr nxt = load_wall(Wv, lst[min(k + 1, cnt - 1) * kBlock]);
+                    pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                    cur = nxt;
+                }
+            } else {
+                FluidNbr cur = load_fluid(F, lst[0]);
+                for (int k = 0; k < cnt; k++) {
+                    const FluidNbr nxt = load_fluid(F, lst[min(k + 1, cnt - 1) * kBlock]);
+                    pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                    cur = nxt;
+                }
+            }
+        }
+        if (!__any(!done)) break;
+    }
+}
+
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
-    k_fluid(DevParams P, const int *__restrict__ perm, const int *__restrict__ fstart,
-            const double *__restrict__ xs, const double *__restrict__ vs,
-            const double *__restrict__ rhos, const double *__restrict__ ps,
-            const double *__restrict__ ms, const int *__restrict__ wstart,
-            const double *__restrict__ wxs, const double *__restrict__ wms,
-            const double *__restrict__ pw, const double *__restrict__ rhow,
+    k_fluid(DevParams P, FluidView F, WallView Wv, const unsigned char *__restrict__ wall_near,
             double *__restrict__ dv)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P.nf) return;
-    const size_t nf = (size_t)P.nf, nw = (size_t)P.nw;
-    double xa[3] = {0.0, 0.0, 0.0}, va[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int d = 0; d < ND; d++) {
-        xa[d] = xs[d * nf + s];
-        va[d] = vs[d * nf + s];
+    __shared__ int lists[kCap * kBlock];
+    int *lst = lists + threadIdx.x;
+    const int s = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool active = s < P.nf;
+    double xa[3] = {0.0, 0.0, 0.0}, va[3] = {0.0, 0.0, 0.0}, rho_a = 1.0, p_a = 0.0;
+    if (active) {
+        const double4 a = F.rec[2 * (size_t)s];
+        const double4 b = F.rec[2 * (size_t)s + 1];
+        xa[0] = a.x;
+        xa[1] = a.y;
+        xa[2] = a.z;
+        rho_a = a.w;
+        va[0] = b.x;
+        va[1] = b.y;
+        va[2] = b.z;
+        p_a = b.w;
     }
-    const double rho_a = rhos[s], p_a = ps[s];
-    int c[3];
-    cell_of<ND>(P, xa, c);
+    const double inv_rho_a = 1.0 / rho_a;
 
     double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
-    foreach_row<ND>(P, c, fstart, [&](int jb, int je) {
-        for (int j = jb; j < je; j++) {
-            double pd[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int d = 0; d < ND; d++) pd[d] = xa[d] - xs[d * nf + j];
-            double d2 = dist2<ND>(pd);
-            if (d2 > P.R2) continue;
-            double r = sqrt(d2);
-            if (r < P.almostzero) continue;
-            double q = r * P.h_inv;
-            double w = 1.0 - q * 0.5;
-            double kdr = P.kgrad_c * (w * w * w);
-            double m_b = ms[j], rho_b = rhos[j], p_b = ps[j];
-            double vd[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int d = 0; d < ND; d++) vd[d] = va[d] - vs[d * nf + j];
-            double vr = vd[0] * pd[0] + vd[1] * pd[1];
-            if constexpr (ND == 3) vr += vd[2] * pd[2];
-            double fac = -m_b * ((p_a + p_b) / (rho_a * rho_b));
-            if (P.use_visc && vr < 0.0) {
-                double mu = (P.h * vr) / (d2 + P.eps_h2);
-                double rho_mean = (rho_a + rho_b) * 0.5;
-                fac += (m_b * P.alpha * P.c * mu + m_b * P.beta * (mu * mu)) / rho_mean;
-            }
-            fac *= kdr;
-#pragma unroll
-            for (int d = 0; d < ND; d++) acc[d] += fac * pd[d];
-            drho += (rho_a / rho_b) * m_b * (kdr * vr);
-            if (P.use_dd) drho += P.dd_coef * (m_b / rho_b) * (2.0 * (rho_a - rho_b) * kdr);
-        }
-    });
-
+    neighbour_sweep<ND, false>(P, F.start, F.x, F.y, F.z, P.hs_fluid, active, F, Wv, lst, xa,
+                               va, rho_a, p_a, inv_rho_a, acc, drho);
     double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
     if (P.nw > 0) {
-        foreach_row<ND>(P, c, wstart, [&](int jb, int je) {
-            for (int j = jb; j < je; j++) {
-                double pd[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-                for (int d = 0; d < ND; d++) pd[d] = xa[d] - wxs[d * nw + j];
-                double d2 = dist2<ND>(pd);
-                if (d2 > P.R2) continue;
-                double r = sqrt(d2);
-                if (r < P.almostzero) continue;
-                double q = r * P.h_inv;
-                double w = 1.0 - q * 0.5;
-                double kdr = P.kgrad_c * (w * w * w);
-                double m_b = wms[j], rho_b = rhow[j], p_b = pw[j];
-                double fac = -m_b * ((p_a + p_b) / (rho_a * rho_b)) * kdr;
-#pragma unroll
-                for (int d = 0; d < ND; d++) accw[d] += fac * pd[d];
-                double vr = va[0] * pd[0] + va[1] * pd[1];
-                if constexpr (ND == 3) vr += va[2] * pd[2];
-                drhow += (rho_a / rho_b) * m_b * (kdr * vr);
-            }
-        });
+        // walls are static: only cells flagged at create can have wall neighbours
+        bool near = false;
+        if (active) {
+            const int c0 = cell_1d(xa[0], P.lo[0], P.inv_cs[0], P.n[0]);
+            const int c1 = cell_1d(xa[1], P.lo[1], P.inv_cs[1], P.n[1]);
+            const int c2 = (ND == 3) ? cell_1d(xa[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+            near = wall_near[c0 + P.n[0] * (c1 + P.n[1] * c2)] != 0;
+        }
+        if (__any(near))
+            neighbour_sweep<ND, true>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, near, F, Wv,
+                                      lst, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
     }
-
-    const int i = perm[s];
+    if (!active) return;
+    const int i = F.perm[s];
 #pragma unroll
     for (int d = 0; d < ND; d++) dv[(size_t)i * (ND + 1) + d] = (acc[d] + accw[d]) + P.g[d];
     dv[(size_t)i * (ND + 1) + ND] = drho + drhow;
+}
+
+// Flag every cell that has a wall particle within R of some point of the cell (static).
+template <int ND>
+__global__ void k_wall_near(DevParams P, int nw, const double4 *__restrict__ wpos,
+                            unsigned char *__restrict__ near)
+{
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const double4 x = wpos[w];
+    const double reach = P.R + P.hs_wall;
+    const int x0 = cell_1d(x.x - reach, P.lo[0], P.inv_cs[0], P.n[0]);
+    const int x1 = cell_1d(x.x + reach, P.lo[0], P.inv_cs[0], P.n[0]);
+    const int y0 = cell_1d(x.y - reach, P.lo[1], P.inv_cs[1], P.n[1]);
+    const int y1 = cell_1d(x.y + reach, P.lo[1], P.inv_cs[1], P.n[1]);
+    const int z0 = (ND == 3) ? cell_1d(x.z - reach, P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+    const int z1 = (ND == 3) ? cell_1d(x.z + reach, P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+    for (int cz = z0; cz <= z1; cz++)
+        for (int cy = y0; cy <= y1; cy++)
+            for (int cx = x0; cx <= x1; cx++) near[cx + P.n[0] * (cy + P.n[1] * cz)] = 1;
 }
 
 // drift!: du = v (semidiscretization.jl:561-571; delta_v = 0 without shifting)
@@ -417,7 +691,7 @@ __global__ void k_drift(int nf, const double *__restrict__ v, double *__restrict
     for (int d = 0; d < ND; d++) du[(size_t)i * ND + d] = v[(size_t)i * (ND + 1) + d];
 }
 
-// One 2N-storage stage of the partition (v, u) with k = (dv, v): the drift slope is read
+// One 2N-storage stage of the partition (v, u) with slopes (dv, v): the drift slope is read
 // before v is updated, as the integrator evaluates kick! and drift! at the same stage state.
 template <int ND>
 __global__ void k_ck_stage(int nf, double *__restrict__ v, double *__restrict__ u,
@@ -456,57 +730,41 @@ __global__ void k_rk_stage(int64_t n, double *__restrict__ y, double *__restrict
 }
 
 // ------------------------------------------------------------------------------------------
-// neighbour CSR export (parity tests)
+// neighbour CSR export (parity tests): rows = cell-sorted points (rx, ry, rz; row_perm to the
+// caller's order), columns = cell-sorted points with cell offsets col_start
 // ------------------------------------------------------------------------------------------
-// rows: points xr (SoA, cell-sorted, row_perm maps to caller order), columns: cell-sorted
-// points xc with col_start cell offsets and col_perm to caller order.
-template <int ND>
-__global__ void k_nbr_count(DevParams P, int nrows, const double *__restrict__ xr,
-                            const int *__restrict__ row_perm, int ncols,
-                            const double *__restrict__ xc, const int *__restrict__ col_start,
-                            int *__restrict__ counts)
+template <int ND, bool FILL>
+__global__ void k_nbr(DevParams P, int nrows, const double *__restrict__ rx,
+                      const double *__restrict__ ry, const double *__restrict__ rz,
+                      const int *__restrict__ row_perm, const int *__restrict__ col_start,
+                      const double *__restrict__ cx, const double *__restrict__ cy,
+                      const double *__restrict__ cz, const int *__restrict__ col_perm,
+                      double hs, int *__restrict__ counts, const int64_t *__restrict__ offsets,
+                      int *__restrict__ idx)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nrows) return;
-    double xa[3] = {0.0, 0.0, 0.0};
-    for (int d = 0; d < ND; d++) xa[d] = xr[(size_t)d * nrows + s];
-    int c[3];
-    cell_of<ND>(P, xa, c);
-    int cnt = 0;
-    foreach_row<ND>(P, c, col_start, [&](int jb, int je) {
-        for (int j = jb; j < je; j++) {
-            double pd[3] = {0.0, 0.0, 0.0};
-            for (int d = 0; d < ND; d++) pd[d] = xa[d] - xc[(size_t)d * ncols + j];
-            if (dist2<ND>(pd) <= P.R2) cnt++;
-        }
-    });
-    counts[row_perm[s]] = cnt;
-}
-
-template <int ND>
-__global__ void k_nbr_fill(DevParams P, int nrows, const double *__restrict__ xr,
-                           const int *__restrict__ row_perm, int ncols,
-                           const double *__restrict__ xc, const int *__restrict__ col_start,
-                           const int *__restrict__ col_perm, const int64_t *__restrict__ offsets,
-                           int *__restrict__ idx)
-{
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nrows) return;
-    double xa[3] = {0.0, 0.0, 0.0};
-    for (int d = 0; d < ND; d++) xa[d] = xr[(size_t)d * nrows + s];
-    int c[3];
-    cell_of<ND>(P, xa, c);
+    const double xa = rx[s], ya = ry[s], za = (ND == 3) ? rz[s] : 0.0;
+    Pencils<ND> it;
+    it.init(P, xa, ya, za, hs);
     const int row = row_perm[s];
-    int64_t o = offsets[row];
+    int64_t o = FILL ? offsets[row] : 0;
     const int64_t b = o;
-    foreach_row<ND>(P, c, col_start, [&](int jb, int je) {
+    int cnt = 0, jb, je;
+    while (it.next(P, col_start, jb, je)) {
         for (int j = jb; j < je; j++) {
-            double pd[3] = {0.0, 0.0, 0.0};
-            for (int d = 0; d < ND; d++) pd[d] = xa[d] - xc[(size_t)d * ncols + j];
-            if (dist2<ND>(pd) <= P.R2) idx[o++] = col_perm[j];
+            const double d2 = dist2<ND>(xa - cx[j], ya - cy[j], (ND == 3) ? za - cz[j] : 0.0);
+            if (d2 <= P.R2) {
+                if (FILL) idx[o++] = col_perm[j];
+                cnt++;
+            }
         }
-    });
-    for (int64_t k = b + 1; k < o; k++) {  // ascending column order
+    }
+    if (!FILL) {
+        counts[row] = cnt;
+        return;
+    }
+    for (int64_t k = b + 1; k < o; k++) {  // ascending columns
         int key = idx[k];
         int64_t j = k - 1;
         while (j >= b && idx[j] > key) {
@@ -517,14 +775,22 @@ __global__ void k_nbr_fill(DevParams P, int nrows, const double *__restrict__ xr
     }
 }
 
+// cell-sorted SoA coordinates (and optionally {x, y, z, mass} records) of n points
 template <int ND>
-__global__ void k_aos_to_soa(int n, const double *__restrict__ x, double *__restrict__ xs,
-                             const int *__restrict__ perm)
+__global__ void k_positions(int n, const double *__restrict__ x, const int *__restrict__ perm,
+                            double *__restrict__ xs, double *__restrict__ ys,
+                            double *__restrict__ zs, double4 *__restrict__ pos,
+                            const double *__restrict__ mass)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    int i = perm[s];
-    for (int d = 0; d < ND; d++) xs[(size_t)d * n + s] = x[(size_t)i * ND + d];
+    const int i = perm[s];
+    const double a = x[(size_t)i * ND], b = x[(size_t)i * ND + 1];
+    const double c = (ND == 3) ? x[(size_t)i * ND + 2] : 0.0;
+    xs[s] = a;
+    ys[s] = b;
+    if (zs) zs[s] = c;
+    if (pos) pos[s] = make_double4(a, b, c, mass ? mass[i] : 0.0);
 }
 
 template <typename T>
@@ -536,7 +802,16 @@ __global__ void k_unpermute(int n, const int *__restrict__ perm, const T *__rest
     out[perm[s]] = sorted[s];
 }
 
+// fluid pressure of the cell-sorted records (p = .w of the second double4 of each record)
+__global__ void k_record_pressure(int n, const double4 *__restrict__ rec, double *__restrict__ p)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    p[s] = rec[2 * (size_t)s + 1].w;
+}
+
 inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+inline int grid_blocks(int64_t n) { return std::max(1, std::min(nblocks(n), kMaxGridBlocks)); }
 
 }  // namespace
 
@@ -548,25 +823,29 @@ struct tpsph_handle {
     DevParams P{};
     int nd = 0;
     int nf = 0, nw = 0;
-    int ncells = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
 
-    // fluid, per kick
+    int *flag = nullptr;  // rebuild of the fluid cell list pending
+    // fluid cell list + cell-sorted state
     int *f_cell = nullptr, *f_rank = nullptr, *f_count = nullptr, *f_start = nullptr,
         *f_perm = nullptr;
     int *scan_tiles = nullptr;
     double *f_mass = nullptr;  // caller order
-    double *xs = nullptr, *vs = nullptr, *rhos = nullptr, *ps = nullptr, *ms = nullptr;
-    // wall, static (cell-sorted) + per kick
+    double *xs = nullptr, *ys = nullptr, *zs = nullptr, *snap = nullptr;
+    double4 *rec = nullptr;
+    double2 *vol = nullptr;
+    // walls (static, cell-sorted) + per kick
     int *w_start = nullptr, *w_perm = nullptr;
-    double *wxs = nullptr, *wms = nullptr, *pw = nullptr, *rhow = nullptr;
+    double *wx = nullptr, *wy = nullptr, *wz = nullptr, *pw = nullptr, *rhow = nullptr;
+    double4 *wpos = nullptr;
+    double2 *wpv = nullptr;
+    unsigned char *wall_near = nullptr;  // per cell: a wall particle lies within reach
     // host-mode staging and solver scratch
     double *st_v = nullptr, *st_u = nullptr, *st_dv = nullptr, *st_du = nullptr;
     double *sol_dv = nullptr, *sol_tv = nullptr, *sol_tu = nullptr;
     bool have_kick = false;
 
-    // timing
     bool timing = false;
     struct Rec {
         int kind;
@@ -575,6 +854,9 @@ struct tpsph_handle {
     std::vector<hipEvent_t> event_pool;
     size_t events_used = 0;
     std::vector<Rec> records;
+
+    FluidView fluid_view() const { return {f_start, f_perm, xs, ys, zs, rec, vol}; }
+    WallView wall_view() const { return {w_start, wx, wy, wz, wpos, wpv}; }
 };
 
 namespace {
@@ -588,16 +870,16 @@ int fail(tpsph_handle *h, int code, const std::string &msg)
     return code;
 }
 
-#define HIP_TRY(h, expr)                                                                    \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
+#define HIP_TRY(h, expr)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
             return fail((h), TPSPH_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
     } while (0)
 
-#define TRY(expr)                  \
-    do {                           \
-        int rc_ = (expr);          \
+#define TRY(expr)                        \
+    do {                                 \
+        int rc_ = (expr);                \
         if (rc_ != TPSPH_OK) return rc_; \
     } while (0)
 
@@ -607,6 +889,13 @@ int dalloc(tpsph_handle *h, T **p, size_t count)
     *p = nullptr;
     if (count == 0) count = 1;
     HIP_TRY(h, hipMalloc((void **)p, count * sizeof(T)));
+    return TPSPH_OK;
+}
+
+// all bits set: a quiet NaN in every double
+int hip_fill(tpsph_handle *h, double *p, size_t count)
+{
+    HIP_TRY(h, hipMemset(p, 0xff, count * sizeof(double)));
     return TPSPH_OK;
 }
 
@@ -620,9 +909,8 @@ int launch_check(tpsph_handle *h, const char *what)
 
 struct TimedScope {
     tpsph_handle *h;
-    int kind;
     hipEvent_t b = nullptr;
-    TimedScope(tpsph_handle *h_, int kind_) : h(h_), kind(kind_)
+    TimedScope(tpsph_handle *h_, int kind) : h(h_)
     {
         if (!h->timing) return;
         if (h->events_used + 2 > h->event_pool.size()) {
@@ -643,59 +931,72 @@ struct TimedScope {
     }
 };
 
-// exclusive scan of n ints (in) into out, on the handle's stream
-int scan(tpsph_handle *h, const int *in, int *out, int n)
+int scan(tpsph_handle *h, const int *flag, const int *in, int *out, int n)
 {
     int tiles = (n + kScanTile - 1) / kScanTile;
-    k_scan_tiles<<<tiles, kBlock, 0, h->stream>>>(in, out, h->scan_tiles, n);
-    k_scan_single<<<1, kBlock, 0, h->stream>>>(h->scan_tiles, tiles);
-    k_scan_add<<<tiles, kBlock, 0, h->stream>>>(out, h->scan_tiles, n);
+    k_scan_tiles<<<tiles, kBlock, 0, h->stream>>>(flag, in, out, h->scan_tiles, n);
+    k_scan_single<<<1, kBlock, 0, h->stream>>>(flag, h->scan_tiles, tiles);
+    k_scan_add<<<tiles, kBlock, 0, h->stream>>>(flag, out, h->scan_tiles, n);
     return launch_check(h, "scan");
 }
 
-// Build the cell list of n points x (stride doubles per point) into start/perm.
+// Cell list of n points x (ND doubles per point, caller order) into start / perm.
 template <int ND>
-int build_cells(tpsph_handle *h, int n, const double *x, int stride, int *cell, int *rank,
+int build_cells(tpsph_handle *h, const int *flag, int n, const double *x, int *cell, int *rank,
                 int *count, int *start, int *perm)
 {
-    const int nc = h->ncells;
-    k_fill_i32<<<nblocks(nc + 1), kBlock, 0, h->stream>>>(count, 0, nc + 1);
-    if (n > 0) k_cells<ND><<<nblocks(n), kBlock, 0, h->stream>>>(h->P, n, x, stride, cell, rank, count);
+    const int nc = h->P.ncells;
+    k_fill_i32<<<grid_blocks(nc + 1), kBlock, 0, h->stream>>>(flag, count, 0, nc + 1);
+    if (n > 0)
+        k_cells<ND><<<nblocks(n), kBlock, 0, h->stream>>>(flag, h->P, n, x, cell, rank, count);
     TRY(launch_check(h, "k_cells"));
-    TRY(scan(h, count, start, nc + 1));
+    TRY(scan(h, flag, count, start, nc + 1));
     if (n > 0) {
-        k_scatter<<<nblocks(n), kBlock, 0, h->stream>>>(n, cell, rank, start, perm);
-        k_sort_cells<<<nblocks(nc), kBlock, 0, h->stream>>>(nc, start, perm);
+        k_scatter<<<nblocks(n), kBlock, 0, h->stream>>>(flag, n, cell, rank, start, perm);
+        k_sort_cells<<<grid_blocks(nc), kBlock, 0, h->stream>>>(flag, nc, start, perm);
     }
     return launch_check(h, "k_scatter/k_sort_cells");
+}
+
+// Fluid cell list (rebuilt only when flagged) + cell-sorted gather of (v, u).
+template <int ND>
+int update_fluid(tpsph_handle *h, const double *v, const double *u)
+{
+    const int nf = h->nf;
+    if (nf == 0) return TPSPH_OK;
+    {
+        TimedScope t(h, TPSPH_TIME_GATHER);
+        k_gather<ND, 0><<<nblocks(nf), kBlock, 0, h->stream>>>(
+            h->flag, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
+        TRY(launch_check(h, "k_gather<0>"));
+    }
+    {
+        TimedScope t(h, TPSPH_TIME_CELL_LIST);
+        TRY(build_cells<ND>(h, h->flag, nf, u, h->f_cell, h->f_rank, h->f_count, h->f_start,
+                            h->f_perm));
+        k_gather<ND, 1><<<nblocks(nf), kBlock, 0, h->stream>>>(
+            h->flag, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
+        k_set_i32<<<1, 1, 0, h->stream>>>(h->flag, 0);
+        TRY(launch_check(h, "k_gather<1>"));
+    }
+    return TPSPH_OK;
 }
 
 template <int ND>
 int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u)
 {
     const int nf = h->nf, nw = h->nw;
-    {
-        TimedScope t(h, TPSPH_TIME_CELL_LIST);
-        TRY(build_cells<ND>(h, nf, u, ND, h->f_cell, h->f_rank, h->f_count, h->f_start,
-                            h->f_perm));
-    }
-    if (nf > 0) {
-        TimedScope t(h, TPSPH_TIME_GATHER);
-        k_gather<ND><<<nblocks(nf), kBlock, 0, h->stream>>>(h->P, h->f_perm, u, v, h->f_mass,
-                                                             h->xs, h->vs, h->rhos, h->ps, h->ms);
-        TRY(launch_check(h, "k_gather"));
-    }
+    TRY(update_fluid<ND>(h, v, u));
     if (nw > 0) {
         TimedScope t(h, TPSPH_TIME_WALL);
-        k_wall<ND><<<nblocks(nw), kBlock, 0, h->stream>>>(h->P, h->wxs, h->f_start, h->xs,
-                                                           h->rhos, h->ps, h->pw, h->rhow);
+        k_wall<ND><<<nblocks(nw), kBlock, 0, h->stream>>>(h->P, h->fluid_view(), h->wpos, h->pw,
+                                                           h->rhow, h->wpv);
         TRY(launch_check(h, "k_wall"));
     }
     if (nf > 0) {
         TimedScope t(h, TPSPH_TIME_FLUID);
-        k_fluid<ND><<<nblocks(nf), kBlock, 0, h->stream>>>(
-            h->P, h->f_perm, h->f_start, h->xs, h->vs, h->rhos, h->ps, h->ms, h->w_start,
-            h->wxs, h->wms, h->pw, h->rhow, dv);
+        k_fluid<ND><<<nblocks(nf), kBlock, 0, h->stream>>>(h->P, h->fluid_view(), h->wall_view(),
+                                                             h->wall_near, dv);
         TRY(launch_check(h, "k_fluid"));
     }
     h->have_kick = true;
@@ -708,7 +1009,6 @@ int kick_dispatch(tpsph_handle *h, double *dv, const double *v, const double *u)
 }
 
 bool is_host(const tpsph_handle *h) { return h->params.memory == TPSPH_MEM_HOST; }
-
 size_t v_len(const tpsph_handle *h) { return (size_t)h->nf * (h->nd + 1); }
 size_t u_len(const tpsph_handle *h) { return (size_t)h->nf * h->nd; }
 
@@ -732,13 +1032,18 @@ int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, co
     if (p->memory != TPSPH_MEM_DEVICE && p->memory != TPSPH_MEM_HOST)
         return bad("memory must be TPSPH_MEM_DEVICE or TPSPH_MEM_HOST");
     if (nf < 0 || nw < 0) return bad("particle counts must be non-negative");
-    if (nf > (int64_t)INT32_MAX / 4 || nw > (int64_t)INT32_MAX / 4)
-        return bad("particle counts exceed the 32-bit index range of this build");
+    if (nf > (int64_t)1 << 26 || nw > (int64_t)1 << 26)
+        return bad("particle counts above 2^26 per system exceed the 32-bit byte offsets of "
+                   "this build");
     if (!(p->smoothing_length > 0.0)) return bad("`smoothing_length` must be positive");
     if (!(p->reference_density > DBL_EPSILON))
         return bad("`reference_density` needs to be positive and larger than eps()");
     if (!(p->sound_speed > 0.0)) return bad("`sound_speed` must be positive");
     if (!(p->exponent > 0.0)) return bad("`exponent` must be positive");
+    if (p->cell_divisions < 0 || p->cell_divisions > 4) return bad("cell_divisions must be in 0..4");
+    if (p->x_cell_divisions < 0 || p->x_cell_divisions > 8)
+        return bad("x_cell_divisions must be in 0..8");
+    if (!(p->skin < 1.0)) return bad("skin must be below 1 (a fraction of the compact support)");
     if (nf > 0 && !fm) return bad("fluid_mass is NULL");
     if (nw > 0 && (!wx || !wm)) return bad("wall_coordinates / wall_mass is NULL");
     return TPSPH_OK;
@@ -776,10 +1081,9 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     const bool host = is_host(h);
     const hipMemcpyKind in_kind = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
 
-    // wall coordinates to the host for the bounding box (the grid is sized at create)
-    std::vector<double> wx((size_t)nd * h->nw);
+    std::vector<double> wxh((size_t)nd * h->nw);
     if (h->nw > 0) {
-        hipError_t e = hipMemcpy(wx.data(), wall_coordinates, wx.size() * sizeof(double),
+        hipError_t e = hipMemcpy(wxh.data(), wall_coordinates, wxh.size() * sizeof(double),
                                  host ? hipMemcpyHostToHost : hipMemcpyDeviceToHost);
         if (e != hipSuccess) {
             std::string m = std::string("copy of wall_coordinates: ") + hipGetErrorString(e);
@@ -801,9 +1105,10 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     P.clip_wall = params->clip_wall_pressure ? 1 : 0;
     P.h = hl;
     P.h_inv = 1.0 / hl;
+    P.R = R;
     P.R2 = R * R;
     P.two_h = 2.0 * hl;
-    P.almostzero = std::sqrt(std::nextafter(R * R, INFINITY) - R * R);
+    P.az2 = std::nextafter(R * R, INFINITY) - R * R;  // sqrt(eps(R^2))^2
     const double sigma = nd == 2 ? (7.0 / (4.0 * M_PI)) * (P.h_inv * P.h_inv)
                                  : (21.0 / (16.0 * M_PI)) * (P.h_inv * P.h_inv * P.h_inv);
     P.kval_c = sigma;
@@ -817,7 +1122,8 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     P.alpha = params->alpha;
     P.beta = params->beta;
     P.eps_h2 = params->epsilon * (hl * hl);
-    P.dd_coef = params->delta * ((hl + hl) / 2.0) * P.c;
+    P.av_c = params->alpha * P.c * hl;
+    P.dd2 = params->use_density_diffusion ? 2.0 * params->delta * hl * P.c : 0.0;
     for (int d = 0; d < 3; d++) P.g[d] = d < nd ? params->acceleration[d] : 0.0;
     P.p_offset = params->pressure_offset;
 
@@ -832,38 +1138,49 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     }
     for (int i = 0; i < h->nw; i++)
         for (int d = 0; d < nd; d++) {
-            lo[d] = std::fmin(lo[d], wx[(size_t)i * nd + d]);
-            hi[d] = std::fmax(hi[d], wx[(size_t)i * nd + d]);
+            lo[d] = std::fmin(lo[d], wxh[(size_t)i * nd + d]);
+            hi[d] = std::fmax(hi[d], wxh[(size_t)i * nd + d]);
         }
     for (int d = 0; d < nd; d++)
         if (!(lo[d] <= hi[d])) {
             lo[d] = 0.0;
             hi[d] = 0.0;
         }
-    const double cell = R * (1.0 + std::ldexp(1.0, -30));
-    P.inv_cell = 1.0 / cell;
+    const int div = params->cell_divisions > 0 ? params->cell_divisions : 2;
+    const int xdiv = params->x_cell_divisions > 0 ? params->x_cell_divisions : 2 * div;
+    const double skin = params->skin >= 0.0 ? params->skin : 0.04;
     int64_t ncells = 1;
+    double extent = 0.0;
     for (int d = 0; d < 3; d++) {
+        const double cs = (R / (d == 0 ? xdiv : div)) * (1.0 + std::ldexp(1.0, -30));
+        P.cs[d] = cs;
+        P.inv_cs[d] = 1.0 / cs;
         if (d < nd) {
-            P.lo[d] = lo[d] - cell;
-            int64_t n = (int64_t)std::ceil((hi[d] + cell - P.lo[d]) * P.inv_cell) + 1;
+            P.lo[d] = lo[d] - cs;
+            int64_t n = (int64_t)std::ceil((hi[d] + cs - P.lo[d]) * P.inv_cs[d]) + 1;
             if (n < 1) n = 1;
             if (n > (1 << 20)) {
                 delete h;
                 return fail(nullptr, TPSPH_ERR_ARGUMENT, "domain box too large for the cell grid");
             }
             P.n[d] = (int)n;
+            extent = std::fmax(extent, std::fabs(P.lo[d]) + n * cs);
         } else {
             P.lo[d] = 0.0;
             P.n[d] = 1;
         }
         ncells *= P.n[d];
     }
-    if (ncells > (int64_t)INT32_MAX / 2) {
+    if (ncells > (int64_t)INT32_MAX / 4) {
         delete h;
         return fail(nullptr, TPSPH_ERR_ARGUMENT, "cell grid too large (domain / smoothing_length)");
     }
-    h->ncells = (int)ncells;
+    P.ncells = (int)ncells;
+    // cell-interval rounding margin: far above the error of floor((x - lo) / cs)
+    const double margin = 1e-9 * R + 64.0 * DBL_EPSILON * extent;
+    P.half_skin = 0.5 * skin * R;
+    P.hs_fluid = P.half_skin + margin;
+    P.hs_wall = margin;
 
     auto cleanup_fail = [&](int rc) {
         std::string m = h->last_error;
@@ -875,6 +1192,7 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     if ((rc = (expr)) != TPSPH_OK) return cleanup_fail(rc)
     const int nf = h->nf, nw = h->nw;
     const int tiles = (int)((ncells + 1 + kScanTile - 1) / kScanTile);
+    CTRY(dalloc(h, &h->flag, 1));
     CTRY(dalloc(h, &h->scan_tiles, (size_t)tiles + 1));
     CTRY(dalloc(h, &h->f_cell, nf));
     CTRY(dalloc(h, &h->f_rank, nf));
@@ -882,17 +1200,34 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     CTRY(dalloc(h, &h->f_count, (size_t)ncells + 1));
     CTRY(dalloc(h, &h->f_start, (size_t)ncells + 1));
     CTRY(dalloc(h, &h->f_mass, nf));
-    CTRY(dalloc(h, &h->xs, (size_t)nd * nf));
-    CTRY(dalloc(h, &h->vs, (size_t)nd * nf));
-    CTRY(dalloc(h, &h->rhos, nf));
-    CTRY(dalloc(h, &h->ps, nf));
-    CTRY(dalloc(h, &h->ms, nf));
+    CTRY(dalloc(h, &h->xs, (size_t)nf + kPad));
+    CTRY(dalloc(h, &h->ys, (size_t)nf + kPad));
+    CTRY(dalloc(h, &h->zs, (size_t)nf + kPad));
+    CTRY(dalloc(h, &h->snap, (size_t)nd * nf));
+    CTRY(dalloc(h, &h->rec, 2 * (size_t)nf));
+    CTRY(dalloc(h, &h->vol, nf));
     CTRY(dalloc(h, &h->w_start, (size_t)ncells + 1));
     CTRY(dalloc(h, &h->w_perm, nw));
-    CTRY(dalloc(h, &h->wxs, (size_t)nd * nw));
-    CTRY(dalloc(h, &h->wms, nw));
+    CTRY(dalloc(h, &h->wx, (size_t)nw + kPad));
+    CTRY(dalloc(h, &h->wy, (size_t)nw + kPad));
+    CTRY(dalloc(h, &h->wz, (size_t)nw + kPad));
+    CTRY(dalloc(h, &h->wall_near, (size_t)ncells));
+    // NaN (all bits set) in the coordinate arrays, including their padding; never zero so a
+    // padded entry can not pass the radius test
+    for (double *a : {h->xs, h->ys, h->zs})
+        CTRY(hip_fill(h, a, (size_t)nf + kPad));
+    for (double *a : {h->wx, h->wy, h->wz})
+        CTRY(hip_fill(h, a, (size_t)nw + kPad));
+    if (hipMemset(h->wall_near, 0, (size_t)ncells) != hipSuccess) {
+        h->last_error = "clearing wall_near failed";
+        return cleanup_fail(TPSPH_ERR_HIP);
+    }
+    CTRY(dalloc(h, &h->wpos, nw));
+    CTRY(dalloc(h, &h->wpv, nw));
     CTRY(dalloc(h, &h->pw, nw));
     CTRY(dalloc(h, &h->rhow, nw));
+    k_set_i32<<<1, 1, 0, h->stream>>>(h->flag, 1);  // the first kick builds the fluid cells
+    CTRY(launch_check(h, "k_set_i32"));
     if (nf > 0) {
         hipError_t e = hipMemcpy(h->f_mass, fluid_mass, (size_t)nf * sizeof(double), in_kind);
         if (e != hipSuccess) {
@@ -900,42 +1235,43 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
             return cleanup_fail(TPSPH_ERR_HIP);
         }
     }
-    // static wall cell list: sort walls by cell once, keep SoA coordinates and masses
+    // static wall cell list
     if (nw > 0) {
         double *wx_d = nullptr, *wm_d = nullptr;
         int *wcell = nullptr, *wrank = nullptr, *wcount = nullptr;
-        CTRY(dalloc(h, &wx_d, (size_t)nd * nw));
-        CTRY(dalloc(h, &wm_d, nw));
-        CTRY(dalloc(h, &wcell, nw));
-        CTRY(dalloc(h, &wrank, nw));
-        CTRY(dalloc(h, &wcount, (size_t)ncells + 1));
-        hipError_t e1 = hipMemcpy(wx_d, wx.data(), wx.size() * sizeof(double), hipMemcpyHostToDevice);
-        hipError_t e2 = hipMemcpy(wm_d, wall_mass, (size_t)nw * sizeof(double), in_kind);
-        if (e1 != hipSuccess || e2 != hipSuccess) {
-            h->last_error = "copy of wall data failed";
-            (void)hipFree(wx_d); (void)hipFree(wm_d); (void)hipFree(wcell); (void)hipFree(wrank); (void)hipFree(wcount);
-            return cleanup_fail(TPSPH_ERR_HIP);
+        rc = dalloc(h, &wx_d, (size_t)nd * nw);
+        if (rc == TPSPH_OK) rc = dalloc(h, &wm_d, nw);
+        if (rc == TPSPH_OK) rc = dalloc(h, &wcell, nw);
+        if (rc == TPSPH_OK) rc = dalloc(h, &wrank, nw);
+        if (rc == TPSPH_OK) rc = dalloc(h, &wcount, (size_t)ncells + 1);
+        if (rc == TPSPH_OK) {
+            hipError_t e1 = hipMemcpy(wx_d, wxh.data(), wxh.size() * sizeof(double), hipMemcpyHostToDevice);
+            hipError_t e2 = hipMemcpy(wm_d, wall_mass, (size_t)nw * sizeof(double), in_kind);
+            if (e1 != hipSuccess || e2 != hipSuccess) rc = fail(h, TPSPH_ERR_HIP, "copy of wall data failed");
         }
-        if (nd == 2)
-            rc = build_cells<2>(h, nw, wx_d, 2, wcell, wrank, wcount, h->w_start, h->w_perm);
-        else
-            rc = build_cells<3>(h, nw, wx_d, 3, wcell, wrank, wcount, h->w_start, h->w_perm);
+        if (rc == TPSPH_OK)
+            rc = nd == 2 ? build_cells<2>(h, nullptr, nw, wx_d, wcell, wrank, wcount, h->w_start, h->w_perm)
+                         : build_cells<3>(h, nullptr, nw, wx_d, wcell, wrank, wcount, h->w_start, h->w_perm);
         if (rc == TPSPH_OK) {
             if (nd == 2)
-                k_aos_to_soa<2><<<nblocks(nw), kBlock, 0, h->stream>>>(nw, wx_d, h->wxs, h->w_perm);
+                k_positions<2><<<nblocks(nw), kBlock, 0, h->stream>>>(nw, wx_d, h->w_perm, h->wx, h->wy, nullptr, h->wpos, wm_d);
             else
-                k_aos_to_soa<3><<<nblocks(nw), kBlock, 0, h->stream>>>(nw, wx_d, h->wxs, h->w_perm);
-            // wms[s] = wm[w_perm[s]]: gather through a 1-D "AoS" of stride 1
-            k_aos_to_soa<1><<<nblocks(nw), kBlock, 0, h->stream>>>(nw, wm_d, h->wms, h->w_perm);
+                k_positions<3><<<nblocks(nw), kBlock, 0, h->stream>>>(nw, wx_d, h->w_perm, h->wx, h->wy, h->wz, h->wpos, wm_d);
+            if (nd == 2)
+                k_wall_near<2><<<nblocks(nw), kBlock, 0, h->stream>>>(h->P, nw, h->wpos, h->wall_near);
+            else
+                k_wall_near<3><<<nblocks(nw), kBlock, 0, h->stream>>>(h->P, nw, h->wpos, h->wall_near);
             rc = launch_check(h, "wall setup");
         }
         hipError_t es = hipStreamSynchronize(h->stream);
-        (void)hipFree(wx_d); (void)hipFree(wm_d); (void)hipFree(wcell); (void)hipFree(wrank); (void)hipFree(wcount);
+        (void)hipFree(wx_d);
+        (void)hipFree(wm_d);
+        (void)hipFree(wcell);
+        (void)hipFree(wrank);
+        (void)hipFree(wcount);
+        if (rc == TPSPH_OK && es != hipSuccess)
+            rc = fail(h, TPSPH_ERR_HIP, std::string("wall setup: ") + hipGetErrorString(es));
         if (rc != TPSPH_OK) return cleanup_fail(rc);
-        if (es != hipSuccess) {
-            h->last_error = std::string("wall setup: ") + hipGetErrorString(es);
-            return cleanup_fail(TPSPH_ERR_HIP);
-        }
     }
 #undef CTRY
     *out = h;
@@ -985,10 +1321,10 @@ int tpsph_get_pressures(tpsph_handle *h, double *fluid_pressure, double *wall_pr
 {
     if (!h) return fail(nullptr, TPSPH_ERR_ARGUMENT, "handle is NULL");
     if (!h->have_kick) return fail(h, TPSPH_ERR_STATE, "no kick has been evaluated yet");
-    double *tmp = nullptr;
     const size_t n = (size_t)std::max(h->nf, h->nw);
+    double *tmp = nullptr, *pf_sorted = nullptr;
     TRY(dalloc(h, &tmp, n));
-    int rc = TPSPH_OK;
+    int rc = dalloc(h, &pf_sorted, (size_t)h->nf);
     auto emit = [&](double *dst, const double *sorted, const int *perm, int cnt) -> int {
         if (!dst || cnt == 0) return TPSPH_OK;
         k_unpermute<double><<<nblocks(cnt), kBlock, 0, h->stream>>>(cnt, perm, sorted, tmp);
@@ -997,10 +1333,15 @@ int tpsph_get_pressures(tpsph_handle *h, double *fluid_pressure, double *wall_pr
         HIP_TRY(h, hipStreamSynchronize(h->stream));
         return TPSPH_OK;
     };
-    rc = emit(fluid_pressure, h->ps, h->f_perm, h->nf);
+    if (rc == TPSPH_OK && fluid_pressure && h->nf > 0) {
+        k_record_pressure<<<nblocks(h->nf), kBlock, 0, h->stream>>>(h->nf, h->rec, pf_sorted);
+        rc = launch_check(h, "k_record_pressure");
+        if (rc == TPSPH_OK) rc = emit(fluid_pressure, pf_sorted, h->f_perm, h->nf);
+    }
     if (rc == TPSPH_OK) rc = emit(wall_pressure, h->pw, h->w_perm, h->nw);
     if (rc == TPSPH_OK) rc = emit(wall_density, h->rhow, h->w_perm, h->nw);
     (void)hipFree(tmp);
+    (void)hipFree(pf_sorted);
     return rc;
 }
 
@@ -1018,27 +1359,35 @@ int tpsph_neighbors_csr(tpsph_handle *h, const double *u_ode, int32_t pair, int6
         HIP_TRY(h, hipMemcpyAsync(h->st_u, u_ode, u_len(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
         u = h->st_u;
     }
-    // fluid cell list + sorted fluid coordinates for the current positions
-    TRY(nd == 2 ? build_cells<2>(h, nf, u, 2, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm)
-                : build_cells<3>(h, nf, u, 3, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm));
+    // fluid cell list of the current positions; the flag stays raised, so the next kick
+    // rebuilds and gathers its own state again
+    k_set_i32<<<1, 1, 0, h->stream>>>(h->flag, 1);
+    TRY(nd == 2 ? build_cells<2>(h, h->flag, nf, u, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm)
+                : build_cells<3>(h, h->flag, nf, u, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm));
     if (nf > 0) {
-        if (nd == 2) k_aos_to_soa<2><<<nblocks(nf), kBlock, 0, h->stream>>>(nf, u, h->xs, h->f_perm);
-        else k_aos_to_soa<3><<<nblocks(nf), kBlock, 0, h->stream>>>(nf, u, h->xs, h->f_perm);
-        TRY(launch_check(h, "k_aos_to_soa"));
+        if (nd == 2)
+            k_positions<2><<<nblocks(nf), kBlock, 0, h->stream>>>(nf, u, h->f_perm, h->xs, h->ys, nullptr, nullptr, nullptr);
+        else
+            k_positions<3><<<nblocks(nf), kBlock, 0, h->stream>>>(nf, u, h->f_perm, h->xs, h->ys, h->zs, nullptr, nullptr);
+        TRY(launch_check(h, "k_positions"));
     }
-    h->have_kick = false;  // xs no longer matches a kick
-    int nrows, ncols;
-    const double *xr, *xc;
+    h->have_kick = false;
+    int nrows;
+    const double *rx, *ry, *rz, *cx, *cy, *cz;
     const int *row_perm, *col_start, *col_perm;
+    double hs;
     if (pair == TPSPH_PAIR_FLUID_FLUID) {
-        nrows = nf; ncols = nf; xr = h->xs; xc = h->xs;
-        row_perm = h->f_perm; col_start = h->f_start; col_perm = h->f_perm;
+        nrows = nf; rx = h->xs; ry = h->ys; rz = h->zs; row_perm = h->f_perm;
+        col_start = h->f_start; cx = h->xs; cy = h->ys; cz = h->zs; col_perm = h->f_perm;
+        hs = h->P.hs_fluid;
     } else if (pair == TPSPH_PAIR_FLUID_WALL) {
-        nrows = nf; ncols = nw; xr = h->xs; xc = h->wxs;
-        row_perm = h->f_perm; col_start = h->w_start; col_perm = h->w_perm;
+        nrows = nf; rx = h->xs; ry = h->ys; rz = h->zs; row_perm = h->f_perm;
+        col_start = h->w_start; cx = h->wx; cy = h->wy; cz = h->wz; col_perm = h->w_perm;
+        hs = h->P.hs_wall;
     } else {
-        nrows = nw; ncols = nf; xr = h->wxs; xc = h->xs;
-        row_perm = h->w_perm; col_start = h->f_start; col_perm = h->f_perm;
+        nrows = nw; rx = h->wx; ry = h->wy; rz = h->wz; row_perm = h->w_perm;
+        col_start = h->f_start; cx = h->xs; cy = h->ys; cz = h->zs; col_perm = h->f_perm;
+        hs = h->P.hs_fluid;
     }
     int *counts = nullptr;
     int64_t *doff = nullptr;
@@ -1049,10 +1398,10 @@ int tpsph_neighbors_csr(tpsph_handle *h, const double *u_ode, int32_t pair, int6
     int rc = TPSPH_OK;
     if (nrows > 0) {
         if (nd == 2)
-            k_nbr_count<2><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, xr, row_perm, ncols, xc, col_start, counts);
+            k_nbr<2, false><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, counts, nullptr, nullptr);
         else
-            k_nbr_count<3><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, xr, row_perm, ncols, xc, col_start, counts);
-        rc = launch_check(h, "k_nbr_count");
+            k_nbr<3, false><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, counts, nullptr, nullptr);
+        rc = launch_check(h, "k_nbr<count>");
         if (rc == TPSPH_OK) {
             hipError_t e = hipMemcpyAsync(hc.data(), counts, (size_t)nrows * sizeof(int), hipMemcpyDeviceToHost, h->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1076,10 +1425,10 @@ int tpsph_neighbors_csr(tpsph_handle *h, const double *u_ode, int32_t pair, int6
         }
         if (rc == TPSPH_OK) {
             if (nd == 2)
-                k_nbr_fill<2><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, xr, row_perm, ncols, xc, col_start, col_perm, doff, didx);
+                k_nbr<2, true><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, nullptr, doff, didx);
             else
-                k_nbr_fill<3><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, xr, row_perm, ncols, xc, col_start, col_perm, doff, didx);
-            rc = launch_check(h, "k_nbr_fill");
+                k_nbr<3, true><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, nullptr, doff, didx);
+            rc = launch_check(h, "k_nbr<fill>");
         }
         if (rc == TPSPH_OK) {
             hipError_t e = hipMemcpyAsync(idx, didx, (size_t)total * sizeof(int), hipMemcpyDefault, h->stream);
@@ -1202,10 +1551,11 @@ void tpsph_destroy(tpsph_handle *h)
 {
     if (!h) return;
     (void)hipStreamSynchronize(h->stream);
-    void *ptrs[] = {h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm, h->scan_tiles,
-                    h->f_mass, h->xs, h->vs, h->rhos, h->ps, h->ms, h->w_start, h->w_perm,
-                    h->wxs, h->wms, h->pw, h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du,
-                    h->sol_dv, h->sol_tv, h->sol_tu};
+    void *ptrs[] = {h->flag, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm,
+                    h->scan_tiles, h->f_mass, h->xs, h->ys, h->zs, h->snap, h->rec, h->vol,
+                    h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near, h->pw,
+                    h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
+                    h->sol_tu};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);

@@ -48,6 +48,8 @@ class TpsphParams(C.Structure):
         ("clip_wall_pressure", C.c_int32),
         ("use_viscosity", C.c_int32),
         ("use_density_diffusion", C.c_int32),
+        ("cell_divisions", C.c_int32),
+        ("x_cell_divisions", C.c_int32),
         ("reserved0", C.c_int32),
         ("smoothing_length", C.c_double),
         ("reference_density", C.c_double),
@@ -62,7 +64,12 @@ class TpsphParams(C.Structure):
         ("pressure_offset", C.c_double),
         ("domain_min", C.c_double * 3),
         ("domain_max", C.c_double * 3),
+        ("skin", C.c_double),
     ]
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        self.skin = -1.0
 
 
 class TpsphError(RuntimeError):

@@ -59,10 +59,15 @@ class Semidiscretization:
 
     ``domain``: optional ``(min_corner, max_corner)`` of the cell grid (the reference's
     ``FullGridCellList`` box).  By default it is the bounding box of both systems, grown
-    by ``domain_margin`` times its extent on every side."""
+    by ``domain_margin`` times its extent on every side.
+
+    ``cell_divisions`` / ``x_cell_divisions`` (cells per compact support along y, z / x) and
+    ``skin`` (fraction of the compact support) tune the device cell list; they change speed,
+    never results' neighbour relation."""
 
     def __init__(self, *systems, neighborhood_search=None, parallelization_backend="hip",
-                 domain=None, domain_margin=0.25, device=0):
+                 domain=None, domain_margin=0.05, device=0, cell_divisions=0,
+                 x_cell_divisions=0, skin=-1.0):
         fluids = [s for s in systems if isinstance(s, WeaklyCompressibleSPHSystem)]
         walls = [s for s in systems if isinstance(s, WallBoundarySystem)]
         if len(fluids) != 1 or len(walls) > 1 or len(fluids) + len(walls) != len(systems):
@@ -97,6 +102,9 @@ class Semidiscretization:
         self.domain = (np.asarray(domain[0], dtype=np.float64),
                        np.asarray(domain[1], dtype=np.float64))
         self.neighborhood_search = neighborhood_search
+        self.cell_divisions = int(cell_divisions)
+        self.x_cell_divisions = int(x_cell_divisions)
+        self.skin = float(skin)
         self._handle = None
 
     # -- parameters of the C ABI
@@ -107,6 +115,9 @@ class Semidiscretization:
         p.ndims = f.ndims
         p.kernel = 0
         p.memory = native.TPSPH_MEM_DEVICE if self.backend == "hip" else native.TPSPH_MEM_HOST
+        p.cell_divisions = self.cell_divisions
+        p.x_cell_divisions = self.x_cell_divisions
+        p.skin = self.skin
         p.clip_fluid_eos = int(se.clip_negative_pressure)
         p.smoothing_length = f.smoothing_length
         p.reference_density = se.reference_density
