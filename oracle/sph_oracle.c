@@ -16,8 +16,9 @@
  *   - `Float64^Float64` is the C library pow().
  *
  * Neighbour iteration: for target a, all points b of the neighbour set with
- * |x_a - x_b|^2 <= R^2, enumerated through a uniform cell grid of cell size R
- * (cells walked z-outer, y, x-inner; inside a cell in ascending index).  The squared
+ * |x_a - x_b|^2 <= R^2, enumerated through a uniform cell grid of cell size R (larger only
+ * for boxes far wider than the point cloud; cells walked z-outer, y, x-inner; inside a
+ * cell in ascending index).  The squared
  * distance is ((dx*dx + dy*dy) + dz*dz) without FMA; the HIP path evaluates the radius
  * test with the same operation order so neighbour sets agree bit for bit.
  *
@@ -212,11 +213,13 @@ void oracle_av_monaghan(const oracle_params *P, const double *pos_diff, double d
 }
 
 /* ---------------------------------------------------------------------------------------
- * Cell grid (stand-in for PointNeighbors' GridNeighborhoodSearch; cell size = R).
+ * Cell grid (stand-in for PointNeighbors' GridNeighborhoodSearch; cell size R, larger
+ * only for boxes that would need more cells than memory holds).
  * ------------------------------------------------------------------------------------- */
 typedef struct {
     int nd;
     double R, R2;
+    double cs;      /* cell size: R, or larger where the box would need too many cells */
     double lo[3];
     int64_t n[3];
     int64_t ncells;
@@ -229,7 +232,7 @@ static int64_t clampi(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : 
 static void cell_of(const grid_t *g, const double *x, int64_t *c)
 {
     for (int d = 0; d < g->nd; d++) {
-        double f = floor((x[d] - g->lo[d]) / g->R);
+        double f = floor((x[d] - g->lo[d]) / g->cs);
         int64_t ci = (f < -1.0e15) ? 0 : (f > 1.0e15 ? g->n[d] - 1 : (int64_t)f);
         c[d] = clampi(ci, 0, g->n[d] - 1);
     }
@@ -245,14 +248,30 @@ static int grid_build(grid_t *g, int nd, double R, const double *lo, const doubl
     g->nd = nd;
     g->R = R;
     g->R2 = R * R;
+    /* Cells of size >= R keep the 3^D sweep exact.  A box far larger than the point cloud
+     * (outliers) would need more cells than memory holds: then the cell size grows until
+     * the grid has at most max(2^22, 8 np) cells.  Clamping an axis to 4096 cells is exact
+     * as well: points beyond the last cell share it (see cell_of). */
+    const double max_cells = fmax(4194304.0, 8.0 * (double)np);
+    g->cs = R;
+    for (;;) {
+        double total = 1.0;
+        for (int d = 0; d < nd; d++) {
+            double span = (hi[d] + R) - (lo[d] - R);
+            double n = ceil(span / g->cs) + 1.0;
+            total *= fmin(fmax(n, 1.0), 4096.0);
+        }
+        if (total <= max_cells) break;
+        g->cs *= 1.25;
+    }
     g->ncells = 1;
     for (int d = 0; d < 3; d++) {
         if (d < nd) {
             g->lo[d] = lo[d] - R;
             double span = (hi[d] + R) - g->lo[d];
-            int64_t n = (int64_t)ceil(span / R) + 1;
+            int64_t n = (int64_t)ceil(span / g->cs) + 1;
             if (n < 1) n = 1;
-            if (n > 4096) n = 4096; /* clamping keeps the search exact; see cell_of */
+            if (n > 4096) n = 4096;
             g->n[d] = n;
         } else {
             g->lo[d] = 0.0;

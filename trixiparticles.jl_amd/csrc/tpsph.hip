@@ -552,7 +552,7 @@ __device__ __forceinline__ void pair_wall(const DevParams &P, const WallNbr &n,
 // candidates per iteration without branches: the arrays cx/cy/cz carry kPad NaN entries
 // past their end, so reads beyond a range stay in bounds and never pass the test, and every
 // candidate is written to the list while the count only advances on a hit.
-template <int ND, bool WALL>
+template <int ND, bool WALL, int CAP = kCap, int STRIDE = kBlock>
 __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *__restrict__ start,
                                                 const double *__restrict__ cx,
                                                 const double *__restrict__ cy,
@@ -569,7 +569,7 @@ __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *_
     while (true) {
         int cnt = 0;
         // phase 1: exact radius test
-        while (!done && cnt <= kCap - 4) {
+        while (!done && cnt <= CAP - 4) {
             if (j >= je) {
                 if (!it.next(P, start, j, je)) done = true;
                 continue;
@@ -586,7 +586,7 @@ __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *_
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const bool hit = (j + k < je) & (d2[k] <= P.R2) & (d2[k] >= P.az2);
-                lst[cnt * kBlock] = j + k;
+                lst[cnt * STRIDE] = j + k;
                 cnt += hit ? 1 : 0;
             }
             j += 4;
@@ -596,14 +596,14 @@ __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *_
             if constexpr (WALL) {
                 WallNbr cur = load_wall(Wv, lst[0]);
                 for (int k = 0; k < cnt; k++) {
-                    const WallNbr nxt = load_wall(Wv, lst[min(k + 1, cnt - 1) * kBlock]);
+                    const WallNbr nxt = load_wall(Wv, lst[min(k + 1, cnt - 1) * STRIDE]);
                     pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
                     cur = nxt;
                 }
             } else {
                 FluidNbr cur = load_fluid(F, lst[0]);
                 for (int k = 0; k < cnt; k++) {
-                    const FluidNbr nxt = load_fluid(F, lst[min(k + 1, cnt - 1) * kBlock]);
+                    const FluidNbr nxt = load_fluid(F, lst[min(k + 1, cnt - 1) * STRIDE]);
                     pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
                     cur = nxt;
                 }
@@ -679,6 +679,360 @@ __global__ void k_wall_near(DevParams P, int nw, const double4 *__restrict__ wpo
     for (int cz = z0; cz <= z1; cz++)
         for (int cy = y0; cy <= y1; cy++)
             for (int cx = x0; cx <= x1; cx++) near[cx + P.n[0] * (cy + P.n[1] * cz)] = 1;
+}
+
+// ------------------------------------------------------------------------------------------
+// Tiled fluid interaction.  The cell grid is cut into tiles of TX x TY x TZ cells; each block
+// takes tiles from a work counter.  For a tile it stages in LDS, with coalesced loads, every
+// fluid particle that can be a neighbour of one of the tile's particles: the cell rows
+// (y, z) within reach of the tile box, each trimmed in x to sqrt(R^2 - d^2) + skin, where d
+// is the row's transverse distance from the box (both grown by half the skin).  Staged per
+// candidate: x, y, z, rho, V and the global index.  The tile's particles then run the
+// two-phase sweep of the global kernel with positions, rho and V read from LDS and only
+// {v, p} gathered from global memory.  A tile whose halo exceeds the LDS capacity runs the
+// global sweep instead.  Wall neighbours always come from the global wall sweep.
+//
+// The halo widths (cells) follow from the cell sizes: a cell is (R + skin/2 + margin) / k
+// along an axis with k divisions, so the reach of any point stays within k cells.
+// ------------------------------------------------------------------------------------------
+template <int ND>
+struct TileCfg;
+template <>
+struct TileCfg<3> {
+    static constexpr int TX = 8, TY = 4, TZ = 8, HX = 4, HY = 2, HZ = 2;
+    static constexpr int THREADS = 256, CAPC = 1280, CAPL = 24;
+};
+template <>
+struct TileCfg<2> {
+    static constexpr int TX = 24, TY = 12, TZ = 1, HX = 4, HY = 2, HZ = 0;
+    static constexpr int THREADS = 256, CAPC = 1280, CAPL = 24;
+};
+
+template <int ND>
+struct TileSmem {
+    using C = TileCfg<ND>;
+    static constexpr int URY = C::TY + 2 * C::HY, URZ = C::TZ + 2 * C::HZ;
+    static constexpr int MAXR = URY * URZ;
+    static constexpr int MAXCS = MAXR * (C::TX + 2 * C::HX + 1);
+    static constexpr int TROWS = C::TY * C::TZ;
+    double x[C::CAPC + 4], y[C::CAPC + 4], z[ND == 3 ? C::CAPC + 4 : 1];
+    double rho[C::CAPC], vol[C::CAPC];
+    int g[C::CAPC];
+    int row_gb[MAXR], row_x0[MAXR], row_nx[MAXR], row_lds[MAXR + 1], row_cs[MAXR + 1];
+    int cstart[MAXCS];
+    int trow_gb[TROWS], trow_pre[TROWS + 1];
+    // neighbour lists: u16 LDS indices in the tiled sweep, reused as int lists (half the
+    // entries) by the global sweeps; see l16 for the lane-disjoint layout
+    unsigned short lst[C::CAPL * C::THREADS];
+    int tile, ntot, ncs, ntargets, uy0, nuy, uz0, nuz;
+};
+
+// largest r in [0, n) with pre[r] <= i (pre ascending, pre[0] = 0)
+__device__ __forceinline__ int bsearch_le(const int *pre, int n, int i)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// bounds of cell c along axis d grown by hs; border cells reach to infinity outwards
+__device__ __forceinline__ double cell_lo(const DevParams &P, int d, int c, double hs)
+{
+    return (c == 0) ? -INFINITY : P.lo[d] + c * P.cs[d] - hs;
+}
+__device__ __forceinline__ double cell_hi(const DevParams &P, int d, int c, double hs)
+{
+    return (c == P.n[d] - 1) ? INFINITY : P.lo[d] + (c + 1) * P.cs[d] + hs;
+}
+
+// Lane lists in the tiled kernel.  The same LDS area serves as u16 lists (tiled sweep) and
+// as int lists with stride THREADS (global sweeps), and waves of a block switch between the
+// two without a barrier.  Entry k of lane t therefore sits in half (k & 1) of the lane's int
+// slot t + THREADS (k >> 1): both views keep every lane inside its own 4-byte slots.
+// lst points at the u16 pair of slot t (S.lst + 2 t).
+template <int THREADS>
+__device__ __forceinline__ int l16(int k)
+{
+    return 2 * THREADS * (k >> 1) + (k & 1);
+}
+
+// Two-phase sweep over the staged fluid candidates of one target (positions, rho, V from
+// LDS, {v, p} from global memory).
+template <int ND>
+__device__ __forceinline__ void tile_sweep(const DevParams &P, const FluidView &F,
+                                           const TileSmem<ND> &S, unsigned short *lst,
+                                           bool active, const double *xa, const double *va,
+                                           double rho_a, double p_a, double inv_rho_a,
+                                           double *acc, double &drho)
+{
+    using C = TileCfg<ND>;
+    const double hs = P.hs_fluid;
+    const double reach = P.R + hs;
+    const int y0 = cell_1d(xa[1] - reach, P.lo[1], P.inv_cs[1], P.n[1]);
+    const int y1 = cell_1d(xa[1] + reach, P.lo[1], P.inv_cs[1], P.n[1]);
+    int z0 = 0, z1 = 0;
+    if constexpr (ND == 3) {
+        z0 = cell_1d(xa[2] - reach, P.lo[2], P.inv_cs[2], P.n[2]);
+        z1 = cell_1d(xa[2] + reach, P.lo[2], P.inv_cs[2], P.n[2]);
+    }
+    int iy = y1 + 1, iz = z0 - 1;
+    double dz2 = 0.0;
+    bool done = !active;
+    int j = 0, je = 0;
+    while (true) {
+        int cnt = 0;
+        while (!done && cnt <= C::CAPL - 4) {
+            if (j >= je) {
+                // next pencil (union row) of this target
+                if (iy > y1) {
+                    if (++iz > z1) {
+                        done = true;
+                        continue;
+                    }
+                    iy = y0;
+                    if constexpr (ND == 3) {
+                        const double dz = transverse(P, 2, iz, xa[2], hs);
+                        dz2 = dz * dz;
+                    }
+                }
+                const int cy = iy++;
+                // rows outside the staged union can not hold neighbours (the union covers
+                // the reach of every point of the grown tile box); skip them defensively
+                if (cy < S.uy0 || cy >= S.uy0 + S.nuy || iz < S.uz0 || iz >= S.uz0 + S.nuz) continue;
+                const int ur = (cy - S.uy0) + S.nuy * (iz - S.uz0);
+                const int nx = S.row_nx[ur];
+                if (nx == 0) continue;
+                const double dy = transverse(P, 1, cy, xa[1], hs);
+                const double r2 = P.R2 - fma(dy, dy, dz2);
+                if (r2 < 0.0) continue;
+                const double rx = (double)(__builtin_amdgcn_sqrtf((float)r2) * 1.000001f) + hs;
+                const int rx0 = S.row_x0[ur];
+                const int x0 = min(max(cell_1d(xa[0] - rx, P.lo[0], P.inv_cs[0], P.n[0]) - rx0, 0), nx);
+                const int x1 = max(min(cell_1d(xa[0] + rx, P.lo[0], P.inv_cs[0], P.n[0]) - rx0, nx - 1), -1);
+                if (x0 > x1) continue;
+                const int cs = S.row_cs[ur], base = S.row_lds[ur] - S.row_gb[ur];
+                j = base + S.cstart[cs + x0];
+                je = base + S.cstart[cs + x1 + 1];
+                continue;
+            }
+            double d2[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double p0 = xa[0] - S.x[j + k], p1 = xa[1] - S.y[j + k];
+                const double p2 = (ND == 3) ? xa[2] - S.z[j + k] : 0.0;
+                d2[k] = dist2<ND>(p0, p1, p2);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool hit = (j + k < je) & (d2[k] <= P.R2) & (d2[k] >= P.az2);
+                lst[l16<C::THREADS>(cnt)] = (unsigned short)(j + k);
+                cnt += hit ? 1 : 0;
+            }
+            j += 4;
+        }
+        // phase 2
+        if (cnt > 0) {
+            int jl = lst[0];
+            double4 vp = F.rec[2 * (size_t)S.g[jl] + 1];
+            for (int k = 0; k < cnt; k++) {
+                const int jn = lst[l16<C::THREADS>(min(k + 1, cnt - 1))];
+                const double4 vpn = F.rec[2 * (size_t)S.g[jn] + 1];
+                FluidNbr n;
+                n.a = make_double4(S.x[jl], S.y[jl], (ND == 3) ? S.z[jl] : 0.0, S.rho[jl]);
+                n.b = vp;
+                n.v = make_double2(S.vol[jl], S.vol[jl] * S.rho[jl]);
+                pair_fluid<ND>(P, n, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                jl = jn;
+                vp = vpn;
+            }
+        }
+        if (!__any(!done)) break;
+    }
+}
+
+template <int ND>
+__global__ void __launch_bounds__(TileCfg<ND>::THREADS)
+    k_fluid_tiled(DevParams P, FluidView F, WallView Wv, const unsigned char *__restrict__ wall_near,
+                  const int *__restrict__ tiles, const int *__restrict__ ntiles,
+                  int *__restrict__ work, int tgx, int tgy, double *__restrict__ dv)
+{
+    using C = TileCfg<ND>;
+    using SM = TileSmem<ND>;
+    __shared__ SM S;
+    const int tid = threadIdx.x;
+    const int nt = *ntiles;
+    while (true) {
+        if (tid == 0) S.tile = atomicAdd(work, 1);
+        __syncthreads();
+        const int tile_slot = S.tile;
+        if (tile_slot >= nt) break;
+        const int tile = tiles[tile_slot];
+        const int tx = tile % tgx, ty = (tile / tgx) % tgy, tz = tile / (tgx * tgy);
+        const int X0 = tx * C::TX, X1 = min(X0 + C::TX, P.n[0]) - 1;
+        const int Y0 = ty * C::TY, Y1 = min(Y0 + C::TY, P.n[1]) - 1;
+        const int Z0 = (ND == 3) ? tz * C::TZ : 0;
+        const int Z1 = (ND == 3) ? min(Z0 + C::TZ, P.n[2]) - 1 : 0;
+        const int uy0 = max(Y0 - C::HY, 0), uy1 = min(Y1 + C::HY, P.n[1] - 1);
+        const int uz0 = (ND == 3) ? max(Z0 - C::HZ, 0) : 0;
+        const int uz1 = (ND == 3) ? min(Z1 + C::HZ, P.n[2] - 1) : 0;
+        const int nuy = uy1 - uy0 + 1, nrows = nuy * (uz1 - uz0 + 1);
+        const double hs = P.hs_fluid;
+        // tile box grown by half the skin (targets may have moved that far)
+        const double bylo = cell_lo(P, 1, Y0, hs), byhi = cell_hi(P, 1, Y1, hs);
+        const double bzlo = (ND == 3) ? cell_lo(P, 2, Z0, hs) : 0.0;
+        const double bzhi = (ND == 3) ? cell_hi(P, 2, Z1, hs) : 0.0;
+        const double bxlo = cell_lo(P, 0, X0, hs), bxhi = cell_hi(P, 0, X1, hs);
+
+        // --- union rows: prune, trim, count
+        int my_cnt = 0, my_cs = 0;
+        if (tid < nrows) {
+            const int cy = uy0 + tid % nuy, cz = uz0 + tid / nuy;
+            const double dy = fmax(0.0, fmax(cell_lo(P, 1, cy, hs) - byhi, bylo - cell_hi(P, 1, cy, hs)));
+            double dz = 0.0;
+            if constexpr (ND == 3)
+                dz = fmax(0.0, fmax(cell_lo(P, 2, cz, hs) - bzhi, bzlo - cell_hi(P, 2, cz, hs)));
+            const double r2 = P.R2 - (dy * dy + dz * dz);
+            int nx = 0, x0 = 0, gb = 0;
+            if (r2 >= 0.0) {
+                const double rx = (double)(__builtin_amdgcn_sqrtf((float)r2) * 1.000001f) + hs;
+                x0 = cell_1d(bxlo - rx, P.lo[0], P.inv_cs[0], P.n[0]);
+                const int x1 = cell_1d(bxhi + rx, P.lo[0], P.inv_cs[0], P.n[0]);
+                const int row = P.n[0] * (cy + P.n[1] * cz);
+                gb = F.start[row + x0];
+                my_cnt = F.start[row + x1 + 1] - gb;
+                nx = x1 - x0 + 1;
+                my_cs = nx + 1;
+            }
+            S.row_nx[tid] = nx;
+            S.row_x0[tid] = x0;
+            S.row_gb[tid] = gb;
+        }
+        // --- target rows
+        int my_t = 0;
+        if (tid < (Y1 - Y0 + 1) * (Z1 - Z0 + 1)) {
+            const int ny = Y1 - Y0 + 1;
+            const int cy = Y0 + tid % ny, cz = Z0 + tid / ny;
+            const int row = P.n[0] * (cy + P.n[1] * cz);
+            const int gb = F.start[row + X0];
+            S.trow_gb[tid] = gb;
+            my_t = F.start[row + X1 + 1] - gb;
+        }
+        int tot;
+        const int lds0 = block_exclusive_scan(my_cnt, (int *)S.lst, tot);
+        if (tid < nrows) S.row_lds[tid] = lds0;
+        if (tid == 0) S.ntot = tot;
+        const int cs0 = block_exclusive_scan(my_cs, (int *)S.lst, tot);
+        if (tid < nrows) S.row_cs[tid] = cs0;
+        if (tid == 0) S.ncs = tot;
+        const int tr0 = block_exclusive_scan(my_t, (int *)S.lst, tot);
+        const int ntrows = (Y1 - Y0 + 1) * (Z1 - Z0 + 1);
+        if (tid < ntrows) S.trow_pre[tid] = tr0;
+        if (tid == 0) {
+            S.ntargets = tot;
+            S.trow_pre[ntrows] = tot;
+            S.row_lds[nrows] = S.ntot;
+            S.row_cs[nrows] = S.ncs;
+            S.uy0 = uy0;
+            S.nuy = nuy;
+            S.uz0 = uz0;
+            S.nuz = uz1 - uz0 + 1;
+        }
+        __syncthreads();
+        const int ntot = S.ntot, ntargets = S.ntargets;
+        const bool staged = ntot <= C::CAPC && S.ncs <= SM::MAXCS;
+        if (staged) {
+            for (int i = tid; i < S.ncs; i += C::THREADS) {
+                const int r = bsearch_le(S.row_cs, nrows, i);
+                const int cy = uy0 + r % nuy, cz = uz0 + r / nuy;
+                S.cstart[i] = F.start[P.n[0] * (cy + P.n[1] * cz) + S.row_x0[r] + (i - S.row_cs[r])];
+            }
+            for (int i = tid; i < ntot; i += C::THREADS) {
+                const int r = bsearch_le(S.row_lds, nrows, i);
+                const int g = S.row_gb[r] + (i - S.row_lds[r]);
+                S.x[i] = F.x[g];
+                S.y[i] = F.y[g];
+                if constexpr (ND == 3) S.z[i] = F.z[g];
+                S.rho[i] = F.rec[2 * (size_t)g].w;
+                S.vol[i] = F.vol[g].x;
+                S.g[i] = g;
+            }
+        }
+        __syncthreads();
+
+        // --- the tile's particles
+        for (int base = 0; base < ntargets; base += C::THREADS) {
+            const int t = base + tid;
+            const bool active = t < ntargets;
+            int s = 0;
+            double xa[3] = {0.0, 0.0, 0.0}, va[3] = {0.0, 0.0, 0.0}, rho_a = 1.0, p_a = 0.0;
+            if (active) {
+                const int r = bsearch_le(S.trow_pre, ntrows, t);
+                s = S.trow_gb[r] + (t - S.trow_pre[r]);
+                const double4 a = F.rec[2 * (size_t)s];
+                const double4 b = F.rec[2 * (size_t)s + 1];
+                xa[0] = a.x; xa[1] = a.y; xa[2] = a.z; rho_a = a.w;
+                va[0] = b.x; va[1] = b.y; va[2] = b.z; p_a = b.w;
+            }
+            const double inv_rho_a = 1.0 / rho_a;
+            double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
+            if (staged)
+                tile_sweep<ND>(P, F, S, S.lst + 2 * tid, active, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+            else
+                neighbour_sweep<ND, false, C::CAPL / 2, C::THREADS>(
+                    P, F.start, F.x, F.y, F.z, hs, active, F, Wv, (int *)S.lst + tid, xa, va,
+                    rho_a, p_a, inv_rho_a, acc, drho);
+            double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
+            if (P.nw > 0) {
+                bool near = false;
+                if (active) {
+                    const int c0 = cell_1d(xa[0], P.lo[0], P.inv_cs[0], P.n[0]);
+                    const int c1 = cell_1d(xa[1], P.lo[1], P.inv_cs[1], P.n[1]);
+                    const int c2 = (ND == 3) ? cell_1d(xa[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+                    near = wall_near[c0 + P.n[0] * (c1 + P.n[1] * c2)] != 0;
+                }
+                if (__any(near))
+                    neighbour_sweep<ND, true, C::CAPL / 2, C::THREADS>(
+                        P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, near, F, Wv, (int *)S.lst + tid,
+                        xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+            }
+            if (active) {
+                const int i = F.perm[s];
+#pragma unroll
+                for (int d = 0; d < ND; d++) dv[(size_t)i * (ND + 1) + d] = (acc[d] + accw[d]) + P.g[d];
+                dv[(size_t)i * (ND + 1) + ND] = drho + drhow;
+            }
+        }
+        __syncthreads();  // LDS is restaged for the next tile
+    }
+}
+
+// Active tiles (at least one fluid particle), listed after every rebuild of the cell list.
+template <int ND>
+__global__ void k_tiles(const int *flag, DevParams P, const int *__restrict__ start, int tgx,
+                        int tgy, int ntiles_all, int *__restrict__ tiles, int *__restrict__ ntiles)
+{
+    if (skip(flag)) return;
+    using C = TileCfg<ND>;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles_all) return;
+    const int tx = t % tgx, ty = (t / tgx) % tgy, tz = t / (tgx * tgy);
+    const int X0 = tx * C::TX, X1 = min(X0 + C::TX, P.n[0]) - 1;
+    const int Y0 = ty * C::TY, Y1 = min(Y0 + C::TY, P.n[1]) - 1;
+    const int Z0 = (ND == 3) ? tz * C::TZ : 0;
+    const int Z1 = (ND == 3) ? min(Z0 + C::TZ, P.n[2]) - 1 : 0;
+    int cnt = 0;
+    for (int cz = Z0; cz <= Z1; cz++)
+        for (int cy = Y0; cy <= Y1; cy++) {
+            const int row = P.n[0] * (cy + P.n[1] * cz);
+            cnt += start[row + X1 + 1] - start[row + X0];
+        }
+    if (cnt > 0) tiles[atomicAdd(ntiles, 1)] = t;
 }
 
 // drift!: du = v (semidiscretization.jl:561-571; delta_v = 0 without shifting)
@@ -841,6 +1195,10 @@ struct tpsph_handle {
     double4 *wpos = nullptr;
     double2 *wpv = nullptr;
     unsigned char *wall_near = nullptr;  // per cell: a wall particle lies within reach
+    // tiled fluid kernel
+    bool tiled = false;
+    int tgx = 1, tgy = 1, ntiles_all = 0, n_cu = 256;
+    int *tiles = nullptr, *ntiles = nullptr, *work = nullptr;
     // host-mode staging and solver scratch
     double *st_v = nullptr, *st_u = nullptr, *st_dv = nullptr, *st_du = nullptr;
     double *sol_dv = nullptr, *sol_tv = nullptr, *sol_tu = nullptr;
@@ -934,9 +1292,9 @@ struct TimedScope {
 int scan(tpsph_handle *h, const int *flag, const int *in, int *out, int n)
 {
     int tiles = (n + kScanTile - 1) / kScanTile;
-    k_scan_tiles<<<tiles, kBlock, 0, h->stream>>>(flag, in, out, h->scan_tiles, n);
-    k_scan_single<<<1, kBlock, 0, h->stream>>>(flag, h->scan_tiles, tiles);
-    k_scan_add<<<tiles, kBlock, 0, h->stream>>>(flag, out, h->scan_tiles, n);
+    hipLaunchKernelGGL((k_scan_tiles), dim3(tiles), dim3(kBlock), 0, h->stream, flag, in, out, h->scan_tiles, n);
+    hipLaunchKernelGGL((k_scan_single), dim3(1), dim3(kBlock), 0, h->stream, flag, h->scan_tiles, tiles);
+    hipLaunchKernelGGL((k_scan_add), dim3(tiles), dim3(kBlock), 0, h->stream, flag, out, h->scan_tiles, n);
     return launch_check(h, "scan");
 }
 
@@ -946,14 +1304,14 @@ int build_cells(tpsph_handle *h, const int *flag, int n, const double *x, int *c
                 int *count, int *start, int *perm)
 {
     const int nc = h->P.ncells;
-    k_fill_i32<<<grid_blocks(nc + 1), kBlock, 0, h->stream>>>(flag, count, 0, nc + 1);
+    hipLaunchKernelGGL((k_fill_i32), dim3(grid_blocks(nc + 1)), dim3(kBlock), 0, h->stream, flag, count, 0, nc + 1);
     if (n > 0)
-        k_cells<ND><<<nblocks(n), kBlock, 0, h->stream>>>(flag, h->P, n, x, cell, rank, count);
+        hipLaunchKernelGGL((k_cells<ND>), dim3(nblocks(n)), dim3(kBlock), 0, h->stream, flag, h->P, n, x, cell, rank, count);
     TRY(launch_check(h, "k_cells"));
     TRY(scan(h, flag, count, start, nc + 1));
     if (n > 0) {
-        k_scatter<<<nblocks(n), kBlock, 0, h->stream>>>(flag, n, cell, rank, start, perm);
-        k_sort_cells<<<grid_blocks(nc), kBlock, 0, h->stream>>>(flag, nc, start, perm);
+        hipLaunchKernelGGL((k_scatter), dim3(nblocks(n)), dim3(kBlock), 0, h->stream, flag, n, cell, rank, start, perm);
+        hipLaunchKernelGGL((k_sort_cells), dim3(grid_blocks(nc)), dim3(kBlock), 0, h->stream, flag, nc, start, perm);
     }
     return launch_check(h, "k_scatter/k_sort_cells");
 }
@@ -966,7 +1324,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
     if (nf == 0) return TPSPH_OK;
     {
         TimedScope t(h, TPSPH_TIME_GATHER);
-        k_gather<ND, 0><<<nblocks(nf), kBlock, 0, h->stream>>>(
+        hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, 
             h->flag, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
         TRY(launch_check(h, "k_gather<0>"));
     }
@@ -974,9 +1332,15 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
         TimedScope t(h, TPSPH_TIME_CELL_LIST);
         TRY(build_cells<ND>(h, h->flag, nf, u, h->f_cell, h->f_rank, h->f_count, h->f_start,
                             h->f_perm));
-        k_gather<ND, 1><<<nblocks(nf), kBlock, 0, h->stream>>>(
+        if (h->tiled) {
+            hipLaunchKernelGGL((k_fill_i32), dim3(1), dim3(1), 0, h->stream, h->flag, h->ntiles, 0, 1);
+            hipLaunchKernelGGL((k_tiles<ND>), dim3(nblocks(h->ntiles_all)), dim3(kBlock), 0, h->stream, 
+                h->flag, h->P, h->f_start, h->tgx, h->tgy, h->ntiles_all, h->tiles, h->ntiles);
+            TRY(launch_check(h, "k_tiles"));
+        }
+        hipLaunchKernelGGL((k_gather<ND, 1>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, 
             h->flag, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
-        k_set_i32<<<1, 1, 0, h->stream>>>(h->flag, 0);
+        hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->flag, 0);
         TRY(launch_check(h, "k_gather<1>"));
     }
     return TPSPH_OK;
@@ -989,14 +1353,22 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u)
     TRY(update_fluid<ND>(h, v, u));
     if (nw > 0) {
         TimedScope t(h, TPSPH_TIME_WALL);
-        k_wall<ND><<<nblocks(nw), kBlock, 0, h->stream>>>(h->P, h->fluid_view(), h->wpos, h->pw,
+        hipLaunchKernelGGL((k_wall<ND>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P, h->fluid_view(), h->wpos, h->pw,
                                                            h->rhow, h->wpv);
         TRY(launch_check(h, "k_wall"));
     }
     if (nf > 0) {
         TimedScope t(h, TPSPH_TIME_FLUID);
-        k_fluid<ND><<<nblocks(nf), kBlock, 0, h->stream>>>(h->P, h->fluid_view(), h->wall_view(),
-                                                             h->wall_near, dv);
+        if (h->tiled) {
+            hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->work, 0);
+            const int grid = std::max(1, std::min(h->ntiles_all, 2 * h->n_cu));
+            hipLaunchKernelGGL((k_fluid_tiled<ND>), dim3(grid), dim3(TileCfg<ND>::THREADS), 0, h->stream, 
+                h->P, h->fluid_view(), h->wall_view(), h->wall_near, h->tiles, h->ntiles, h->work,
+                h->tgx, h->tgy, dv);
+        } else {
+            hipLaunchKernelGGL((k_fluid<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, h->P, h->fluid_view(), h->wall_view(),
+                                                                 h->wall_near, dv);
+        }
         TRY(launch_check(h, "k_fluid"));
     }
     h->have_kick = true;
@@ -1149,10 +1521,19 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     const int div = params->cell_divisions > 0 ? params->cell_divisions : 2;
     const int xdiv = params->x_cell_divisions > 0 ? params->x_cell_divisions : 2 * div;
     const double skin = params->skin >= 0.0 ? params->skin : 0.04;
+    double extent = R;
+    for (int d = 0; d < nd; d++) extent = std::fmax(extent, std::fmax(std::fabs(lo[d]), std::fabs(hi[d])) + 4.0 * R);
+    // cell-interval rounding margin: far above the error of floor((x - lo) / cs)
+    const double margin = 1e-9 * R + 64.0 * DBL_EPSILON * extent;
+    P.half_skin = 0.5 * skin * R;
+    P.hs_fluid = P.half_skin + margin;
+    P.hs_wall = margin;
+    // k cells span the reach of a point: R (as bounded by the FP32 square roots of the pencil
+    // widths, which run at most 2e-6 R above it), its own drift and a neighbour's drift
+    const double span = (R * 1.00001 + 2.0 * P.hs_fluid) * (1.0 + 1e-9);
     int64_t ncells = 1;
-    double extent = 0.0;
     for (int d = 0; d < 3; d++) {
-        const double cs = (R / (d == 0 ? xdiv : div)) * (1.0 + std::ldexp(1.0, -30));
+        const double cs = span / (d == 0 ? xdiv : div);
         P.cs[d] = cs;
         P.inv_cs[d] = 1.0 / cs;
         if (d < nd) {
@@ -1164,7 +1545,6 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
                 return fail(nullptr, TPSPH_ERR_ARGUMENT, "domain box too large for the cell grid");
             }
             P.n[d] = (int)n;
-            extent = std::fmax(extent, std::fabs(P.lo[d]) + n * cs);
         } else {
             P.lo[d] = 0.0;
             P.n[d] = 1;
@@ -1176,11 +1556,24 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         return fail(nullptr, TPSPH_ERR_ARGUMENT, "cell grid too large (domain / smoothing_length)");
     }
     P.ncells = (int)ncells;
-    // cell-interval rounding margin: far above the error of floor((x - lo) / cs)
-    const double margin = 1e-9 * R + 64.0 * DBL_EPSILON * extent;
-    P.half_skin = 0.5 * skin * R;
-    P.hs_fluid = P.half_skin + margin;
-    P.hs_wall = margin;
+    // tiled fluid kernel: needs the default cell divisions (its halo widths)
+    h->tiled = (nd == 3) ? (div == TileCfg<3>::HY && xdiv == TileCfg<3>::HX)
+                         : (div == TileCfg<2>::HY && xdiv == TileCfg<2>::HX);
+    if (nd == 3) {
+        h->tgx = (P.n[0] + TileCfg<3>::TX - 1) / TileCfg<3>::TX;
+        h->tgy = (P.n[1] + TileCfg<3>::TY - 1) / TileCfg<3>::TY;
+        h->ntiles_all = h->tgx * h->tgy * ((P.n[2] + TileCfg<3>::TZ - 1) / TileCfg<3>::TZ);
+    } else {
+        h->tgx = (P.n[0] + TileCfg<2>::TX - 1) / TileCfg<2>::TX;
+        h->tgy = (P.n[1] + TileCfg<2>::TY - 1) / TileCfg<2>::TY;
+        h->ntiles_all = h->tgx * h->tgy;
+    }
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            h->n_cu = cus;
+    }
 
     auto cleanup_fail = [&](int rc) {
         std::string m = h->last_error;
@@ -1212,6 +1605,9 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     CTRY(dalloc(h, &h->wy, (size_t)nw + kPad));
     CTRY(dalloc(h, &h->wz, (size_t)nw + kPad));
     CTRY(dalloc(h, &h->wall_near, (size_t)ncells));
+    CTRY(dalloc(h, &h->tiles, (size_t)h->ntiles_all));
+    CTRY(dalloc(h, &h->ntiles, 1));
+    CTRY(dalloc(h, &h->work, 1));
     // NaN (all bits set) in the coordinate arrays, including their padding; never zero so a
     // padded entry can not pass the radius test
     for (double *a : {h->xs, h->ys, h->zs})
@@ -1226,7 +1622,7 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     CTRY(dalloc(h, &h->wpv, nw));
     CTRY(dalloc(h, &h->pw, nw));
     CTRY(dalloc(h, &h->rhow, nw));
-    k_set_i32<<<1, 1, 0, h->stream>>>(h->flag, 1);  // the first kick builds the fluid cells
+    hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->flag, 1);  // the first kick builds the fluid cells
     CTRY(launch_check(h, "k_set_i32"));
     if (nf > 0) {
         hipError_t e = hipMemcpy(h->f_mass, fluid_mass, (size_t)nf * sizeof(double), in_kind);
@@ -1254,13 +1650,13 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
                          : build_cells<3>(h, nullptr, nw, wx_d, wcell, wrank, wcount, h->w_start, h->w_perm);
         if (rc == TPSPH_OK) {
             if (nd == 2)
-                k_positions<2><<<nblocks(nw), kBlock, 0, h->stream>>>(nw, wx_d, h->w_perm, h->wx, h->wy, nullptr, h->wpos, wm_d);
+                hipLaunchKernelGGL((k_positions<2>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, nw, wx_d, h->w_perm, h->wx, h->wy, nullptr, h->wpos, wm_d);
             else
-                k_positions<3><<<nblocks(nw), kBlock, 0, h->stream>>>(nw, wx_d, h->w_perm, h->wx, h->wy, h->wz, h->wpos, wm_d);
+                hipLaunchKernelGGL((k_positions<3>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, nw, wx_d, h->w_perm, h->wx, h->wy, h->wz, h->wpos, wm_d);
             if (nd == 2)
-                k_wall_near<2><<<nblocks(nw), kBlock, 0, h->stream>>>(h->P, nw, h->wpos, h->wall_near);
+                hipLaunchKernelGGL((k_wall_near<2>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P, nw, h->wpos, h->wall_near);
             else
-                k_wall_near<3><<<nblocks(nw), kBlock, 0, h->stream>>>(h->P, nw, h->wpos, h->wall_near);
+                hipLaunchKernelGGL((k_wall_near<3>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P, nw, h->wpos, h->wall_near);
             rc = launch_check(h, "wall setup");
         }
         hipError_t es = hipStreamSynchronize(h->stream);
@@ -1305,9 +1701,9 @@ int tpsph_drift(tpsph_handle *h, double *du_ode, const double *v_ode, const doub
     if (!du_ode || !v_ode) return fail(h, TPSPH_ERR_ARGUMENT, "du_ode and v_ode must not be NULL");
     if (!is_host(h)) {
         if (h->nd == 2)
-            k_drift<2><<<nblocks(h->nf), kBlock, 0, h->stream>>>(h->nf, v_ode, du_ode);
+            hipLaunchKernelGGL((k_drift<2>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, v_ode, du_ode);
         else
-            k_drift<3><<<nblocks(h->nf), kBlock, 0, h->stream>>>(h->nf, v_ode, du_ode);
+            hipLaunchKernelGGL((k_drift<3>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, v_ode, du_ode);
         return launch_check(h, "k_drift");
     }
     for (int i = 0; i < h->nf; i++)
@@ -1327,14 +1723,14 @@ int tpsph_get_pressures(tpsph_handle *h, double *fluid_pressure, double *wall_pr
     int rc = dalloc(h, &pf_sorted, (size_t)h->nf);
     auto emit = [&](double *dst, const double *sorted, const int *perm, int cnt) -> int {
         if (!dst || cnt == 0) return TPSPH_OK;
-        k_unpermute<double><<<nblocks(cnt), kBlock, 0, h->stream>>>(cnt, perm, sorted, tmp);
+        hipLaunchKernelGGL((k_unpermute<double>), dim3(nblocks(cnt)), dim3(kBlock), 0, h->stream, cnt, perm, sorted, tmp);
         TRY(launch_check(h, "k_unpermute"));
         HIP_TRY(h, hipMemcpyAsync(dst, tmp, (size_t)cnt * sizeof(double), hipMemcpyDefault, h->stream));
         HIP_TRY(h, hipStreamSynchronize(h->stream));
         return TPSPH_OK;
     };
     if (rc == TPSPH_OK && fluid_pressure && h->nf > 0) {
-        k_record_pressure<<<nblocks(h->nf), kBlock, 0, h->stream>>>(h->nf, h->rec, pf_sorted);
+        hipLaunchKernelGGL((k_record_pressure), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, h->rec, pf_sorted);
         rc = launch_check(h, "k_record_pressure");
         if (rc == TPSPH_OK) rc = emit(fluid_pressure, pf_sorted, h->f_perm, h->nf);
     }
@@ -1361,14 +1757,14 @@ int tpsph_neighbors_csr(tpsph_handle *h, const double *u_ode, int32_t pair, int6
     }
     // fluid cell list of the current positions; the flag stays raised, so the next kick
     // rebuilds and gathers its own state again
-    k_set_i32<<<1, 1, 0, h->stream>>>(h->flag, 1);
+    hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->flag, 1);
     TRY(nd == 2 ? build_cells<2>(h, h->flag, nf, u, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm)
                 : build_cells<3>(h, h->flag, nf, u, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm));
     if (nf > 0) {
         if (nd == 2)
-            k_positions<2><<<nblocks(nf), kBlock, 0, h->stream>>>(nf, u, h->f_perm, h->xs, h->ys, nullptr, nullptr, nullptr);
+            hipLaunchKernelGGL((k_positions<2>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, nf, u, h->f_perm, h->xs, h->ys, nullptr, nullptr, nullptr);
         else
-            k_positions<3><<<nblocks(nf), kBlock, 0, h->stream>>>(nf, u, h->f_perm, h->xs, h->ys, h->zs, nullptr, nullptr);
+            hipLaunchKernelGGL((k_positions<3>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, nf, u, h->f_perm, h->xs, h->ys, h->zs, nullptr, nullptr);
         TRY(launch_check(h, "k_positions"));
     }
     h->have_kick = false;
@@ -1398,9 +1794,9 @@ int tpsph_neighbors_csr(tpsph_handle *h, const double *u_ode, int32_t pair, int6
     int rc = TPSPH_OK;
     if (nrows > 0) {
         if (nd == 2)
-            k_nbr<2, false><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, counts, nullptr, nullptr);
+            hipLaunchKernelGGL((k_nbr<2, false>), dim3(nblocks(nrows)), dim3(kBlock), 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, counts, nullptr, nullptr);
         else
-            k_nbr<3, false><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, counts, nullptr, nullptr);
+            hipLaunchKernelGGL((k_nbr<3, false>), dim3(nblocks(nrows)), dim3(kBlock), 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, counts, nullptr, nullptr);
         rc = launch_check(h, "k_nbr<count>");
         if (rc == TPSPH_OK) {
             hipError_t e = hipMemcpyAsync(hc.data(), counts, (size_t)nrows * sizeof(int), hipMemcpyDeviceToHost, h->stream);
@@ -1425,9 +1821,9 @@ int tpsph_neighbors_csr(tpsph_handle *h, const double *u_ode, int32_t pair, int6
         }
         if (rc == TPSPH_OK) {
             if (nd == 2)
-                k_nbr<2, true><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, nullptr, doff, didx);
+                hipLaunchKernelGGL((k_nbr<2, true>), dim3(nblocks(nrows)), dim3(kBlock), 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, nullptr, doff, didx);
             else
-                k_nbr<3, true><<<nblocks(nrows), kBlock, 0, h->stream>>>(h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, nullptr, doff, didx);
+                hipLaunchKernelGGL((k_nbr<3, true>), dim3(nblocks(nrows)), dim3(kBlock), 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, nullptr, doff, didx);
             rc = launch_check(h, "k_nbr<fill>");
         }
         if (rc == TPSPH_OK) {
@@ -1458,7 +1854,7 @@ int tpsph_rk_stage(tpsph_handle *h, double *y, double *tmp, const double *k, dou
         return TPSPH_OK;
     }
     TimedScope ts(h, TPSPH_TIME_STAGE);
-    k_rk_stage<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, h->stream>>>(n, y, tmp, k, a, b, dt, first_stage);
+    hipLaunchKernelGGL((k_rk_stage), dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, h->stream, n, y, tmp, k, a, b, dt, first_stage);
     return launch_check(h, "k_rk_stage");
 }
 
@@ -1497,9 +1893,9 @@ int tpsph_solve_ck2n54(tpsph_handle *h, double *v_ode, double *u_ode, double t0,
             TRY(kick_dispatch(h, h->sol_dv, v, u));
             TimedScope ts(h, TPSPH_TIME_STAGE);
             if (h->nd == 2)
-                k_ck_stage<2><<<nblocks(h->nf), kBlock, 0, h->stream>>>(h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt, i == 0);
+                hipLaunchKernelGGL((k_ck_stage<2>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt, i == 0);
             else
-                k_ck_stage<3><<<nblocks(h->nf), kBlock, 0, h->stream>>>(h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt, i == 0);
+                hipLaunchKernelGGL((k_ck_stage<3>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt, i == 0);
             TRY(launch_check(h, "k_ck_stage"));
         }
     }
@@ -1553,7 +1949,8 @@ void tpsph_destroy(tpsph_handle *h)
     (void)hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->flag, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm,
                     h->scan_tiles, h->f_mass, h->xs, h->ys, h->zs, h->snap, h->rec, h->vol,
-                    h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near, h->pw,
+                    h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
+                    h->tiles, h->ntiles, h->work, h->pw,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu};
     for (void *p : ptrs)
