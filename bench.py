@@ -32,6 +32,10 @@ def parse():
     ap.add_argument("--dt", type=float, default=1e-5)
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="CK2N54 steps of the oracle in the CPU-baseline sample (0: skip)")
+    ap.add_argument("--pair-search", default="auto", choices=["auto", "lists", "cells", "tiles"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before the run, copy MIB MiB device to device once (a plain copy "
+                         "of known size that calibrates FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -51,7 +55,8 @@ def main():
     native = tp.native
     lib = native.load()
     db = tp.dam_break(3, args.dx)
-    semi = tp.Semidiscretization(db.fluid_system, db.boundary_system)
+    semi = tp.Semidiscretization(db.fluid_system, db.boundary_system,
+                                 pair_search=args.pair_search)
     ode = tp.semidiscretize(semi, (0.0, args.dt * (args.warmup + args.steps)))
     v, u = ode.v0.clone(), ode.u0.clone()
     h = semi.handle
@@ -61,6 +66,14 @@ def main():
         native.check(lib.tpsph_solve_ck2n54(h, v.data_ptr(), u.data_ptr(), 0.0, args.dt,
                                             nsteps), h)
 
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     run(args.warmup)
     torch.cuda.synchronize()
     native.check(lib.tpsph_enable_timing(h, 1), h)
@@ -79,6 +92,10 @@ def main():
     vf = v.cpu().numpy()
     finite = bool(np.isfinite(vf).all())
 
+    search = args.pair_search
+    if search == "auto":
+        search = "lists" if semi.skin != 0.0 else "cells"
+    fluid_kernel = {"lists": "k_fluid_list", "cells": "k_fluid", "tiles": "k_fluid_tiled"}[search]
     nd = 3
     fluid_bytes = nf * (8 * nd + 8 * nd + 8 + 8 + 8 + 8 * (nd + 1))
     wall_bytes = nw * (8 * nd + 8 + 8 + 8)
@@ -134,7 +151,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_fluid (fused fluid-fluid + fluid-wall interaction)",
+            "kernel": f"{fluid_kernel} (fused fluid-fluid + fluid-wall interaction)",
             "avg_launch_us": avg_s * 1e6,
             "algorithmic_bytes_per_launch": fluid_bytes + wall_bytes,
         },

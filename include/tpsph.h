@@ -61,9 +61,17 @@ enum tpsph_pair {
     TPSPH_PAIR_WALL_FLUID = 2   /* rows: wall particles,  columns: fluid particles */
 };
 
+/* Pair search of the fluid kernel (tpsph_params.pair_search). */
+enum tpsph_pair_search {
+    TPSPH_SEARCH_AUTO = 0,   /* Verlet lists when skin > 0, else the cell walk           */
+    TPSPH_SEARCH_LISTS = 1,  /* Verlet lists rebuilt with the cell list                  */
+    TPSPH_SEARCH_CELLS = 2,  /* cell walk on every kick                                  */
+    TPSPH_SEARCH_TILES = 3   /* cell walk with LDS-staged tiles (default cell divisions) */
+};
+
 /* Kernel identifiers for tpsph_kernel_time. */
 enum tpsph_timed_kernel {
-    TPSPH_TIME_CELL_LIST = 0,   /* counting sort of the fluid cell list (all its launches) */
+    TPSPH_TIME_CELL_LIST = 0,   /* fluid cell list (counting sort) and Verlet lists        */
     TPSPH_TIME_GATHER = 1,      /* reorder into cell-sorted SoA + fluid state equation   */
     TPSPH_TIME_WALL = 2,        /* Adami pressure extrapolation + inverse state equation  */
     TPSPH_TIME_FLUID = 3,       /* fused fluid-fluid + fluid-wall interaction + gravity   */
@@ -99,6 +107,8 @@ enum tpsph_timed_kernel {
  *                                      Every kick still tests all pairs exactly against the
  *                                      current positions, so the neighbour relation is the
  *                                      reference's one whatever the skin.
+ *   pair_search                        how the fluid kernel finds candidate pairs
+ *                                      (enum tpsph_pair_search); all give the same pairs
  */
 typedef struct tpsph_params {
     int32_t ndims;                 /* 2 or 3 */
@@ -110,7 +120,7 @@ typedef struct tpsph_params {
     int32_t use_density_diffusion; /* 0/1 */
     int32_t cell_divisions;        /* 0 = default (2) */
     int32_t x_cell_divisions;      /* 0 = default (2 * cell_divisions) */
-    int32_t reserved0;
+    int32_t pair_search;           /* enum tpsph_pair_search, 0 = auto */
     double smoothing_length;
     double reference_density;
     double sound_speed;

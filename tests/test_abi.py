@@ -56,11 +56,21 @@ def _create(tp, **overrides):
     ("exponent", 0.0, "exponent"),
     ("kernel", 3, "WendlandC2"),
     ("memory", 7, "memory"),
+    ("pair_search", 4, "pair_search"),
+    ("pair_search", -1, "pair_search"),
+    ("skin", 1.0, "skin"),
+    ("cell_divisions", 5, "cell_divisions"),
 ])
 def test_create_rejects_invalid_parameters(tp, field, value, msg):
     rc, err = _create(tp, **{field: value})
     assert rc == tp.native.TPSPH_ERR_ARGUMENT
     assert msg in err
+
+
+def test_semidiscretization_rejects_unknown_pair_search(tp):
+    db = tp.dam_break(2, 0.05)
+    with pytest.raises(ValueError, match="pair_search"):
+        tp.Semidiscretization(db.fluid_system, db.boundary_system, pair_search="octree")
 
 
 def test_create_rejects_null_pointers(tp):

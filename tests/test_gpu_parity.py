@@ -93,6 +93,34 @@ def test_kick_matches_oracle(tp, orc, nd, dx, grid):
     assert rel_err(rw, rw_ref) <= RTOL
 
 
+@pytest.mark.parametrize("search", ["lists", "cells", "tiles"])
+@pytest.mark.parametrize("nd,dx", [(2, 0.01), (3, 0.02)])
+def test_kick_matches_oracle_for_every_pair_search(tp, orc, nd, dx, search):
+    """Verlet lists, the per-kick cell walk and the LDS-tiled cell walk give the oracle's
+    right-hand side."""
+    db, semi = setup_case(tp, nd, dx, grid={"pair_search": search})
+    v0, u0 = semi.initial_state()
+    v, u = v0.cpu().numpy(), u0.cpu().numpy().copy()
+    rng = np.random.default_rng(11)
+    v = v.copy()
+    v[:, :nd] = rng.normal(0.0, 0.5, size=(v.shape[0], nd))
+    dv_ref, *_ = oracle_kick(orc, semi, db, v, u)
+    check_columns(gpu_kick(tp, semi, v, u), dv_ref)
+
+
+def test_list_overflow_falls_back_to_the_cell_walk(tp, orc):
+    """List capacities come from the mean particle volume m / rho0; with 40x the masses
+    they are far too small, every list overflows and each particle walks the cells."""
+    db = tp.dam_break(3, 0.025)
+    db.fluid_system.mass = db.fluid_system.mass * 40.0
+    semi = tp.Semidiscretization(db.fluid_system, db.boundary_system,
+                                 parallelization_backend="hip", pair_search="lists")
+    v0, u0 = semi.initial_state()
+    v, u = v0.cpu().numpy(), u0.cpu().numpy()
+    dv_ref, *_ = oracle_kick(orc, semi, db, v, u)
+    check_columns(gpu_kick(tp, semi, v, u), dv_ref)
+
+
 def test_kick_host_memory_mode(tp, orc):
     db, semi = setup_case(tp, 2, 0.02, backend="hip_host")
     v, u = semi.initial_state()
@@ -174,10 +202,11 @@ def test_particles_outside_domain_are_clamped(tp, orc, nd):
     assert np.array_equal(off, roff) and np.array_equal(idx.astype(np.int64), ridx)
 
 
-def test_skin_rebuild_keeps_results_exact(tp, orc):
+@pytest.mark.parametrize("search", ["lists", "cells"])
+def test_skin_rebuild_keeps_results_exact(tp, orc, search):
     """Kicks at positions that moved less and more than half the skin after the last
-    rebuild both match the oracle (the cell order is reused, then rebuilt)."""
-    db, semi = setup_case(tp, 3, 0.025, grid={"skin": 0.2})
+    rebuild both match the oracle (the cell order and lists are reused, then rebuilt)."""
+    db, semi = setup_case(tp, 3, 0.025, grid={"skin": 0.2, "pair_search": search})
     v0, u0 = semi.initial_state()
     v, u = v0.cpu().numpy(), u0.cpu().numpy()
     R = 2 * db.fluid_system.smoothing_length

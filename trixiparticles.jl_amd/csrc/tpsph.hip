@@ -6,10 +6,12 @@
 //                    moved by more than half the skin since the last cell-list build;
 //   2. rebuild, only when flagged (every kernel reads the flag and returns otherwise):
 //      k_cells + k_scan_* + k_scatter + k_sort_cells (a stable counting sort on the linear
-//      cell index) and k_gather<1> (gather again, snapshot of the build positions);
+//      cell index), k_gather<1> (gather again, snapshot of the build positions) and, with
+//      Verlet lists, k_lists (fluid and wall candidates within R + skin);
 //   3. k_wall        Adami pressure extrapolation + inverse state equation per wall particle;
-//   4. k_fluid       fused fluid-fluid + fluid-wall interaction + gravity, results scattered
-//                    back to the caller's particle order.
+//   4. the fused fluid-fluid + fluid-wall interaction + gravity, results scattered back to
+//      the caller's particle order: k_fluid_list (Verlet lists), k_fluid (cell walk) or
+//      k_fluid_tiled (cell walk over LDS-staged tiles).
 // The wall cell list is static and built once at create.
 //
 // Neighbour search.  Cells have edge R/k (k = cell_divisions, R = compact support 2h).  For
@@ -25,7 +27,9 @@
 // Pair evaluation.  k_fluid is two-phase per wavefront: phase 1 runs the exact radius test
 // over the candidates and appends hits to a per-lane list in LDS (kCap entries per lane);
 // phase 2 evaluates the pair terms on the dense lists.  Phases alternate until every lane
-// has exhausted its pencils, so any neighbour count is handled.
+// has exhausted its pencils, so any neighbour count is handled.  k_fluid_list streams the
+// Verlet lists instead (about 1.1x the neighbour count per particle, read as contiguous
+// 256-byte rows per wavefront) and applies the same exact test to every entry.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -70,7 +74,9 @@ struct DevParams {
     int ncells;
     double hs_fluid;  // half skin + rounding margin (fluid cells)
     double hs_wall;   // rounding margin (static wall cells)
-    double half_skin;
+    double half_skin, half_skin2;
+    // Verlet lists: squared list radii and the cell growth that makes the pencils reach them
+    double rl2_fluid, rl2_wall, hs_list_fluid, hs_list_wall;
 };
 
 // Cell-sorted fluid data.  rec[2s], rec[2s+1] = {x, y, z, rho}, {vx, vy, vz, p} (one 64-byte
@@ -381,10 +387,15 @@ __global__ void k_gather(int *flag, DevParams P, const int *__restrict__ perm,
     rec[2 * (size_t)s + 1] = make_double4(vv[0], vv[1], vv[2], state_equation(P, rho));
     vol[s] = make_double2(mass / rho, mass);
     if constexpr (MODE == 0) {
-        bool moved = false;
+        // Euclidean displacement since the build (bounds the per-axis drift the grown cells
+        // allow for, and the drift the Verlet lists allow for)
+        double e2 = 0.0;
 #pragma unroll
-        for (int d = 0; d < ND; d++) moved |= !(fabs(x[d] - snap[d * nf + s]) <= P.half_skin);
-        if (moved) atomicOr(flag, 1);
+        for (int d = 0; d < ND; d++) {
+            const double e = x[d] - snap[d * nf + s];
+            e2 = fma(e, e, e2);
+        }
+        if (!(e2 <= P.half_skin2)) atomicOr(flag, 1);
     } else {
 #pragma unroll
         for (int d = 0; d < ND; d++) snap[d * nf + s] = x[d];
@@ -679,6 +690,123 @@ __global__ void k_wall_near(DevParams P, int nw, const double4 *__restrict__ wpo
     for (int cz = z0; cz <= z1; cz++)
         for (int cy = y0; cy <= y1; cy++)
             for (int cx = x0; cx <= x1; cx++) near[cx + P.n[0] * (cy + P.n[1] * cz)] = 1;
+}
+
+// ------------------------------------------------------------------------------------------
+// Verlet lists, rebuilt together with the cell list.  Until the next rebuild no particle
+// moves by more than half the skin, so a fluid pair within R now was within R + skin at the
+// build and a fluid-wall pair within R + skin/2 (walls do not move).  The lists hold those
+// candidates; every kick still applies the exact radius test, so the neighbour sets are the
+// same as those of a full search.  Entry k of sorted particle s sits at k * ls + s: the 64
+// lanes of a wavefront read one contiguous 256-byte row per slot.  A particle with more
+// candidates than the capacity stores count -1 and walks the cells in the kick instead.
+// ------------------------------------------------------------------------------------------
+template <int ND, bool WALL>
+__global__ void __launch_bounds__(kBlock)
+    k_lists(const int *flag, DevParams P, FluidView F, WallView Wv, int ls, int cap,
+            int *__restrict__ nbr, int *__restrict__ cnt)
+{
+    if (skip(flag)) return;
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= P.nf) return;
+    const double xa = F.x[s], ya = F.y[s], za = (ND == 3) ? F.z[s] : 0.0;
+    const int *start = WALL ? Wv.start : F.start;
+    const double *cx = WALL ? Wv.x : F.x;
+    const double *cy = WALL ? Wv.y : F.y;
+    const double *cz = WALL ? Wv.z : F.z;
+    const double rl2 = WALL ? P.rl2_wall : P.rl2_fluid;
+    Pencils<ND> it;
+    it.init(P, xa, ya, za, WALL ? P.hs_list_wall : P.hs_list_fluid);
+    int n = 0, j, je;
+    while (it.next(P, start, j, je)) {
+        for (; j < je; j++) {
+            const double d2 = dist2<ND>(xa - cx[j], ya - cy[j], (ND == 3) ? za - cz[j] : 0.0);
+            if (d2 <= rl2 && (WALL || j != s)) {
+                if (n < cap) nbr[(size_t)n * ls + s] = j;
+                n++;
+            }
+        }
+    }
+    cnt[s] = n <= cap ? n : -1;
+}
+
+// Fluid kernel over the Verlet lists.  Candidates are loaded one slot ahead of the pair
+// terms; the exact radius test (and the zero-distance skip) selects the pairs.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_fluid_list(DevParams P, FluidView F, WallView Wv, const int *__restrict__ nbr,
+                 const int *__restrict__ cnt, const int *__restrict__ wnbr,
+                 const int *__restrict__ wcnt, int ls, double *__restrict__ dv)
+{
+    __shared__ int lists[kCap * kBlock];  // only for particles whose list overflowed
+    int *lst = lists + threadIdx.x;
+    const int s = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool active = s < P.nf;
+    const int sl = active ? s : 0;  // in-bounds index for the inactive lanes' loads
+    double xa[3] = {0.0, 0.0, 0.0}, va[3] = {0.0, 0.0, 0.0}, rho_a = 1.0, p_a = 0.0;
+    int n = 0, nw = 0;
+    if (active) {
+        const double4 a = F.rec[2 * (size_t)s];
+        const double4 b = F.rec[2 * (size_t)s + 1];
+        xa[0] = a.x;
+        xa[1] = a.y;
+        xa[2] = a.z;
+        rho_a = a.w;
+        va[0] = b.x;
+        va[1] = b.y;
+        va[2] = b.z;
+        p_a = b.w;
+        n = cnt[s];
+        if (P.nw > 0) nw = wcnt[s];
+    }
+    const double inv_rho_a = 1.0 / rho_a;
+
+    double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
+    {
+        const int m = max(n, 0);
+        int j = (m > 0) ? nbr[sl] : sl;
+        FluidNbr cur = load_fluid(F, j);
+        for (int k = 0; k < m; k++) {
+            const int jn = (k + 1 < m) ? nbr[(size_t)(k + 1) * ls + sl] : j;
+            const FluidNbr nxt = load_fluid(F, jn);
+            const double p0 = xa[0] - cur.a.x, p1 = xa[1] - cur.a.y;
+            const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
+            const double d2 = dist2<ND>(p0, p1, p2);
+            if (d2 <= P.R2 && d2 >= P.az2)
+                pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+            cur = nxt;
+            j = jn;
+        }
+    }
+    if (__any(n < 0))
+        neighbour_sweep<ND, false>(P, F.start, F.x, F.y, F.z, P.hs_fluid, n < 0, F, Wv, lst,
+                                   xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+
+    double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
+    if (__any(nw != 0)) {
+        const int m = max(nw, 0);
+        int j = (m > 0) ? wnbr[sl] : 0;
+        WallNbr cur = load_wall(Wv, j);
+        for (int k = 0; k < m; k++) {
+            const int jn = (k + 1 < m) ? wnbr[(size_t)(k + 1) * ls + sl] : j;
+            const WallNbr nxt = load_wall(Wv, jn);
+            const double p0 = xa[0] - cur.a.x, p1 = xa[1] - cur.a.y;
+            const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
+            const double d2 = dist2<ND>(p0, p1, p2);
+            if (d2 <= P.R2 && d2 >= P.az2)
+                pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+            cur = nxt;
+            j = jn;
+        }
+        if (__any(nw < 0))
+            neighbour_sweep<ND, true>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0, F, Wv,
+                                      lst, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+    }
+    if (!active) return;
+    const int i = F.perm[s];
+#pragma unroll
+    for (int d = 0; d < ND; d++) dv[(size_t)i * (ND + 1) + d] = (acc[d] + accw[d]) + P.g[d];
+    dv[(size_t)i * (ND + 1) + ND] = drho + drhow;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1195,6 +1323,10 @@ struct tpsph_handle {
     double4 *wpos = nullptr;
     double2 *wpv = nullptr;
     unsigned char *wall_near = nullptr;  // per cell: a wall particle lies within reach
+    int search = TPSPH_SEARCH_CELLS;  // resolved pair search
+    // Verlet lists (slot-major, ls slots per row) and their counts
+    int *nbr = nullptr, *cnt = nullptr, *wnbr = nullptr, *wcnt = nullptr;
+    int ls = 0, cap_f = 0, cap_w = 0;
     // tiled fluid kernel
     bool tiled = false;
     int tgx = 1, tgy = 1, ntiles_all = 0, n_cu = 256;
@@ -1332,7 +1464,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
         TimedScope t(h, TPSPH_TIME_CELL_LIST);
         TRY(build_cells<ND>(h, h->flag, nf, u, h->f_cell, h->f_rank, h->f_count, h->f_start,
                             h->f_perm));
-        if (h->tiled) {
+        if (h->search == TPSPH_SEARCH_TILES) {
             hipLaunchKernelGGL((k_fill_i32), dim3(1), dim3(1), 0, h->stream, h->flag, h->ntiles, 0, 1);
             hipLaunchKernelGGL((k_tiles<ND>), dim3(nblocks(h->ntiles_all)), dim3(kBlock), 0, h->stream, 
                 h->flag, h->P, h->f_start, h->tgx, h->tgy, h->ntiles_all, h->tiles, h->ntiles);
@@ -1340,6 +1472,14 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
         }
         hipLaunchKernelGGL((k_gather<ND, 1>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, 
             h->flag, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
+        if (h->search == TPSPH_SEARCH_LISTS) {
+            hipLaunchKernelGGL((k_lists<ND, false>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
+                h->flag, h->P, h->fluid_view(), h->wall_view(), h->ls, h->cap_f, h->nbr, h->cnt);
+            if (h->nw > 0)
+                hipLaunchKernelGGL((k_lists<ND, true>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
+                    h->flag, h->P, h->fluid_view(), h->wall_view(), h->ls, h->cap_w, h->wnbr, h->wcnt);
+            TRY(launch_check(h, "k_lists"));
+        }
         hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->flag, 0);
         TRY(launch_check(h, "k_gather<1>"));
     }
@@ -1359,7 +1499,10 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u)
     }
     if (nf > 0) {
         TimedScope t(h, TPSPH_TIME_FLUID);
-        if (h->tiled) {
+        if (h->search == TPSPH_SEARCH_LISTS) {
+            hipLaunchKernelGGL((k_fluid_list<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
+                h->P, h->fluid_view(), h->wall_view(), h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
+        } else if (h->search == TPSPH_SEARCH_TILES) {
             hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->work, 0);
             const int grid = std::max(1, std::min(h->ntiles_all, 2 * h->n_cu));
             hipLaunchKernelGGL((k_fluid_tiled<ND>), dim3(grid), dim3(TileCfg<ND>::THREADS), 0, h->stream, 
@@ -1415,6 +1558,8 @@ int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, co
     if (p->cell_divisions < 0 || p->cell_divisions > 4) return bad("cell_divisions must be in 0..4");
     if (p->x_cell_divisions < 0 || p->x_cell_divisions > 8)
         return bad("x_cell_divisions must be in 0..8");
+    if (p->pair_search < TPSPH_SEARCH_AUTO || p->pair_search > TPSPH_SEARCH_TILES)
+        return bad("pair_search must be one of enum tpsph_pair_search");
     if (!(p->skin < 1.0)) return bad("skin must be below 1 (a fraction of the compact support)");
     if (nf > 0 && !fm) return bad("fluid_mass is NULL");
     if (nw > 0 && (!wx || !wm)) return bad("wall_coordinates / wall_mass is NULL");
@@ -1526,8 +1671,16 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     // cell-interval rounding margin: far above the error of floor((x - lo) / cs)
     const double margin = 1e-9 * R + 64.0 * DBL_EPSILON * extent;
     P.half_skin = 0.5 * skin * R;
+    P.half_skin2 = P.half_skin * P.half_skin;
     P.hs_fluid = P.half_skin + margin;
     P.hs_wall = margin;
+    {
+        const double rlf = R + 2.0 * P.half_skin + margin, rlw = R + P.half_skin + margin;
+        P.rl2_fluid = rlf * rlf;
+        P.rl2_wall = rlw * rlw;
+        P.hs_list_fluid = 2.0 * P.half_skin + 2.0 * margin;
+        P.hs_list_wall = P.half_skin + 2.0 * margin;
+    }
     // k cells span the reach of a point: R (as bounded by the FP32 square roots of the pencil
     // widths, which run at most 2e-6 R above it), its own drift and a neighbour's drift
     const double span = (R * 1.00001 + 2.0 * P.hs_fluid) * (1.0 + 1e-9);
@@ -1559,6 +1712,42 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     // tiled fluid kernel: needs the default cell divisions (its halo widths)
     h->tiled = (nd == 3) ? (div == TileCfg<3>::HY && xdiv == TileCfg<3>::HX)
                          : (div == TileCfg<2>::HY && xdiv == TileCfg<2>::HX);
+    h->search = params->pair_search;
+    if (h->search == TPSPH_SEARCH_AUTO)
+        h->search = P.half_skin > 0.0 ? TPSPH_SEARCH_LISTS : TPSPH_SEARCH_CELLS;
+    if (h->search == TPSPH_SEARCH_TILES && !h->tiled) {
+        delete h;
+        return fail(nullptr, TPSPH_ERR_ARGUMENT,
+                    "pair_search TPSPH_SEARCH_TILES needs the default cell divisions");
+    }
+    if (h->search == TPSPH_SEARCH_LISTS) {
+        // capacities: 1.5 x the expected count in the list ball (particle volume m / rho0)
+        // plus 16; walls fill at most 7/8 (3D) or 3/4 (2D) of the ball around a corner
+        std::vector<double> mh((size_t)std::max(h->nf, h->nw));
+        auto mean = [&](const double *src, int n) {
+            if (n == 0) return 0.0;
+            if (hipMemcpy(mh.data(), src, (size_t)n * sizeof(double),
+                          host ? hipMemcpyHostToHost : hipMemcpyDeviceToHost) != hipSuccess)
+                return -1.0;
+            double acc = 0.0;
+            for (int i = 0; i < n; i++) acc += mh[i];
+            return acc / n;
+        };
+        const double mf = mean(fluid_mass, h->nf), mw = mean(wall_mass, h->nw);
+        if (mf < 0.0 || mw < 0.0) {
+            delete h;
+            return fail(nullptr, TPSPH_ERR_HIP, "copy of the masses failed");
+        }
+        auto ball = [&](double r) { return nd == 2 ? M_PI * r * r : 4.0 / 3.0 * M_PI * r * r * r; };
+        auto cap = [&](double expected) {
+            const double c = std::ceil(1.5 * expected) + 16.0;
+            return (int)std::min(4096.0, std::ceil(c / 8.0) * 8.0);
+        };
+        const double rlf = std::sqrt(P.rl2_fluid), rlw = std::sqrt(P.rl2_wall);
+        h->cap_f = cap(mf > 0.0 ? ball(rlf) / (mf / P.rho0) : 0.0);
+        h->cap_w = h->nw > 0 ? cap(mw > 0.0 ? (nd == 2 ? 0.75 : 0.875) * ball(rlw) / (mw / P.rho0) : 0.0) : 0;
+        h->ls = (h->nf + 63) / 64 * 64;
+    }
     if (nd == 3) {
         h->tgx = (P.n[0] + TileCfg<3>::TX - 1) / TileCfg<3>::TX;
         h->tgy = (P.n[1] + TileCfg<3>::TY - 1) / TileCfg<3>::TY;
@@ -1605,6 +1794,14 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     CTRY(dalloc(h, &h->wy, (size_t)nw + kPad));
     CTRY(dalloc(h, &h->wz, (size_t)nw + kPad));
     CTRY(dalloc(h, &h->wall_near, (size_t)ncells));
+    if (h->search == TPSPH_SEARCH_LISTS) {
+        CTRY(dalloc(h, &h->nbr, (size_t)h->cap_f * h->ls));
+        CTRY(dalloc(h, &h->cnt, nf));
+        if (nw > 0) {
+            CTRY(dalloc(h, &h->wnbr, (size_t)h->cap_w * h->ls));
+            CTRY(dalloc(h, &h->wcnt, nf));
+        }
+    }
     CTRY(dalloc(h, &h->tiles, (size_t)h->ntiles_all));
     CTRY(dalloc(h, &h->ntiles, 1));
     CTRY(dalloc(h, &h->work, 1));
@@ -1950,7 +2147,7 @@ void tpsph_destroy(tpsph_handle *h)
     void *ptrs[] = {h->flag, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm,
                     h->scan_tiles, h->f_mass, h->xs, h->ys, h->zs, h->snap, h->rec, h->vol,
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
-                    h->tiles, h->ntiles, h->work, h->pw,
+                    h->tiles, h->ntiles, h->work, h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu};
     for (void *p : ptrs)

@@ -23,6 +23,13 @@ PAIR_FLUID_FLUID = 0
 PAIR_FLUID_WALL = 1
 PAIR_WALL_FLUID = 2
 
+SEARCH_AUTO = 0
+SEARCH_LISTS = 1
+SEARCH_CELLS = 2
+SEARCH_TILES = 3
+PAIR_SEARCH = {"auto": SEARCH_AUTO, "lists": SEARCH_LISTS, "cells": SEARCH_CELLS,
+               "tiles": SEARCH_TILES}
+
 TIME_CELL_LIST = 0
 TIME_GATHER = 1
 TIME_WALL = 2
@@ -50,7 +57,7 @@ class TpsphParams(C.Structure):
         ("use_density_diffusion", C.c_int32),
         ("cell_divisions", C.c_int32),
         ("x_cell_divisions", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("pair_search", C.c_int32),
         ("smoothing_length", C.c_double),
         ("reference_density", C.c_double),
         ("sound_speed", C.c_double),

@@ -61,13 +61,14 @@ class Semidiscretization:
     ``FullGridCellList`` box).  By default it is the bounding box of both systems, grown
     by ``domain_margin`` times its extent on every side.
 
-    ``cell_divisions`` / ``x_cell_divisions`` (cells per compact support along y, z / x) and
-    ``skin`` (fraction of the compact support) tune the device cell list; they change speed,
-    never results' neighbour relation."""
+    ``cell_divisions`` / ``x_cell_divisions`` (cells per compact support along y, z / x),
+    ``skin`` (fraction of the compact support) and ``pair_search`` ("auto", "lists",
+    "cells" or "tiles") tune the device neighbour search; they change speed, never the
+    neighbour relation."""
 
     def __init__(self, *systems, neighborhood_search=None, parallelization_backend="hip",
                  domain=None, domain_margin=0.05, device=0, cell_divisions=0,
-                 x_cell_divisions=0, skin=-1.0):
+                 x_cell_divisions=0, skin=-1.0, pair_search="auto"):
         fluids = [s for s in systems if isinstance(s, WeaklyCompressibleSPHSystem)]
         walls = [s for s in systems if isinstance(s, WallBoundarySystem)]
         if len(fluids) != 1 or len(walls) > 1 or len(fluids) + len(walls) != len(systems):
@@ -105,6 +106,9 @@ class Semidiscretization:
         self.cell_divisions = int(cell_divisions)
         self.x_cell_divisions = int(x_cell_divisions)
         self.skin = float(skin)
+        if pair_search not in native.PAIR_SEARCH:
+            raise ValueError(f"pair_search must be one of {sorted(native.PAIR_SEARCH)}")
+        self.pair_search = pair_search
         self._handle = None
 
     # -- parameters of the C ABI
@@ -118,6 +122,7 @@ class Semidiscretization:
         p.cell_divisions = self.cell_divisions
         p.x_cell_divisions = self.x_cell_divisions
         p.skin = self.skin
+        p.pair_search = native.PAIR_SEARCH[self.pair_search]
         p.clip_fluid_eos = int(se.clip_negative_pressure)
         p.smoothing_length = f.smoothing_length
         p.reference_density = se.reference_density
