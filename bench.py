@@ -95,7 +95,8 @@ def main():
     search = args.pair_search
     if search == "auto":
         search = "lists" if semi.skin != 0.0 else "cells"
-    fluid_kernel = {"lists": "k_fluid_list", "cells": "k_fluid", "tiles": "k_fluid_tiled"}[search]
+    # with Verlet lists and the default cell divisions the fluid kernel stages bricks in LDS
+    fluid_kernel = {"lists": "k_fluid_brick", "cells": "k_fluid", "tiles": "k_fluid_tiled"}[search]
     nd = 3
     fluid_bytes = nf * (8 * nd + 8 * nd + 8 + 8 + 8 + 8 * (nd + 1))
     wall_bytes = nw * (8 * nd + 8 + 8 + 8)

@@ -4,7 +4,8 @@
 //   1. k_gather<0>   gather the fluid state into cell-sorted order with the current
 //                    permutation, fluid state equation, and flag a rebuild if any particle
 //                    moved by more than half the skin since the last cell-list build;
-//   2. rebuild, only when flagged (every kernel reads the flag and returns otherwise):
+//   2. rebuild, only when flagged (every kernel reads the flag and returns otherwise; the
+//      fluid kernel of step 4 lowers the flag again):
 //      k_cells + k_scan_* + k_scatter + k_sort_cells (a stable counting sort on the linear
 //      cell index), k_gather<1> (gather again, snapshot of the build positions) and, with
 //      Verlet lists, k_lists (fluid and wall candidates within R + skin);
@@ -34,6 +35,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -49,7 +51,12 @@ constexpr int kBlock = 256;  // 4 wavefronts of 64
 constexpr int kCap = 32;     // neighbour-list entries per lane and phase
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
-constexpr int kMaxGridBlocks = 4096;  // grid-stride kernels over the cell grid
+constexpr int kBrickThreads = 1024;   // threads of the brick kernel (one block per CU), two
+                                      // per target
+constexpr int kBrickCap = 2048;       // particles a brick can stage in LDS
+constexpr int kBrickMaxRows = 256;    // union rows per brick (one scan block)
+constexpr int kGateBlocks = 1024;     // grid-stride kernels behind the rebuild flag: a small
+                                      // grid keeps the launch cheap when the flag is down
 constexpr int kPad = 4;               // NaN entries after the SoA coordinate arrays
 
 thread_local std::string g_create_error;
@@ -77,6 +84,12 @@ struct DevParams {
     double half_skin, half_skin2;
     // Verlet lists: squared list radii and the cell growth that makes the pencils reach them
     double rl2_fluid, rl2_wall, hs_list_fluid, hs_list_wall;
+    // bricks of cells (processing order of the list kernels): cells per brick, bricks per axis
+    int bc[3], nb[3];
+    // union rows of a brick: cells beyond the brick along y and z, rows per brick, and the
+    // radius (list radius + margin) and cell growth (margin) of the union
+    int ur[3], maxr;
+    double rl_union2, hs_union;
 };
 
 // Cell-sorted fluid data.  rec[2s], rec[2s+1] = {x, y, z, rho}, {vx, vy, vz, p} (one 64-byte
@@ -235,15 +248,15 @@ __global__ void k_cells(const int *flag, DevParams P, int n, const double *__res
                         int *__restrict__ cell, int *__restrict__ rank, int *__restrict__ count)
 {
     if (skip(flag)) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double *xi = x + (size_t)i * ND;
-    int c0 = cell_1d(xi[0], P.lo[0], P.inv_cs[0], P.n[0]);
-    int c1 = cell_1d(xi[1], P.lo[1], P.inv_cs[1], P.n[1]);
-    int c2 = (ND == 3) ? cell_1d(xi[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
-    int lin = c0 + P.n[0] * (c1 + P.n[1] * c2);
-    cell[i] = lin;
-    rank[i] = atomicAdd(&count[lin], 1);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double *xi = x + (size_t)i * ND;
+        int c0 = cell_1d(xi[0], P.lo[0], P.inv_cs[0], P.n[0]);
+        int c1 = cell_1d(xi[1], P.lo[1], P.inv_cs[1], P.n[1]);
+        int c2 = (ND == 3) ? cell_1d(xi[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+        int lin = c0 + P.n[0] * (c1 + P.n[1] * c2);
+        cell[i] = lin;
+        rank[i] = atomicAdd(&count[lin], 1);
+    }
 }
 
 __device__ __forceinline__ int block_exclusive_scan(int value, int *s, int &total)
@@ -319,14 +332,18 @@ __global__ void k_scan_add(const int *flag, int *out, const int *tile_sums, int 
         if (i < n) out[i] += add;
 }
 
+// Also clears the histogram again: the counts are zero between builds, so the next build
+// needs no separate clearing pass.
 __global__ void k_scatter(const int *flag, int n, const int *__restrict__ cell,
                           const int *__restrict__ rank, const int *__restrict__ start,
-                          int *__restrict__ perm)
+                          int *__restrict__ perm, int *__restrict__ count)
 {
     if (skip(flag)) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    perm[start[cell[i]] + rank[i]] = i;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c = cell[i];
+        perm[start[c] + rank[i]] = i;
+        count[c] = 0;
+    }
 }
 
 // Order every cell's members by particle index: the result is the stable counting sort,
@@ -351,6 +368,40 @@ __global__ void k_sort_cells(const int *flag, int ncells, const int *__restrict_
 
 __global__ void k_set_i32(int *p, int value) { *p = value; }
 
+// Processing order of the list kernel: the cell-sorted particles grouped by bricks of cells
+// about 2R wide (4R in 2D), so the 64 targets of a wavefront are a compact cluster and the
+// union of their neighbourhoods is small.  The order inside a brick follows the histogram
+// atomics; it only decides which lane handles which particle, never a result.
+template <int ND>
+__global__ void k_brick_count(const int *flag, DevParams P, const double *__restrict__ xs,
+                              const double *__restrict__ ys, const double *__restrict__ zs,
+                              int *__restrict__ bid, int *__restrict__ rank,
+                              int *__restrict__ bcount)
+{
+    if (skip(flag)) return;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < P.nf; s += gridDim.x * blockDim.x) {
+        const int c0 = cell_1d(xs[s], P.lo[0], P.inv_cs[0], P.n[0]);
+        const int c1 = cell_1d(ys[s], P.lo[1], P.inv_cs[1], P.n[1]);
+        const int c2 = (ND == 3) ? cell_1d(zs[s], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+        const int b = c0 / P.bc[0] + P.nb[0] * (c1 / P.bc[1] + P.nb[1] * (c2 / P.bc[2]));
+        bid[s] = b;
+        rank[s] = atomicAdd(&bcount[b], 1);
+    }
+}
+
+// ord[t] = sorted index of the t-th particle in brick order; clears the brick histogram.
+__global__ void k_brick_scatter(const int *flag, int n, const int *__restrict__ bid,
+                                const int *__restrict__ rank, const int *__restrict__ bstart,
+                                int *__restrict__ ord, int *__restrict__ bcount)
+{
+    if (skip(flag)) return;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const int b = bid[s];
+        ord[bstart[b] + rank[s]] = s;
+        bcount[b] = 0;
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // gather into cell-sorted order + fluid state equation (wcsph/system.jl:426-438)
 //   MODE 0: runs when no rebuild is pending; flags one if a particle left its skin.  A block
@@ -359,7 +410,7 @@ __global__ void k_set_i32(int *p, int value) { *p = value; }
 //   MODE 1: runs after a rebuild; stores the build positions.
 // ------------------------------------------------------------------------------------------
 template <int ND, int MODE>
-__global__ void k_gather(int *flag, DevParams P, const int *__restrict__ perm,
+__global__ void k_gather(int *flag, int *work, DevParams P, const int *__restrict__ perm,
                          const double *__restrict__ u, const double *__restrict__ v,
                          const double *__restrict__ m, double *__restrict__ xs,
                          double *__restrict__ ys, double *__restrict__ zs,
@@ -367,10 +418,11 @@ __global__ void k_gather(int *flag, DevParams P, const int *__restrict__ perm,
                          double *__restrict__ snap)
 {
     const int f = *flag;
+    // MODE 0 runs first in every kick: it resets the work counter of the brick kernel
+    if (MODE == 0 && work && blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
     if ((MODE == 0) == (f != 0)) return;
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= P.nf) return;
     const size_t nf = (size_t)P.nf;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < P.nf; s += gridDim.x * blockDim.x) {
     const int i = perm[s];
     double x[3] = {0.0, 0.0, 0.0}, vv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -400,38 +452,55 @@ __global__ void k_gather(int *flag, DevParams P, const int *__restrict__ perm,
 #pragma unroll
         for (int d = 0; d < ND; d++) snap[d * nf + s] = x[d];
     }
+    }
 }
 
 // ------------------------------------------------------------------------------------------
 // Adami pressure extrapolation (dummy_particles.jl:489-569): one thread per wall particle in
 // wall-cell order over its fluid neighbours within 2h, then the inverse state equation.
 // ------------------------------------------------------------------------------------------
+// Contribution of fluid particle j (sorted index) to the sums of the wall particle at xw.
+template <int ND>
+__device__ __forceinline__ void adami_term(const DevParams &P, const FluidView &F,
+                                           const double4 &xw, int j, double &pressure,
+                                           double &volume)
+{
+    const double4 a = F.rec[2 * (size_t)j];
+    const double p0 = xw.x - a.x, p1 = xw.y - a.y;
+    const double p2 = (ND == 3) ? xw.z - a.z : 0.0;
+    const double d2 = dist2<ND>(p0, p1, p2);
+    if (d2 > P.R2) return;
+    const double W = wendland_c2(P, sqrt(d2));
+    const double rho_f = a.w;
+    const double p_f = F.rec[2 * (size_t)j + 1].w;
+    double hydro = P.g[0] * (rho_f * p0) + P.g[1] * (rho_f * p1);
+    if constexpr (ND == 3) hydro += P.g[2] * (rho_f * p2);
+    pressure += ((P.p_offset + p_f) + hydro) * W;
+    volume += W;
+}
+
+// alist / acnt: the wall particle's Verlet list of fluid candidates (slot-major, row length
+// lw), or nullptr for the cell walk.  A count of -1 (overflow) also walks the cells.
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
     k_wall(DevParams P, FluidView F, const double4 *__restrict__ wpos, double *__restrict__ pw,
-           double *__restrict__ rhow, double2 *__restrict__ wpv)
+           double *__restrict__ rhow, double2 *__restrict__ wpv, const int *__restrict__ alist,
+           const int *__restrict__ acnt, int lw)
 {
     int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= P.nw) return;
     const double4 xw = wpos[w];
-    Pencils<ND> it;
-    it.init(P, xw.x, xw.y, xw.z, P.hs_fluid);
     double pressure = 0.0, volume = 0.0;
-    int jb, je;
-    while (it.next(P, F.start, jb, je)) {
-        for (int j = jb; j < je; j++) {
-            const double p0 = xw.x - F.x[j], p1 = xw.y - F.y[j];
-            const double p2 = (ND == 3) ? xw.z - F.z[j] : 0.0;
-            const double d2 = dist2<ND>(p0, p1, p2);
-            if (d2 > P.R2) continue;
-            const double W = wendland_c2(P, sqrt(d2));
-            const double rho_f = F.rec[2 * (size_t)j].w;
-            const double p_f = F.rec[2 * (size_t)j + 1].w;
-            double hydro = P.g[0] * (rho_f * p0) + P.g[1] * (rho_f * p1);
-            if constexpr (ND == 3) hydro += P.g[2] * (rho_f * p2);
-            pressure += ((P.p_offset + p_f) + hydro) * W;
-            volume += W;
-        }
+    const int n = alist ? acnt[w] : -1;
+    if (n >= 0) {
+        for (int k = 0; k < n; k++)
+            adami_term<ND>(P, F, xw, alist[(size_t)k * lw + w], pressure, volume);
+    } else {
+        Pencils<ND> it;
+        it.init(P, xw.x, xw.y, xw.z, P.hs_fluid);
+        int jb, je;
+        while (it.next(P, F.start, jb, je))
+            for (int j = jb; j < je; j++) adami_term<ND>(P, F, xw, j, pressure, volume);
     }
     if (volume > DBL_EPSILON) pressure /= volume;
     if (P.clip_wall) pressure = julia_max0(pressure);
@@ -558,6 +627,45 @@ __device__ __forceinline__ void pair_wall(const DevParams &P, const WallNbr &n,
     drho = fma(n.pv.y * kdr, rho_a * vr, drho);
 }
 
+// pair_fluid without branches, for candidates evaluated in groups: an out-of-range (or
+// invalid) candidate gets kdr = 0, so its terms are exact zeros, and the viscosity term is
+// selected by the sign of v_ab.r_ab.  For an in-range candidate every operation is the one
+// of pair_fluid, so the sums agree bit for bit.
+template <int ND>
+__device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const FluidNbr &n,
+                                                  bool valid, const double *xa,
+                                                  const double *va, double rho_a, double p_a,
+                                                  double inv_rho_a, double *acc, double &drho)
+{
+    const double p0 = xa[0] - n.a.x, p1 = xa[1] - n.a.y, p2 = (ND == 3) ? xa[2] - n.a.z : 0.0;
+    const double d2 = dist2<ND>(p0, p1, p2);
+    const bool in = valid & (d2 <= P.R2) & (d2 >= P.az2);
+    const double d2s = in ? d2 : P.R2;  // a normal argument for the square root
+    const double r = sqrt_pos(d2s);
+    const double w = fma(-0.5 * P.h_inv, r, 1.0);
+    const double kdr = in ? P.kgrad_c * (w * w * w) : 0.0;
+    const double v0 = va[0] - n.b.x, v1 = va[1] - n.b.y, v2 = (ND == 3) ? va[2] - n.b.z : 0.0;
+    double vr = v0 * p0 + v1 * p1;
+    if constexpr (ND == 3) vr += v2 * p2;
+    const double rho_b = n.a.w;
+    double fac = -n.v.x * inv_rho_a * (p_a + n.b.w);
+    if (P.use_visc) {
+        const double q = d2s + P.eps_h2;
+        const double sum_rho = rho_a + rho_b;
+        double fv = fma(2.0 * P.av_c * n.v.y * vr, rcp_pos(q * sum_rho), fac);
+        if (P.beta != 0.0) {
+            const double mu = P.h * vr * rcp_pos(q);
+            fv += 2.0 * n.v.y * P.beta * (mu * mu) * rcp_pos(sum_rho);
+        }
+        fac = vr < 0.0 ? fv : fac;
+    }
+    fac *= kdr;
+    acc[0] = fma(fac, p0, acc[0]);
+    acc[1] = fma(fac, p1, acc[1]);
+    if constexpr (ND == 3) acc[2] = fma(fac, p2, acc[2]);
+    drho = fma(n.v.x * kdr, fma(P.dd2, rho_a - rho_b, rho_a * vr), drho);
+}
+
 // Two-phase sweep over the neighbours of one target (see the file header).  lst points at
 // this lane's column of the block's LDS lists (stride kBlock).  Phase 1 tests four
 // candidates per iteration without branches: the arrays cx/cy/cz carry kPad NaN entries
@@ -626,9 +734,10 @@ __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *_
 
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
-    k_fluid(DevParams P, FluidView F, WallView Wv, const unsigned char *__restrict__ wall_near,
-            double *__restrict__ dv)
+    k_fluid(int *flag, DevParams P, FluidView F, WallView Wv,
+            const unsigned char *__restrict__ wall_near, double *__restrict__ dv)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;  // the rebuild (if any) is done
     __shared__ int lists[kCap * kBlock];
     int *lst = lists + threadIdx.x;
     const int s = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -702,13 +811,10 @@ __global__ void k_wall_near(DevParams P, int nw, const double4 *__restrict__ wpo
 // candidates than the capacity stores count -1 and walks the cells in the kick instead.
 // ------------------------------------------------------------------------------------------
 template <int ND, bool WALL>
-__global__ void __launch_bounds__(kBlock)
-    k_lists(const int *flag, DevParams P, FluidView F, WallView Wv, int ls, int cap,
-            int *__restrict__ nbr, int *__restrict__ cnt)
+__device__ __forceinline__ void build_list(const DevParams &P, const FluidView &F,
+                                           const WallView &Wv, int s, int col, int ls, int cap,
+                                           int *__restrict__ nbr, int *__restrict__ cnt)
 {
-    if (skip(flag)) return;
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= P.nf) return;
     const double xa = F.x[s], ya = F.y[s], za = (ND == 3) ? F.z[s] : 0.0;
     const int *start = WALL ? Wv.start : F.start;
     const double *cx = WALL ? Wv.x : F.x;
@@ -722,27 +828,94 @@ __global__ void __launch_bounds__(kBlock)
         for (; j < je; j++) {
             const double d2 = dist2<ND>(xa - cx[j], ya - cy[j], (ND == 3) ? za - cz[j] : 0.0);
             if (d2 <= rl2 && (WALL || j != s)) {
-                if (n < cap) nbr[(size_t)n * ls + s] = j;
+                if (n < cap) nbr[(size_t)n * ls + col] = j;
                 n++;
             }
         }
     }
-    cnt[s] = n <= cap ? n : -1;
+    cnt[col] = n <= cap ? n : -1;
 }
 
-// Fluid kernel over the Verlet lists.  Candidates are loaded one slot ahead of the pair
-// terms; the exact radius test (and the zero-distance skip) selects the pairs.
+// Fluid and wall lists of every fluid particle (the wall lists only with walls), stored in
+// column t for the particle ord[t].
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
-    k_fluid_list(DevParams P, FluidView F, WallView Wv, const int *__restrict__ nbr,
-                 const int *__restrict__ cnt, const int *__restrict__ wnbr,
-                 const int *__restrict__ wcnt, int ls, double *__restrict__ dv)
+    k_lists(const int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
+            int ls, int cap_f, int *__restrict__ nbr, int *__restrict__ cnt, int cap_w,
+            int *__restrict__ wnbr, int *__restrict__ wcnt)
 {
-    __shared__ int lists[kCap * kBlock];  // only for particles whose list overflowed
-    int *lst = lists + threadIdx.x;
-    const int s = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    const bool active = s < P.nf;
-    const int sl = active ? s : 0;  // in-bounds index for the inactive lanes' loads
+    if (skip(flag)) return;
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < P.nf; t += gridDim.x * kBlock) {
+        const int s = ord[t];
+        build_list<ND, false>(P, F, Wv, s, t, ls, cap_f, nbr, cnt);
+        if (P.nw > 0) build_list<ND, true>(P, F, Wv, s, t, ls, cap_w, wnbr, wcnt);
+    }
+}
+
+// Verlet lists of the wall particles (wall-sorted order w) over the fluid: candidates within
+// R + skin/2 of the static wall particle, for the Adami pass.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_wall_lists(const int *flag, DevParams P, FluidView F, const double4 *__restrict__ wpos,
+                 int lw, int cap, int *__restrict__ nbr, int *__restrict__ cnt)
+{
+    if (skip(flag)) return;
+    for (int w = blockIdx.x * kBlock + threadIdx.x; w < P.nw; w += gridDim.x * kBlock) {
+        const double4 xw = wpos[w];
+        Pencils<ND> it;
+        it.init(P, xw.x, xw.y, xw.z, P.hs_list_wall);
+        int n = 0, j, je;
+        while (it.next(P, F.start, j, je)) {
+            for (; j < je; j++) {
+                const double d2 = dist2<ND>(xw.x - F.x[j], xw.y - F.y[j], (ND == 3) ? xw.z - F.z[j] : 0.0);
+                if (d2 <= P.rl2_wall) {
+                    if (n < cap) nbr[(size_t)n * lw + w] = j;
+                    n++;
+                }
+            }
+        }
+        cnt[w] = n <= cap ? n : -1;
+    }
+}
+
+// Broadcast of lane I of each quad (four consecutive lanes) to the whole quad (DPP).
+template <int I>
+__device__ __forceinline__ int quad_bcast(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, I * 0x55, 0xF, 0xF, false);
+}
+
+// Fluid kernel over the Verlet lists.
+//
+// Loads are quad-cooperative.  In slot k every lane needs the 64-byte record rec[2j], rec[2j+1]
+// of its own candidate j.  The four lanes of a quad fetch the four records of the quad
+// together: one 16-byte chunk per lane and record, so each load instruction reads 16 whole
+// records (half cache lines) instead of 64 scattered 16-byte pieces.  The chunks pass
+// through a wavefront-private LDS stage (80-byte stride: the writes and the reads are both
+// free of bank conflicts), from which each lane reads its own record.  The candidate
+// indices are read two slots ahead, the records one slot ahead of the pair terms.  The exact
+// radius test (and the zero-distance skip) selects the pairs.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_fluid_list(int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
+                 const int *__restrict__ nbr, const int *__restrict__ cnt,
+                 const int *__restrict__ wnbr, const int *__restrict__ wcnt, int ls,
+                 double *__restrict__ dv)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;  // the rebuild (if any) is done
+    // 8 KiB per wavefront: the staging area (64 records x 80 bytes) while the lists are
+    // streamed, then the int lists (kCap x 64 lanes) of the cell-walk fallback
+    constexpr int kWaveUnits = 64 * kCap / 4;  // 16-byte units
+    __shared__ double2 smem[kBlock / 64][kWaveUnits];
+    const int lane = threadIdx.x & 63;
+    double2 *stg = smem[threadIdx.x >> 6];
+    int *lst = reinterpret_cast<int *>(stg) + lane;  // fallback lists, stride 64
+    // t: position in the processing order (the list column), s: the cell-sorted particle
+    const int t = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool active = t < P.nf;
+    const int col = active ? t : 0;  // in-bounds column for the inactive lanes' loads
+    const int s = ord[col];
+    const int sl = s;
     double xa[3] = {0.0, 0.0, 0.0}, va[3] = {0.0, 0.0, 0.0}, rho_a = 1.0, p_a = 0.0;
     int n = 0, nw = 0;
     if (active) {
@@ -756,39 +929,62 @@ __global__ void __launch_bounds__(kBlock)
         va[1] = b.y;
         va[2] = b.z;
         p_a = b.w;
-        n = cnt[s];
-        if (P.nw > 0) nw = wcnt[s];
+        n = cnt[col];
+        if (P.nw > 0) nw = wcnt[col];
     }
     const double inv_rho_a = 1.0 / rho_a;
 
     double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
     {
         const int m = max(n, 0);
-        int j = (m > 0) ? nbr[sl] : sl;
-        FluidNbr cur = load_fluid(F, j);
-        for (int k = 0; k < m; k++) {
-            const int jn = (k + 1 < m) ? nbr[(size_t)(k + 1) * ls + sl] : j;
-            const FluidNbr nxt = load_fluid(F, jn);
-            const double p0 = xa[0] - cur.a.x, p1 = xa[1] - cur.a.y;
-            const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
-            const double d2 = dist2<ND>(p0, p1, p2);
-            if (d2 <= P.R2 && d2 >= P.az2)
-                pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
-            cur = nxt;
-            j = jn;
+        int mw = m;  // longest list of the wavefront
+        for (int o = 32; o > 0; o >>= 1) mw = max(mw, __shfl_xor(mw, o));
+        const double2 *rec2 = reinterpret_cast<const double2 *>(F.rec);
+        const uint32_t r16 = 16u * (uint32_t)(lane & 3);
+        double2 *mine = stg + 5 * (lane & ~3) + (lane & 3);  // chunk (lane & 3) of the quad's records
+        const double2 *own = stg + 5 * lane;
+        double2 c0, c1, c2, c3, vb;
+        auto issue = [&](int j) {
+            c0 = ldo(rec2, (uint32_t)quad_bcast<0>(j) * 64u + r16);
+            c1 = ldo(rec2, (uint32_t)quad_bcast<1>(j) * 64u + r16);
+            c2 = ldo(rec2, (uint32_t)quad_bcast<2>(j) * 64u + r16);
+            c3 = ldo(rec2, (uint32_t)quad_bcast<3>(j) * 64u + r16);
+            vb = ldo(F.vol, (uint32_t)j * 16u);
+        };
+        if (mw > 0) issue((m > 0) ? nbr[col] : sl);
+        int jn = (1 < m) ? nbr[(size_t)ls + col] : sl;  // candidate of the next slot
+        for (int k = 0; k < mw; k++) {
+            mine[0] = c0;
+            mine[5] = c1;
+            mine[10] = c2;
+            mine[15] = c3;
+            __builtin_amdgcn_wave_barrier();
+            const double2 a0 = own[0], a1 = own[1], b0 = own[2], b1 = own[3];
+            const FluidNbr cur = {make_double4(a0.x, a0.y, a1.x, a1.y),
+                                  make_double4(b0.x, b0.y, b1.x, b1.y), vb};
+            __builtin_amdgcn_wave_barrier();
+            if (k + 1 < mw) issue(jn);
+            jn = (k + 2 < m) ? nbr[(size_t)(k + 2) * ls + col] : sl;
+            if (k < m) {
+                const double p0 = xa[0] - cur.a.x, p1 = xa[1] - cur.a.y;
+                const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
+                const double d2 = dist2<ND>(p0, p1, p2);
+                if (d2 <= P.R2 && d2 >= P.az2)
+                    pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+            }
         }
     }
     if (__any(n < 0))
-        neighbour_sweep<ND, false>(P, F.start, F.x, F.y, F.z, P.hs_fluid, n < 0, F, Wv, lst,
-                                   xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+        neighbour_sweep<ND, false, kCap, 64>(P, F.start, F.x, F.y, F.z, P.hs_fluid, n < 0, F,
+                                             Wv, lst, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
 
     double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
     if (__any(nw != 0)) {
         const int m = max(nw, 0);
-        int j = (m > 0) ? wnbr[sl] : 0;
+        int j = (m > 0) ? wnbr[col] : 0;
         WallNbr cur = load_wall(Wv, j);
         for (int k = 0; k < m; k++) {
-            const int jn = (k + 1 < m) ? wnbr[(size_t)(k + 1) * ls + sl] : j;
+            const int jn = (k + 1 < m) ? wnbr[(size_t)(k + 1) * ls + col] : j;
             const WallNbr nxt = load_wall(Wv, jn);
             const double p0 = xa[0] - cur.a.x, p1 = xa[1] - cur.a.y;
             const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
@@ -799,8 +995,9 @@ __global__ void __launch_bounds__(kBlock)
             j = jn;
         }
         if (__any(nw < 0))
-            neighbour_sweep<ND, true>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0, F, Wv,
-                                      lst, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+            neighbour_sweep<ND, true, kCap, 64>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0,
+                                                F, Wv, lst, xa, va, rho_a, p_a, inv_rho_a, accw,
+                                                drhow);
     }
     if (!active) return;
     const int i = F.perm[s];
@@ -986,7 +1183,8 @@ __device__ __forceinline__ void tile_sweep(const DevParams &P, const FluidView &
 
 template <int ND>
 __global__ void __launch_bounds__(TileCfg<ND>::THREADS)
-    k_fluid_tiled(DevParams P, FluidView F, WallView Wv, const unsigned char *__restrict__ wall_near,
+    k_fluid_tiled(int *flag, DevParams P, FluidView F, WallView Wv,
+                  const unsigned char *__restrict__ wall_near,
                   const int *__restrict__ tiles, const int *__restrict__ ntiles,
                   int *__restrict__ work, int tgx, int tgy, double *__restrict__ dv)
 {
@@ -994,6 +1192,7 @@ __global__ void __launch_bounds__(TileCfg<ND>::THREADS)
     using SM = TileSmem<ND>;
     __shared__ SM S;
     const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && tid == 0) *flag = 0;  // the rebuild (if any) is done
     const int nt = *ntiles;
     while (true) {
         if (tid == 0) S.tile = atomicAdd(work, 1);
@@ -1140,6 +1339,290 @@ __global__ void __launch_bounds__(TileCfg<ND>::THREADS)
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Bricks in LDS.  A brick is a block of bc[0] x bc[1] x bc[2] cells; its targets are the
+// fluid particles in those cells (ord[bstart[b] .. bstart[b+1])).  Its union is every fluid
+// particle in a cell row (y, z) near the brick, trimmed in x: the cells within the list radius
+// R + skin of the brick box, as contiguous ranges of the cell-sorted arrays.  Every Verlet list
+// entry of a target lies in the union of its brick, so after one coalesced staging pass of the
+// union into LDS the pair terms read every candidate from LDS.  The lists also store the
+// candidate's LDS index (u16, four slots per ushort4).  A brick whose union exceeds kBrickCap,
+// or with a target whose list overflowed, is marked (bntot = INT_MAX) and its targets read
+// their candidates from global memory.
+// ------------------------------------------------------------------------------------------
+
+// Union rows of every brick (one block per brick, grid-stride) and the list of non-empty
+// bricks.  Row r = (cy - (by bc[1] - ur[1])) + nuy (cz - (bz bc[2] - ur[2])).
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_brick_union(const int *flag, DevParams P, const int *__restrict__ start,
+                  const int *__restrict__ bstart, int nbricks, int *__restrict__ brow_gb,
+                  int *__restrict__ brow_lds, int *__restrict__ bntot, int *__restrict__ nel,
+                  int *__restrict__ nne)
+{
+    if (skip(flag)) return;
+    __shared__ int scan_buf[kBlock];
+    const int tid = threadIdx.x;
+    const int nuy = P.bc[1] + 2 * P.ur[1];
+    for (int b = blockIdx.x; b < nbricks; b += gridDim.x) {
+        const int bx = b % P.nb[0], by = (b / P.nb[0]) % P.nb[1], bz = b / (P.nb[0] * P.nb[1]);
+        const int X0 = bx * P.bc[0], X1 = min(X0 + P.bc[0], P.n[0]) - 1;
+        const int Y0 = by * P.bc[1], Y1 = min(Y0 + P.bc[1], P.n[1]) - 1;
+        const int Z0 = (ND == 3) ? bz * P.bc[2] : 0;
+        const int Z1 = (ND == 3) ? min(Z0 + P.bc[2], P.n[2]) - 1 : 0;
+        const double m = P.hs_union;
+        const double bxlo = cell_lo(P, 0, X0, m), bxhi = cell_hi(P, 0, X1, m);
+        const double bylo = cell_lo(P, 1, Y0, m), byhi = cell_hi(P, 1, Y1, m);
+        const double bzlo = (ND == 3) ? cell_lo(P, 2, Z0, m) : 0.0;
+        const double bzhi = (ND == 3) ? cell_hi(P, 2, Z1, m) : 0.0;
+        int cntr = 0, gb = 0;
+        if (tid < P.maxr) {
+            const int cy = Y0 - P.ur[1] + tid % nuy;
+            const int cz = (ND == 3) ? Z0 - P.ur[2] + tid / nuy : 0;
+            if (cy >= 0 && cy < P.n[1] && cz >= 0 && cz < P.n[2]) {
+                const double dy = fmax(0.0, fmax(cell_lo(P, 1, cy, m) - byhi, bylo - cell_hi(P, 1, cy, m)));
+                double dz = 0.0;
+                if constexpr (ND == 3)
+                    dz = fmax(0.0, fmax(cell_lo(P, 2, cz, m) - bzhi, bzlo - cell_hi(P, 2, cz, m)));
+                const double r2 = P.rl_union2 - (dy * dy + dz * dz);
+                if (r2 >= 0.0) {
+                    const double rx = sqrt(r2) * (1.0 + 1e-9) + m;
+                    const int x0 = cell_1d(bxlo - rx, P.lo[0], P.inv_cs[0], P.n[0]);
+                    const int x1 = cell_1d(bxhi + rx, P.lo[0], P.inv_cs[0], P.n[0]);
+                    const int row = P.n[0] * (cy + P.n[1] * cz);
+                    gb = start[row + x0];
+                    cntr = start[row + x1 + 1] - gb;
+                }
+            }
+            brow_gb[(size_t)b * P.maxr + tid] = gb;
+        }
+        int tot;
+        const int lds0 = block_exclusive_scan(cntr, scan_buf, tot);
+        if (tid < P.maxr) brow_lds[(size_t)b * (P.maxr + 1) + tid] = lds0;
+        if (tid == 0) {
+            brow_lds[(size_t)b * (P.maxr + 1) + P.maxr] = tot;
+            bntot[b] = tot;
+            if (bstart[b + 1] > bstart[b]) nel[atomicAdd(nne, 1)] = b;
+        }
+    }
+}
+
+// LDS indices of the fluid lists of every target (the same entries as nbr, in the same
+// order), and the marks of the bricks that cannot stage.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_brick_lists(const int *flag, DevParams P, FluidView F, const int *__restrict__ ord,
+                  const int *__restrict__ nbr, const int *__restrict__ cnt,
+                  const int *__restrict__ wcnt, const int *__restrict__ brow_gb,
+                  const int *__restrict__ brow_lds, int *__restrict__ bntot, int ls,
+                  unsigned short *__restrict__ nbr16)
+{
+    if (skip(flag)) return;
+    const int nuy = P.bc[1] + 2 * P.ur[1];
+    for (int t = blockIdx.x * kBlock + threadIdx.x; t < P.nf; t += gridDim.x * kBlock) {
+        const int s = ord[t];
+        const int c0 = cell_1d(F.x[s], P.lo[0], P.inv_cs[0], P.n[0]);
+        const int c1 = cell_1d(F.y[s], P.lo[1], P.inv_cs[1], P.n[1]);
+        const int c2 = (ND == 3) ? cell_1d(F.z[s], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+        const int bx = c0 / P.bc[0], by = c1 / P.bc[1], bz = c2 / P.bc[2];
+        const int b = bx + P.nb[0] * (by + P.nb[1] * bz);
+        const int n = cnt[t];
+        bool ok = n >= 0 && (P.nw == 0 || wcnt[t] >= 0);
+        const int uy0 = by * P.bc[1] - P.ur[1], uz0 = (ND == 3) ? bz * P.bc[2] - P.ur[2] : 0;
+        const int *gb = brow_gb + (size_t)b * P.maxr;
+        const int *lds = brow_lds + (size_t)b * (P.maxr + 1);
+        for (int k = 0; ok && k < n; k++) {
+            const int j = nbr[(size_t)k * ls + t];
+            const int cy = cell_1d(F.y[j], P.lo[1], P.inv_cs[1], P.n[1]);
+            const int cz = (ND == 3) ? cell_1d(F.z[j], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+            const int ry = cy - uy0, rz = cz - uz0;
+            const int r = ry + nuy * rz;
+            if (ry < 0 || ry >= nuy || rz < 0 || r >= P.maxr) {
+                ok = false;
+                break;
+            }
+            const int e = lds[r] + (j - gb[r]);
+            if (j < gb[r] || e >= lds[r + 1] || e >= kBrickCap) {
+                ok = false;
+                break;
+            }
+            nbr16[((size_t)(k >> 2) * ls + t) * 4 + (k & 3)] = (unsigned short)e;
+        }
+        if (!ok) bntot[b] = INT_MAX;
+    }
+}
+
+template <int ND>
+struct BrickSmem {
+    // staged union (SoA); with a global brick the same bytes hold the lanes' int lists
+    double x[kBrickCap], y[kBrickCap], z[ND == 3 ? kBrickCap : 1], rho[kBrickCap];
+    double vx[kBrickCap], vy[kBrickCap], vz[ND == 3 ? kBrickCap : 1], p[kBrickCap];
+    double V[kBrickCap];
+    int row_gb[kBrickMaxRows], row_lds[kBrickMaxRows + 1];
+    int item;
+};
+
+template <int ND>
+__device__ __forceinline__ FluidNbr lds_fluid(const BrickSmem<ND> &S, int e)
+{
+    FluidNbr n;
+    n.a = make_double4(S.x[e], S.y[e], (ND == 3) ? S.z[e] : 0.0, S.rho[e]);
+    n.b = make_double4(S.vx[e], S.vy[e], (ND == 3) ? S.vz[e] : 0.0, S.p[e]);
+    n.v = make_double2(S.V[e], S.V[e] * S.rho[e]);
+    return n;
+}
+
+// Fluid kernel over the bricks (one block of kBrickThreads per non-empty brick, taken from a
+// work counter that k_gather<0> resets).
+template <int ND>
+__global__ void __launch_bounds__(kBrickThreads)
+    k_fluid_brick(int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
+                  const int *__restrict__ bstart, const int *__restrict__ nel,
+                  const int *__restrict__ nne, int *__restrict__ work,
+                  const int *__restrict__ brow_gb, const int *__restrict__ brow_lds,
+                  const int *__restrict__ bntot, const unsigned short *__restrict__ nbr16,
+                  const int *__restrict__ nbr, const int *__restrict__ cnt,
+                  const int *__restrict__ wnbr, const int *__restrict__ wcnt, int ls,
+                  double *__restrict__ dv)
+{
+    // cell-walk fallback lists of a global brick: as many entries per lane as the staging
+    // area holds (at most kCap)
+    constexpr int kFallbackCap = std::min<int>(kCap, sizeof(BrickSmem<ND>) / (sizeof(int) * kBrickThreads));
+    static_assert(kFallbackCap >= 8, "the fallback lists must fit the staging area");
+    __shared__ BrickSmem<ND> S;
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && tid == 0) *flag = 0;  // the rebuild (if any) is done
+    int *lst = reinterpret_cast<int *>(&S) + tid;  // fallback lists, stride kBrickThreads
+    const int ne = *nne;
+    while (true) {
+        if (tid == 0) S.item = atomicAdd(work, 1);
+        __syncthreads();
+        const int item = S.item;
+        if (item >= ne) break;
+        const int b = nel[item];
+        const int t0 = bstart[b], t1 = bstart[b + 1];
+        const int ntot = bntot[b];
+        const bool staged = ntot <= kBrickCap;
+        if (staged) {
+            for (int r = tid; r <= P.maxr; r += kBrickThreads) {
+                if (r < P.maxr) S.row_gb[r] = brow_gb[(size_t)b * P.maxr + r];
+                S.row_lds[r] = brow_lds[(size_t)b * (P.maxr + 1) + r];
+            }
+            __syncthreads();
+            for (int i = tid; i < ntot; i += kBrickThreads) {
+                const int r = bsearch_le(S.row_lds, P.maxr + 1, i);
+                const int g = S.row_gb[r] + (i - S.row_lds[r]);
+                const double4 a = F.rec[2 * (size_t)g];
+                const double4 c = F.rec[2 * (size_t)g + 1];
+                S.x[i] = a.x;
+                S.y[i] = a.y;
+                if constexpr (ND == 3) S.z[i] = a.z;
+                S.rho[i] = a.w;
+                S.vx[i] = c.x;
+                S.vy[i] = c.y;
+                if constexpr (ND == 3) S.vz[i] = c.z;
+                S.p[i] = c.w;
+                S.V[i] = F.vol[g].x;
+            }
+            __syncthreads();
+        }
+        // Two lanes per target: lanes 2i and 2i+1 take alternate groups of four list slots
+        // (alternate slots of the global and wall lists); lane 2i adds its partner's sums.
+        const int half = tid & 1;
+        for (int base = t0; base < t1; base += kBrickThreads / 2) {
+            const int t = base + (tid >> 1);
+            const bool active = t < t1;
+            const int col = active ? t : t0;
+            const int s = ord[col];
+            double xa[3] = {0.0, 0.0, 0.0}, va[3] = {0.0, 0.0, 0.0}, rho_a = 1.0, p_a = 0.0;
+            int n = 0, nw = 0;
+            if (active) {
+                const double4 a = F.rec[2 * (size_t)s];
+                const double4 c = F.rec[2 * (size_t)s + 1];
+                xa[0] = a.x;
+                xa[1] = a.y;
+                xa[2] = a.z;
+                rho_a = a.w;
+                va[0] = c.x;
+                va[1] = c.y;
+                va[2] = c.z;
+                p_a = c.w;
+                n = cnt[col];
+                if (P.nw > 0) nw = wcnt[col];
+            }
+            const double inv_rho_a = 1.0 / rho_a;
+            double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
+            const int m = max(n, 0);
+            if (staged) {
+                // four LDS indices per 8-byte load, one group ahead of the pair terms
+                const ushort4 *l4 = reinterpret_cast<const ushort4 *>(nbr16);
+                const int ng = (m + 3) >> 2;
+                ushort4 cur4 = (half < ng) ? l4[(size_t)half * ls + col] : make_ushort4(0, 0, 0, 0);
+                for (int g = half; g < ng; g += 2) {
+                    const ushort4 nxt4 = (g + 2 < ng) ? l4[(size_t)(g + 2) * ls + col] : cur4;
+                    // the candidates are independent chains without branches: two at a time
+                    // interleave within the register budget of four waves per SIMD
+#pragma unroll 1
+                    for (int h2 = 0; h2 < 2; h2++) {
+                        const int k = 4 * g + 2 * h2;
+                        const int ea = h2 ? cur4.z : cur4.x, eb = h2 ? cur4.w : cur4.y;
+                        const bool va0 = k < m, vb0 = k + 1 < m;
+                        const FluidNbr na = lds_fluid<ND>(S, va0 ? ea : 0);
+                        const FluidNbr nb = lds_fluid<ND>(S, vb0 ? eb : 0);
+                        pair_fluid_masked<ND>(P, na, va0, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                        pair_fluid_masked<ND>(P, nb, vb0, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                    }
+                    cur4 = nxt4;
+                }
+            } else {
+                for (int k = half; k < m; k += 2) {
+                    const FluidNbr nb = load_fluid(F, nbr[(size_t)k * ls + col]);
+                    const double p0 = xa[0] - nb.a.x, p1 = xa[1] - nb.a.y;
+                    const double p2 = (ND == 3) ? xa[2] - nb.a.z : 0.0;
+                    const double d2 = dist2<ND>(p0, p1, p2);
+                    if (d2 <= P.R2 && d2 >= P.az2)
+                        pair_fluid<ND>(P, nb, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                }
+                if (__any(n < 0))
+                    neighbour_sweep<ND, false, kFallbackCap, kBrickThreads>(
+                        P, F.start, F.x, F.y, F.z, P.hs_fluid, n < 0 && half == 0, F, Wv, lst, xa,
+                        va, rho_a, p_a, inv_rho_a, acc, drho);
+            }
+            double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
+            if (__any(nw != 0)) {
+                const int mw = max(nw, 0);
+                for (int k = half; k < mw; k += 2) {
+                    const WallNbr wb = load_wall(Wv, wnbr[(size_t)k * ls + col]);
+                    const double p0 = xa[0] - wb.a.x, p1 = xa[1] - wb.a.y;
+                    const double p2 = (ND == 3) ? xa[2] - wb.a.z : 0.0;
+                    const double d2 = dist2<ND>(p0, p1, p2);
+                    if (d2 <= P.R2 && d2 >= P.az2)
+                        pair_wall<ND>(P, wb, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+                }
+                // a wall list overflow marks the brick as global, so the LDS is free here
+                if (!staged && __any(nw < 0))
+                    neighbour_sweep<ND, true, kFallbackCap, kBrickThreads>(
+                        P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0 && half == 0, F, Wv, lst,
+                        xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+            }
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                acc[d] += __shfl_xor(acc[d], 1);
+                accw[d] += __shfl_xor(accw[d], 1);
+            }
+            drho += __shfl_xor(drho, 1);
+            drhow += __shfl_xor(drhow, 1);
+            if (active && half == 0) {
+                const int i = F.perm[s];
+#pragma unroll
+                for (int d = 0; d < ND; d++) dv[(size_t)i * (ND + 1) + d] = (acc[d] + accw[d]) + P.g[d];
+                dv[(size_t)i * (ND + 1) + ND] = drho + drhow;
+            }
+        }
+        __syncthreads();  // the LDS is staged again for the next brick
+    }
+}
+
 // Active tiles (at least one fluid particle), listed after every rebuild of the cell list.
 template <int ND>
 __global__ void k_tiles(const int *flag, DevParams P, const int *__restrict__ start, int tgx,
@@ -1175,28 +1658,31 @@ __global__ void k_drift(int nf, const double *__restrict__ v, double *__restrict
 
 // One 2N-storage stage of the partition (v, u) with slopes (dv, v): the drift slope is read
 // before v is updated, as the integrator evaluates kick! and drift! at the same stage state.
+// A block owns kBlock particles and walks their u and then their v entries as flat,
+// contiguous ranges, so every load and store of a wavefront covers whole cache lines.
 template <int ND>
-__global__ void k_ck_stage(int nf, double *__restrict__ v, double *__restrict__ u,
-                           double *__restrict__ tv, double *__restrict__ tu,
-                           const double *__restrict__ dv, double a, double b, double dt,
-                           int first)
+__global__ void __launch_bounds__(kBlock)
+    k_ck_stage(int nf, double *__restrict__ v, double *__restrict__ u, double *__restrict__ tv,
+               double *__restrict__ tu, const double *__restrict__ dv, double a, double b,
+               double dt, int first)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nf) return;
-#pragma unroll
-    for (int d = 0; d < ND; d++) {
-        const size_t ku = (size_t)i * ND + d, kv = (size_t)i * (ND + 1) + d;
-        double slope = v[kv];
-        double t = first ? dt * slope : fma(dt, slope, a * tu[ku]);
-        tu[ku] = t;
-        u[ku] = fma(b, t, u[ku]);
+    const int i0 = blockIdx.x * kBlock;
+    const int np = min(kBlock, nf - i0);
+    const size_t u0 = (size_t)i0 * ND, v0 = (size_t)i0 * (ND + 1);
+    for (int e = threadIdx.x; e < np * ND; e += kBlock) {
+        const int p = e / ND, d = e - p * ND;
+        const double slope = v[v0 + (size_t)p * (ND + 1) + d];
+        const size_t k = u0 + e;
+        const double t = first ? dt * slope : fma(dt, slope, a * tu[k]);
+        tu[k] = t;
+        u[k] = fma(b, t, u[k]);
     }
-#pragma unroll
-    for (int d = 0; d <= ND; d++) {
-        const size_t kv = (size_t)i * (ND + 1) + d;
-        double t = first ? dt * dv[kv] : fma(dt, dv[kv], a * tv[kv]);
-        tv[kv] = t;
-        v[kv] = fma(b, t, v[kv]);
+    __syncthreads();  // the v entries read above are written below
+    for (int e = threadIdx.x; e < np * (ND + 1); e += kBlock) {
+        const size_t k = v0 + e;
+        const double t = first ? dt * dv[k] : fma(dt, dv[k], a * tv[k]);
+        tv[k] = t;
+        v[k] = fma(b, t, v[k]);
     }
 }
 
@@ -1293,7 +1779,7 @@ __global__ void k_record_pressure(int n, const double4 *__restrict__ rec, double
 }
 
 inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
-inline int grid_blocks(int64_t n) { return std::max(1, std::min(nblocks(n), kMaxGridBlocks)); }
+inline int gate_blocks(int64_t n) { return std::max(1, std::min(nblocks(n), kGateBlocks)); }
 
 }  // namespace
 
@@ -1327,6 +1813,16 @@ struct tpsph_handle {
     // Verlet lists (slot-major, ls slots per row) and their counts
     int *nbr = nullptr, *cnt = nullptr, *wnbr = nullptr, *wcnt = nullptr;
     int ls = 0, cap_f = 0, cap_w = 0;
+    // processing order of the list kernels (bricks of cells) and the bricks in LDS
+    int *ord = nullptr, *bcount = nullptr, *bstart = nullptr;
+    int nbricks = 0;
+    bool bricks = false;
+    int *brow_gb = nullptr, *brow_lds = nullptr, *bntot = nullptr, *nel = nullptr, *nne = nullptr,
+        *bwork = nullptr;
+    unsigned short *nbr16 = nullptr;
+    // Verlet lists of the wall particles over the fluid (Adami pass)
+    int *anbr = nullptr, *acnt = nullptr;
+    int lw = 0, cap_a = 0;
     // tiled fluid kernel
     bool tiled = false;
     int tgx = 1, tgy = 1, ntiles_all = 0, n_cu = 256;
@@ -1435,15 +1931,15 @@ template <int ND>
 int build_cells(tpsph_handle *h, const int *flag, int n, const double *x, int *cell, int *rank,
                 int *count, int *start, int *perm)
 {
+    // count is zero on entry (cleared at create, and again by k_scatter)
     const int nc = h->P.ncells;
-    hipLaunchKernelGGL((k_fill_i32), dim3(grid_blocks(nc + 1)), dim3(kBlock), 0, h->stream, flag, count, 0, nc + 1);
     if (n > 0)
-        hipLaunchKernelGGL((k_cells<ND>), dim3(nblocks(n)), dim3(kBlock), 0, h->stream, flag, h->P, n, x, cell, rank, count);
+        hipLaunchKernelGGL((k_cells<ND>), dim3(gate_blocks(n)), dim3(kBlock), 0, h->stream, flag, h->P, n, x, cell, rank, count);
     TRY(launch_check(h, "k_cells"));
     TRY(scan(h, flag, count, start, nc + 1));
     if (n > 0) {
-        hipLaunchKernelGGL((k_scatter), dim3(nblocks(n)), dim3(kBlock), 0, h->stream, flag, n, cell, rank, start, perm);
-        hipLaunchKernelGGL((k_sort_cells), dim3(grid_blocks(nc)), dim3(kBlock), 0, h->stream, flag, nc, start, perm);
+        hipLaunchKernelGGL((k_scatter), dim3(gate_blocks(n)), dim3(kBlock), 0, h->stream, flag, n, cell, rank, start, perm, count);
+        hipLaunchKernelGGL((k_sort_cells), dim3(gate_blocks(nc)), dim3(kBlock), 0, h->stream, flag, nc, start, perm);
     }
     return launch_check(h, "k_scatter/k_sort_cells");
 }
@@ -1456,8 +1952,8 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
     if (nf == 0) return TPSPH_OK;
     {
         TimedScope t(h, TPSPH_TIME_GATHER);
-        hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, 
-            h->flag, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
+        hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
+            h->flag, h->bwork, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
         TRY(launch_check(h, "k_gather<0>"));
     }
     {
@@ -1470,17 +1966,33 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
                 h->flag, h->P, h->f_start, h->tgx, h->tgy, h->ntiles_all, h->tiles, h->ntiles);
             TRY(launch_check(h, "k_tiles"));
         }
-        hipLaunchKernelGGL((k_gather<ND, 1>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, 
-            h->flag, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
+        hipLaunchKernelGGL((k_gather<ND, 1>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
+            h->flag, nullptr, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
         if (h->search == TPSPH_SEARCH_LISTS) {
-            hipLaunchKernelGGL((k_lists<ND, false>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
-                h->flag, h->P, h->fluid_view(), h->wall_view(), h->ls, h->cap_f, h->nbr, h->cnt);
+            // brick order (f_cell / f_rank are free again after the cell list build)
+            hipLaunchKernelGGL((k_brick_count<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
+                h->flag, h->P, h->xs, h->ys, h->zs, h->f_cell, h->f_rank, h->bcount);
+            TRY(scan(h, h->flag, h->bcount, h->bstart, h->nbricks + 1));
+            hipLaunchKernelGGL((k_brick_scatter), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
+                h->flag, nf, h->f_cell, h->f_rank, h->bstart, h->ord, h->bcount);
+            hipLaunchKernelGGL((k_lists<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
+                h->flag, h->P, h->fluid_view(), h->wall_view(), h->ord, h->ls, h->cap_f, h->nbr,
+                h->cnt, h->cap_w, h->wnbr, h->wcnt);
+            if (h->bricks) {
+                hipLaunchKernelGGL((k_fill_i32), dim3(1), dim3(1), 0, h->stream, h->flag, h->nne, 0, 1);
+                hipLaunchKernelGGL((k_brick_union<ND>), dim3(std::max(1, std::min(h->nbricks, kGateBlocks))),
+                    dim3(kBlock), 0, h->stream, h->flag, h->P, h->f_start, h->bstart, h->nbricks,
+                    h->brow_gb, h->brow_lds, h->bntot, h->nel, h->nne);
+                hipLaunchKernelGGL((k_brick_lists<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
+                    h->flag, h->P, h->fluid_view(), h->ord, h->nbr, h->cnt, h->wcnt, h->brow_gb,
+                    h->brow_lds, h->bntot, h->ls, h->nbr16);
+            }
             if (h->nw > 0)
-                hipLaunchKernelGGL((k_lists<ND, true>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
-                    h->flag, h->P, h->fluid_view(), h->wall_view(), h->ls, h->cap_w, h->wnbr, h->wcnt);
+                hipLaunchKernelGGL((k_wall_lists<ND>), dim3(gate_blocks(h->nw)), dim3(kBlock), 0, h->stream,
+                    h->flag, h->P, h->fluid_view(), h->wpos, h->lw, h->cap_a, h->anbr, h->acnt);
             TRY(launch_check(h, "k_lists"));
         }
-        hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->flag, 0);
+        // the fluid kernel of this kick lowers the flag
         TRY(launch_check(h, "k_gather<1>"));
     }
     return TPSPH_OK;
@@ -1493,24 +2005,29 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u)
     TRY(update_fluid<ND>(h, v, u));
     if (nw > 0) {
         TimedScope t(h, TPSPH_TIME_WALL);
-        hipLaunchKernelGGL((k_wall<ND>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P, h->fluid_view(), h->wpos, h->pw,
-                                                           h->rhow, h->wpv);
+        hipLaunchKernelGGL((k_wall<ND>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P,
+            h->fluid_view(), h->wpos, h->pw, h->rhow, h->wpv, h->anbr, h->acnt, h->lw);
         TRY(launch_check(h, "k_wall"));
     }
     if (nf > 0) {
         TimedScope t(h, TPSPH_TIME_FLUID);
-        if (h->search == TPSPH_SEARCH_LISTS) {
+        if (h->search == TPSPH_SEARCH_LISTS && h->bricks) {
+            hipLaunchKernelGGL((k_fluid_brick<ND>), dim3(std::max(1, std::min(h->nbricks, h->n_cu))),
+                dim3(kBrickThreads), 0, h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
+                h->ord, h->bstart, h->nel, h->nne, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
+                h->nbr16, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
+        } else if (h->search == TPSPH_SEARCH_LISTS) {
             hipLaunchKernelGGL((k_fluid_list<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
-                h->P, h->fluid_view(), h->wall_view(), h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
+                h->flag, h->P, h->fluid_view(), h->wall_view(), h->ord, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
         } else if (h->search == TPSPH_SEARCH_TILES) {
             hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->work, 0);
             const int grid = std::max(1, std::min(h->ntiles_all, 2 * h->n_cu));
-            hipLaunchKernelGGL((k_fluid_tiled<ND>), dim3(grid), dim3(TileCfg<ND>::THREADS), 0, h->stream, 
-                h->P, h->fluid_view(), h->wall_view(), h->wall_near, h->tiles, h->ntiles, h->work,
+            hipLaunchKernelGGL((k_fluid_tiled<ND>), dim3(grid), dim3(TileCfg<ND>::THREADS), 0, h->stream,
+                h->flag, h->P, h->fluid_view(), h->wall_view(), h->wall_near, h->tiles, h->ntiles, h->work,
                 h->tgx, h->tgy, dv);
         } else {
-            hipLaunchKernelGGL((k_fluid<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, h->P, h->fluid_view(), h->wall_view(),
-                                                                 h->wall_near, dv);
+            hipLaunchKernelGGL((k_fluid<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, h->flag,
+                h->P, h->fluid_view(), h->wall_view(), h->wall_near, dv);
         }
         TRY(launch_check(h, "k_fluid"));
     }
@@ -1709,6 +2226,28 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         return fail(nullptr, TPSPH_ERR_ARGUMENT, "cell grid too large (domain / smoothing_length)");
     }
     P.ncells = (int)ncells;
+    {
+        // bricks of 3 (3D) or 8 (2D) cell spans per side: about 3.1 R (8.3 R) wide, at
+        // h = 1.2 dx about 420 (400) targets and a union of about 1700 (600) particles
+        const int per = nd == 3 ? 3 : 8;
+        int64_t nb = 1;
+        for (int d = 0; d < 3; d++) {
+            P.bc[d] = d < nd ? per * (d == 0 ? xdiv : div) : 1;
+            P.nb[d] = (P.n[d] + P.bc[d] - 1) / P.bc[d];
+            nb *= P.nb[d];
+        }
+        h->nbricks = (int)nb;
+        // union rows: cells within the list radius of the brick box (grown by the margin)
+        const double rl = R + 2.0 * P.half_skin + margin;
+        int rows = 1;
+        for (int d = 0; d < 3; d++) {
+            P.ur[d] = (d >= 1 && d < nd) ? (int)std::ceil(rl / P.cs[d]) + 1 : 0;
+            if (d >= 1 && d < nd) rows *= P.bc[d] + 2 * P.ur[d];
+        }
+        P.maxr = rows;
+        P.rl_union2 = (rl + margin) * (rl + margin);
+        P.hs_union = margin;
+    }
     // tiled fluid kernel: needs the default cell divisions (its halo widths)
     h->tiled = (nd == 3) ? (div == TileCfg<3>::HY && xdiv == TileCfg<3>::HX)
                          : (div == TileCfg<2>::HY && xdiv == TileCfg<2>::HX);
@@ -1747,6 +2286,9 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         h->cap_f = cap(mf > 0.0 ? ball(rlf) / (mf / P.rho0) : 0.0);
         h->cap_w = h->nw > 0 ? cap(mw > 0.0 ? (nd == 2 ? 0.75 : 0.875) * ball(rlw) / (mw / P.rho0) : 0.0) : 0;
         h->ls = (h->nf + 63) / 64 * 64;
+        // a wall particle of a flat wall sees fluid in at most half its ball
+        h->cap_a = h->nw > 0 ? cap(mf > 0.0 ? 0.6 * ball(rlw) / (mf / P.rho0) : 0.0) : 0;
+        h->lw = (h->nw + 63) / 64 * 64;
     }
     if (nd == 3) {
         h->tgx = (P.n[0] + TileCfg<3>::TX - 1) / TileCfg<3>::TX;
@@ -1797,9 +2339,28 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     if (h->search == TPSPH_SEARCH_LISTS) {
         CTRY(dalloc(h, &h->nbr, (size_t)h->cap_f * h->ls));
         CTRY(dalloc(h, &h->cnt, nf));
+        CTRY(dalloc(h, &h->ord, nf));
+        h->bricks = P.maxr <= kBrickMaxRows;
+        if (h->bricks) {
+            CTRY(dalloc(h, &h->brow_gb, (size_t)h->nbricks * P.maxr));
+            CTRY(dalloc(h, &h->brow_lds, (size_t)h->nbricks * (P.maxr + 1)));
+            CTRY(dalloc(h, &h->bntot, h->nbricks));
+            CTRY(dalloc(h, &h->nel, h->nbricks));
+            CTRY(dalloc(h, &h->nne, 1));
+            CTRY(dalloc(h, &h->bwork, 1));
+            CTRY(dalloc(h, &h->nbr16, (size_t)h->cap_f * h->ls));
+        }
+        CTRY(dalloc(h, &h->bcount, (size_t)h->nbricks + 1));
+        CTRY(dalloc(h, &h->bstart, (size_t)h->nbricks + 1));
+        if (hipMemset(h->bcount, 0, ((size_t)h->nbricks + 1) * sizeof(int)) != hipSuccess) {
+            h->last_error = "clearing the brick counts failed";
+            return cleanup_fail(TPSPH_ERR_HIP);
+        }
         if (nw > 0) {
             CTRY(dalloc(h, &h->wnbr, (size_t)h->cap_w * h->ls));
             CTRY(dalloc(h, &h->wcnt, nf));
+            CTRY(dalloc(h, &h->anbr, (size_t)h->cap_a * h->lw));
+            CTRY(dalloc(h, &h->acnt, nw));
         }
     }
     CTRY(dalloc(h, &h->tiles, (size_t)h->ntiles_all));
@@ -1811,6 +2372,10 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         CTRY(hip_fill(h, a, (size_t)nf + kPad));
     for (double *a : {h->wx, h->wy, h->wz})
         CTRY(hip_fill(h, a, (size_t)nw + kPad));
+    if (hipMemset(h->f_count, 0, ((size_t)ncells + 1) * sizeof(int)) != hipSuccess) {
+        h->last_error = "clearing the cell counts failed";
+        return cleanup_fail(TPSPH_ERR_HIP);
+    }
     if (hipMemset(h->wall_near, 0, (size_t)ncells) != hipSuccess) {
         h->last_error = "clearing wall_near failed";
         return cleanup_fail(TPSPH_ERR_HIP);
@@ -1837,6 +2402,8 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         if (rc == TPSPH_OK) rc = dalloc(h, &wcell, nw);
         if (rc == TPSPH_OK) rc = dalloc(h, &wrank, nw);
         if (rc == TPSPH_OK) rc = dalloc(h, &wcount, (size_t)ncells + 1);
+        if (rc == TPSPH_OK && hipMemset(wcount, 0, ((size_t)ncells + 1) * sizeof(int)) != hipSuccess)
+            rc = fail(h, TPSPH_ERR_HIP, "clearing the wall cell counts failed");
         if (rc == TPSPH_OK) {
             hipError_t e1 = hipMemcpy(wx_d, wxh.data(), wxh.size() * sizeof(double), hipMemcpyHostToDevice);
             hipError_t e2 = hipMemcpy(wm_d, wall_mass, (size_t)nw * sizeof(double), in_kind);
@@ -2147,7 +2714,8 @@ void tpsph_destroy(tpsph_handle *h)
     void *ptrs[] = {h->flag, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm,
                     h->scan_tiles, h->f_mass, h->xs, h->ys, h->zs, h->snap, h->rec, h->vol,
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
-                    h->tiles, h->ntiles, h->work, h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt,
+                    h->tiles, h->ntiles, h->work, h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
+                    h->brow_gb, h->brow_lds, h->bntot, h->nel, h->nne, h->bwork, h->nbr16,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu};
     for (void *p : ptrs)
