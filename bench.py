@@ -23,6 +23,19 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from its committed rocprofv3 summary (FETCH_SIZE +
+    WRITE_SIZE, calibrated against a copy of known size), or None."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", f"{kernel}_*.md"))):
+        m = re.search(r"`%s`: fetch [\d.]+ MB \+ write [\d.]+ MB = \*\*([\d.]+) MB\*\* per launch"
+                      % re.escape(kernel), open(path).read())
+        if m:
+            return float(m.group(1)) * 1e6, os.path.relpath(path, REPO)
+    return None, None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +113,7 @@ def main():
     nd = 3
     fluid_bytes = nf * (8 * nd + 8 * nd + 8 + 8 + 8 + 8 * (nd + 1))
     wall_bytes = nw * (8 * nd + 8 + 8 + 8)
+    traffic, traffic_src = profiled_traffic(fluid_kernel)
     n_launch, fluid_ms = times["fluid"]
     avg_s = fluid_ms / 1e3 / max(n_launch, 1)
     achieved = (fluid_bytes + wall_bytes) / avg_s / 1e9
@@ -151,7 +165,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": f"{fluid_kernel} (fused fluid-fluid + fluid-wall interaction)",
             "avg_launch_us": avg_s * 1e6,
             "algorithmic_bytes_per_launch": fluid_bytes + wall_bytes,

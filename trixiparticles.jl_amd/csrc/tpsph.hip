@@ -479,35 +479,50 @@ __device__ __forceinline__ void adami_term(const DevParams &P, const FluidView &
     volume += W;
 }
 
-// alist / acnt: the wall particle's Verlet list of fluid candidates (slot-major, row length
-// lw), or nullptr for the cell walk.  A count of -1 (overflow) also walks the cells.
-template <int ND>
-__global__ void __launch_bounds__(kBlock)
-    k_wall(DevParams P, FluidView F, const double4 *__restrict__ wpos, double *__restrict__ pw,
-           double *__restrict__ rhow, double2 *__restrict__ wpv, const int *__restrict__ alist,
-           const int *__restrict__ acnt, int lw)
+// p_w = P_w / V_w (if V_w > eps), optional clipping, the inverse state equation.
+__device__ __forceinline__ void adami_finish(const DevParams &P, int w, double mass,
+                                             double pressure, double volume,
+                                             double *__restrict__ pw, double *__restrict__ rhow,
+                                             double2 *__restrict__ wpv)
 {
-    int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= P.nw) return;
-    const double4 xw = wpos[w];
-    double pressure = 0.0, volume = 0.0;
-    const int n = alist ? acnt[w] : -1;
-    if (n >= 0) {
-        for (int k = 0; k < n; k++)
-            adami_term<ND>(P, F, xw, alist[(size_t)k * lw + w], pressure, volume);
-    } else {
-        Pencils<ND> it;
-        it.init(P, xw.x, xw.y, xw.z, P.hs_fluid);
-        int jb, je;
-        while (it.next(P, F.start, jb, je))
-            for (int j = jb; j < je; j++) adami_term<ND>(P, F, xw, j, pressure, volume);
-    }
     if (volume > DBL_EPSILON) pressure /= volume;
     if (P.clip_wall) pressure = julia_max0(pressure);
     const double rho = P.rho0 * pow((pressure - P.p_bg) / P.B + 1.0, P.inv_gamma);
     pw[w] = pressure;
     rhow[w] = rho;
-    wpv[w] = make_double2(pressure, xw.w / rho);
+    wpv[w] = make_double2(pressure, mass / rho);
+}
+
+// alist / acnt: the wall particles' Verlet lists of fluid candidates (slot-major, row
+// length lw), or nullptr for the cell walk (every wall particle).  With lists only the
+// active wall particles (a non-empty list; active / nactive, built at the rebuild) are
+// visited: the others keep the values the rebuild gave them.  A count of -1 (overflow)
+// walks the cells.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_wall(DevParams P, FluidView F, const double4 *__restrict__ wpos, double *__restrict__ pw,
+           double *__restrict__ rhow, double2 *__restrict__ wpv, const int *__restrict__ alist,
+           const int *__restrict__ acnt, int lw, const int *__restrict__ active,
+           const int *__restrict__ nactive)
+{
+    const int na = alist ? *nactive : P.nw;
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < na; a += gridDim.x * blockDim.x) {
+        const int w = alist ? active[a] : a;
+        const double4 xw = wpos[w];
+        double pressure = 0.0, volume = 0.0;
+        const int n = alist ? acnt[w] : -1;
+        if (n >= 0) {
+            for (int k = 0; k < n; k++)
+                adami_term<ND>(P, F, xw, alist[(size_t)k * lw + w], pressure, volume);
+        } else {
+            Pencils<ND> it;
+            it.init(P, xw.x, xw.y, xw.z, P.hs_fluid);
+            int jb, je;
+            while (it.next(P, F.start, jb, je))
+                for (int j = jb; j < je; j++) adami_term<ND>(P, F, xw, j, pressure, volume);
+        }
+        adami_finish(P, w, xw.w, pressure, volume, pw, rhow, wpv);
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -857,7 +872,9 @@ __global__ void __launch_bounds__(kBlock)
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
     k_wall_lists(const int *flag, DevParams P, FluidView F, const double4 *__restrict__ wpos,
-                 int lw, int cap, int *__restrict__ nbr, int *__restrict__ cnt)
+                 int lw, int cap, int *__restrict__ nbr, int *__restrict__ cnt,
+                 int *__restrict__ active, int *__restrict__ nactive, double *__restrict__ pw,
+                 double *__restrict__ rhow, double2 *__restrict__ wpv)
 {
     if (skip(flag)) return;
     for (int w = blockIdx.x * kBlock + threadIdx.x; w < P.nw; w += gridDim.x * kBlock) {
@@ -875,6 +892,10 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
         cnt[w] = n <= cap ? n : -1;
+        if (n == 0)  // no fluid within reach until the next rebuild: the values of empty sums
+            adami_finish(P, w, xw.w, 0.0, 0.0, pw, rhow, wpv);
+        else
+            active[atomicAdd(nactive, 1)] = w;
     }
 }
 
@@ -1817,12 +1838,13 @@ struct tpsph_handle {
     int *ord = nullptr, *bcount = nullptr, *bstart = nullptr;
     int nbricks = 0;
     bool bricks = false;
-    int *brow_gb = nullptr, *brow_lds = nullptr, *bntot = nullptr, *nel = nullptr, *nne = nullptr,
-        *bwork = nullptr;
+    int *brow_gb = nullptr, *brow_lds = nullptr, *bntot = nullptr, *nel = nullptr, *bwork = nullptr;
     unsigned short *nbr16 = nullptr;
-    // Verlet lists of the wall particles over the fluid (Adami pass)
-    int *anbr = nullptr, *acnt = nullptr;
+    // Verlet lists of the wall particles over the fluid (Adami pass), active wall particles
+    int *anbr = nullptr, *acnt = nullptr, *wactive = nullptr;
     int lw = 0, cap_a = 0;
+    // counters rebuilt with the lists: [0] non-empty bricks, [1] active wall particles
+    int *counters = nullptr;
     // tiled fluid kernel
     bool tiled = false;
     int tgx = 1, tgy = 1, ntiles_all = 0, n_cu = 256;
@@ -1969,6 +1991,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
         hipLaunchKernelGGL((k_gather<ND, 1>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, nullptr, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
         if (h->search == TPSPH_SEARCH_LISTS) {
+            hipLaunchKernelGGL((k_fill_i32), dim3(1), dim3(kBlock), 0, h->stream, h->flag, h->counters, 0, 2);
             // brick order (f_cell / f_rank are free again after the cell list build)
             hipLaunchKernelGGL((k_brick_count<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
                 h->flag, h->P, h->xs, h->ys, h->zs, h->f_cell, h->f_rank, h->bcount);
@@ -1979,17 +2002,17 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
                 h->flag, h->P, h->fluid_view(), h->wall_view(), h->ord, h->ls, h->cap_f, h->nbr,
                 h->cnt, h->cap_w, h->wnbr, h->wcnt);
             if (h->bricks) {
-                hipLaunchKernelGGL((k_fill_i32), dim3(1), dim3(1), 0, h->stream, h->flag, h->nne, 0, 1);
                 hipLaunchKernelGGL((k_brick_union<ND>), dim3(std::max(1, std::min(h->nbricks, kGateBlocks))),
                     dim3(kBlock), 0, h->stream, h->flag, h->P, h->f_start, h->bstart, h->nbricks,
-                    h->brow_gb, h->brow_lds, h->bntot, h->nel, h->nne);
+                    h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters);
                 hipLaunchKernelGGL((k_brick_lists<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
                     h->flag, h->P, h->fluid_view(), h->ord, h->nbr, h->cnt, h->wcnt, h->brow_gb,
                     h->brow_lds, h->bntot, h->ls, h->nbr16);
             }
             if (h->nw > 0)
                 hipLaunchKernelGGL((k_wall_lists<ND>), dim3(gate_blocks(h->nw)), dim3(kBlock), 0, h->stream,
-                    h->flag, h->P, h->fluid_view(), h->wpos, h->lw, h->cap_a, h->anbr, h->acnt);
+                    h->flag, h->P, h->fluid_view(), h->wpos, h->lw, h->cap_a, h->anbr, h->acnt,
+                    h->wactive, h->counters + 1, h->pw, h->rhow, h->wpv);
             TRY(launch_check(h, "k_lists"));
         }
         // the fluid kernel of this kick lowers the flag
@@ -2005,8 +2028,9 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u)
     TRY(update_fluid<ND>(h, v, u));
     if (nw > 0) {
         TimedScope t(h, TPSPH_TIME_WALL);
-        hipLaunchKernelGGL((k_wall<ND>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P,
-            h->fluid_view(), h->wpos, h->pw, h->rhow, h->wpv, h->anbr, h->acnt, h->lw);
+        hipLaunchKernelGGL((k_wall<ND>), dim3(h->anbr ? gate_blocks(nw) : nblocks(nw)), dim3(kBlock), 0,
+            h->stream, h->P, h->fluid_view(), h->wpos, h->pw, h->rhow, h->wpv, h->anbr, h->acnt,
+            h->lw, h->wactive, h->counters ? h->counters + 1 : nullptr);
         TRY(launch_check(h, "k_wall"));
     }
     if (nf > 0) {
@@ -2014,7 +2038,7 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u)
         if (h->search == TPSPH_SEARCH_LISTS && h->bricks) {
             hipLaunchKernelGGL((k_fluid_brick<ND>), dim3(std::max(1, std::min(h->nbricks, h->n_cu))),
                 dim3(kBrickThreads), 0, h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
-                h->ord, h->bstart, h->nel, h->nne, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
+                h->ord, h->bstart, h->nel, h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
                 h->nbr16, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
         } else if (h->search == TPSPH_SEARCH_LISTS) {
             hipLaunchKernelGGL((k_fluid_list<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
@@ -2340,13 +2364,13 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         CTRY(dalloc(h, &h->nbr, (size_t)h->cap_f * h->ls));
         CTRY(dalloc(h, &h->cnt, nf));
         CTRY(dalloc(h, &h->ord, nf));
+        CTRY(dalloc(h, &h->counters, 4));
         h->bricks = P.maxr <= kBrickMaxRows;
         if (h->bricks) {
             CTRY(dalloc(h, &h->brow_gb, (size_t)h->nbricks * P.maxr));
             CTRY(dalloc(h, &h->brow_lds, (size_t)h->nbricks * (P.maxr + 1)));
             CTRY(dalloc(h, &h->bntot, h->nbricks));
             CTRY(dalloc(h, &h->nel, h->nbricks));
-            CTRY(dalloc(h, &h->nne, 1));
             CTRY(dalloc(h, &h->bwork, 1));
             CTRY(dalloc(h, &h->nbr16, (size_t)h->cap_f * h->ls));
         }
@@ -2361,6 +2385,7 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
             CTRY(dalloc(h, &h->wcnt, nf));
             CTRY(dalloc(h, &h->anbr, (size_t)h->cap_a * h->lw));
             CTRY(dalloc(h, &h->acnt, nw));
+            CTRY(dalloc(h, &h->wactive, nw));
         }
     }
     CTRY(dalloc(h, &h->tiles, (size_t)h->ntiles_all));
@@ -2715,7 +2740,7 @@ void tpsph_destroy(tpsph_handle *h)
                     h->scan_tiles, h->f_mass, h->xs, h->ys, h->zs, h->snap, h->rec, h->vol,
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
                     h->tiles, h->ntiles, h->work, h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
-                    h->brow_gb, h->brow_lds, h->bntot, h->nel, h->nne, h->bwork, h->nbr16,
+                    h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu};
     for (void *p : ptrs)
