@@ -857,13 +857,17 @@ template <int ND>
 __global__ void __launch_bounds__(kBlock)
     k_lists(const int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
             int ls, int cap_f, int *__restrict__ nbr, int *__restrict__ cnt, int cap_w,
-            int *__restrict__ wnbr, int *__restrict__ wcnt)
+            int *__restrict__ wnbr, int *__restrict__ wcnt, int *__restrict__ wtargets,
+            int *__restrict__ nwtargets)
 {
     if (skip(flag)) return;
     for (int t = blockIdx.x * kBlock + threadIdx.x; t < P.nf; t += gridDim.x * kBlock) {
         const int s = ord[t];
         build_list<ND, false>(P, F, Wv, s, t, ls, cap_f, nbr, cnt);
-        if (P.nw > 0) build_list<ND, true>(P, F, Wv, s, t, ls, cap_w, wnbr, wcnt);
+        if (P.nw > 0) {
+            build_list<ND, true>(P, F, Wv, s, t, ls, cap_w, wnbr, wcnt);
+            if (wtargets && wcnt[t] != 0) wtargets[atomicAdd(nwtargets, 1)] = t;
+        }
     }
 }
 
@@ -1448,7 +1452,7 @@ __global__ void __launch_bounds__(kBlock)
         const int bx = c0 / P.bc[0], by = c1 / P.bc[1], bz = c2 / P.bc[2];
         const int b = bx + P.nb[0] * (by + P.nb[1] * bz);
         const int n = cnt[t];
-        bool ok = n >= 0 && (P.nw == 0 || wcnt[t] >= 0);
+        bool ok = n >= 0;
         const int uy0 = by * P.bc[1] - P.ur[1], uz0 = (ND == 3) ? bz * P.bc[2] - P.ur[2] : 0;
         const int *gb = brow_gb + (size_t)b * P.maxr;
         const int *lds = brow_lds + (size_t)b * (P.maxr + 1);
@@ -1473,12 +1477,13 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Staged union: one record per particle (AoS, so one address per candidate and paired
+// 8-byte reads): 3D {x, y, z, rho, vx, vy, vz, p, V}, 2D {x, y, rho, vx, vy, p, V}.  With a
+// global brick the same bytes hold the lanes' int lists.
 template <int ND>
 struct BrickSmem {
-    // staged union (SoA); with a global brick the same bytes hold the lanes' int lists
-    double x[kBrickCap], y[kBrickCap], z[ND == 3 ? kBrickCap : 1], rho[kBrickCap];
-    double vx[kBrickCap], vy[kBrickCap], vz[ND == 3 ? kBrickCap : 1], p[kBrickCap];
-    double V[kBrickCap];
+    static constexpr int kRec = ND == 3 ? 9 : 7;
+    double rec[kBrickCap * kRec];
     int row_gb[kBrickMaxRows], row_lds[kBrickMaxRows + 1];
     int item;
 };
@@ -1486,15 +1491,23 @@ struct BrickSmem {
 template <int ND>
 __device__ __forceinline__ FluidNbr lds_fluid(const BrickSmem<ND> &S, int e)
 {
+    const double *r = S.rec + BrickSmem<ND>::kRec * e;
     FluidNbr n;
-    n.a = make_double4(S.x[e], S.y[e], (ND == 3) ? S.z[e] : 0.0, S.rho[e]);
-    n.b = make_double4(S.vx[e], S.vy[e], (ND == 3) ? S.vz[e] : 0.0, S.p[e]);
-    n.v = make_double2(S.V[e], S.V[e] * S.rho[e]);
+    if constexpr (ND == 3) {
+        n.a = make_double4(r[0], r[1], r[2], r[3]);
+        n.b = make_double4(r[4], r[5], r[6], r[7]);
+        n.v = make_double2(r[8], r[8] * r[3]);
+    } else {
+        n.a = make_double4(r[0], r[1], 0.0, r[2]);
+        n.b = make_double4(r[3], r[4], 0.0, r[5]);
+        n.v = make_double2(r[6], r[6] * r[2]);
+    }
     return n;
 }
 
-// Fluid kernel over the bricks (one block of kBrickThreads per non-empty brick, taken from a
-// work counter that k_gather<0> resets).
+// Fluid-fluid kernel over the bricks (one block of kBrickThreads per non-empty brick, taken
+// from a work counter that k_gather<0> resets).  Targets with wall candidates are completed
+// by k_fluid_walls.
 template <int ND>
 __global__ void __launch_bounds__(kBrickThreads)
     k_fluid_brick(int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
@@ -1535,15 +1548,26 @@ __global__ void __launch_bounds__(kBrickThreads)
                 const int g = S.row_gb[r] + (i - S.row_lds[r]);
                 const double4 a = F.rec[2 * (size_t)g];
                 const double4 c = F.rec[2 * (size_t)g + 1];
-                S.x[i] = a.x;
-                S.y[i] = a.y;
-                if constexpr (ND == 3) S.z[i] = a.z;
-                S.rho[i] = a.w;
-                S.vx[i] = c.x;
-                S.vy[i] = c.y;
-                if constexpr (ND == 3) S.vz[i] = c.z;
-                S.p[i] = c.w;
-                S.V[i] = F.vol[g].x;
+                double *o = S.rec + BrickSmem<ND>::kRec * i;
+                if constexpr (ND == 3) {
+                    o[0] = a.x;
+                    o[1] = a.y;
+                    o[2] = a.z;
+                    o[3] = a.w;
+                    o[4] = c.x;
+                    o[5] = c.y;
+                    o[6] = c.z;
+                    o[7] = c.w;
+                    o[8] = F.vol[g].x;
+                } else {
+                    o[0] = a.x;
+                    o[1] = a.y;
+                    o[2] = a.w;
+                    o[3] = c.x;
+                    o[4] = c.y;
+                    o[5] = c.w;
+                    o[6] = F.vol[g].x;
+                }
             }
             __syncthreads();
         }
@@ -1609,38 +1633,69 @@ __global__ void __launch_bounds__(kBrickThreads)
                         P, F.start, F.x, F.y, F.z, P.hs_fluid, n < 0 && half == 0, F, Wv, lst, xa,
                         va, rho_a, p_a, inv_rho_a, acc, drho);
             }
-            double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
-            if (__any(nw != 0)) {
-                const int mw = max(nw, 0);
-                for (int k = half; k < mw; k += 2) {
-                    const WallNbr wb = load_wall(Wv, wnbr[(size_t)k * ls + col]);
-                    const double p0 = xa[0] - wb.a.x, p1 = xa[1] - wb.a.y;
-                    const double p2 = (ND == 3) ? xa[2] - wb.a.z : 0.0;
-                    const double d2 = dist2<ND>(p0, p1, p2);
-                    if (d2 <= P.R2 && d2 >= P.az2)
-                        pair_wall<ND>(P, wb, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
-                }
-                // a wall list overflow marks the brick as global, so the LDS is free here
-                if (!staged && __any(nw < 0))
-                    neighbour_sweep<ND, true, kFallbackCap, kBrickThreads>(
-                        P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0 && half == 0, F, Wv, lst,
-                        xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
-            }
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                acc[d] += __shfl_xor(acc[d], 1);
-                accw[d] += __shfl_xor(accw[d], 1);
-            }
+            for (int d = 0; d < 3; d++) acc[d] += __shfl_xor(acc[d], 1);
             drho += __shfl_xor(drho, 1);
-            drhow += __shfl_xor(drhow, 1);
             if (active && half == 0) {
+                // with wall candidates k_fluid_walls adds the wall sum and then gravity
                 const int i = F.perm[s];
+                const bool walls = nw != 0;
 #pragma unroll
-                for (int d = 0; d < ND; d++) dv[(size_t)i * (ND + 1) + d] = (acc[d] + accw[d]) + P.g[d];
-                dv[(size_t)i * (ND + 1) + ND] = drho + drhow;
+                for (int d = 0; d < ND; d++)
+                    dv[(size_t)i * (ND + 1) + d] = walls ? acc[d] : acc[d] + P.g[d];
+                dv[(size_t)i * (ND + 1) + ND] = drho;
             }
         }
         __syncthreads();  // the LDS is staged again for the next brick
+    }
+}
+
+// Wall sums of the targets with wall candidates (wtargets, built at the rebuild) after
+// k_fluid_brick: dv = (dv + wall sum) + gravity, the reference's order of the sums.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_fluid_walls(DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
+                  const int *__restrict__ wtargets, const int *__restrict__ nwtargets,
+                  const int *__restrict__ wnbr, const int *__restrict__ wcnt, int ls,
+                  double *__restrict__ dv)
+{
+    __shared__ int lists[kCap * kBlock];  // only for particles whose wall list overflowed
+    int *lst = lists + threadIdx.x;
+    const int nt = *nwtargets;
+    for (int base = blockIdx.x * kBlock; base < nt; base += gridDim.x * kBlock) {
+        const int idx = base + threadIdx.x;
+        const bool active = idx < nt;
+        const int t = wtargets[active ? idx : base];
+        const int s = ord[t];
+        const double4 a = F.rec[2 * (size_t)s];
+        const double4 c = F.rec[2 * (size_t)s + 1];
+        const double xa[3] = {a.x, a.y, a.z}, va[3] = {c.x, c.y, c.z};
+        const double rho_a = a.w, p_a = c.w, inv_rho_a = 1.0 / rho_a;
+        const int nw = active ? wcnt[t] : 0;
+        double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
+        const int m = max(nw, 0);
+        int j = (m > 0) ? wnbr[t] : 0;
+        WallNbr cur = load_wall(Wv, j);
+        for (int k = 0; k < m; k++) {
+            const int jn = (k + 1 < m) ? wnbr[(size_t)(k + 1) * ls + t] : j;
+            const WallNbr nxt = load_wall(Wv, jn);
+            const double p0 = xa[0] - cur.a.x, p1 = xa[1] - cur.a.y;
+            const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
+            const double d2 = dist2<ND>(p0, p1, p2);
+            if (d2 <= P.R2 && d2 >= P.az2)
+                pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+            cur = nxt;
+            j = jn;
+        }
+        if (__any(nw < 0))
+            neighbour_sweep<ND, true>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0, F, Wv, lst,
+                                      xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+        if (active) {
+            double *o = dv + (size_t)F.perm[s] * (ND + 1);
+#pragma unroll
+            for (int d = 0; d < ND; d++) o[d] = (o[d] + accw[d]) + P.g[d];
+            o[ND] = o[ND] + drhow;
+        }
     }
 }
 
@@ -1843,8 +1898,9 @@ struct tpsph_handle {
     // Verlet lists of the wall particles over the fluid (Adami pass), active wall particles
     int *anbr = nullptr, *acnt = nullptr, *wactive = nullptr;
     int lw = 0, cap_a = 0;
-    // counters rebuilt with the lists: [0] non-empty bricks, [1] active wall particles
-    int *counters = nullptr;
+    // counters rebuilt with the lists: [0] non-empty bricks, [1] active wall particles,
+    // [2] fluid particles with wall candidates (wtargets)
+    int *counters = nullptr, *wtargets = nullptr;
     // tiled fluid kernel
     bool tiled = false;
     int tgx = 1, tgy = 1, ntiles_all = 0, n_cu = 256;
@@ -1991,7 +2047,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
         hipLaunchKernelGGL((k_gather<ND, 1>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, nullptr, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
         if (h->search == TPSPH_SEARCH_LISTS) {
-            hipLaunchKernelGGL((k_fill_i32), dim3(1), dim3(kBlock), 0, h->stream, h->flag, h->counters, 0, 2);
+            hipLaunchKernelGGL((k_fill_i32), dim3(1), dim3(kBlock), 0, h->stream, h->flag, h->counters, 0, 3);
             // brick order (f_cell / f_rank are free again after the cell list build)
             hipLaunchKernelGGL((k_brick_count<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
                 h->flag, h->P, h->xs, h->ys, h->zs, h->f_cell, h->f_rank, h->bcount);
@@ -2000,7 +2056,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
                 h->flag, nf, h->f_cell, h->f_rank, h->bstart, h->ord, h->bcount);
             hipLaunchKernelGGL((k_lists<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
                 h->flag, h->P, h->fluid_view(), h->wall_view(), h->ord, h->ls, h->cap_f, h->nbr,
-                h->cnt, h->cap_w, h->wnbr, h->wcnt);
+                h->cnt, h->cap_w, h->wnbr, h->wcnt, h->wtargets, h->counters + 2);
             if (h->bricks) {
                 hipLaunchKernelGGL((k_brick_union<ND>), dim3(std::max(1, std::min(h->nbricks, kGateBlocks))),
                     dim3(kBlock), 0, h->stream, h->flag, h->P, h->f_start, h->bstart, h->nbricks,
@@ -2040,6 +2096,10 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u)
                 dim3(kBrickThreads), 0, h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
                 h->ord, h->bstart, h->nel, h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
                 h->nbr16, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
+            if (nw > 0)
+                hipLaunchKernelGGL((k_fluid_walls<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
+                    h->P, h->fluid_view(), h->wall_view(), h->ord, h->wtargets, h->counters + 2,
+                    h->wnbr, h->wcnt, h->ls, dv);
         } else if (h->search == TPSPH_SEARCH_LISTS) {
             hipLaunchKernelGGL((k_fluid_list<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
                 h->flag, h->P, h->fluid_view(), h->wall_view(), h->ord, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
@@ -2386,6 +2446,7 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
             CTRY(dalloc(h, &h->anbr, (size_t)h->cap_a * h->lw));
             CTRY(dalloc(h, &h->acnt, nw));
             CTRY(dalloc(h, &h->wactive, nw));
+            if (h->bricks) CTRY(dalloc(h, &h->wtargets, nf));
         }
     }
     CTRY(dalloc(h, &h->tiles, (size_t)h->ntiles_all));
@@ -2741,6 +2802,7 @@ void tpsph_destroy(tpsph_handle *h)
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
                     h->tiles, h->ntiles, h->work, h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,
+                    h->wtargets,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu};
     for (void *p : ptrs)
