@@ -415,7 +415,7 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
                          const double *__restrict__ m, double *__restrict__ xs,
                          double *__restrict__ ys, double *__restrict__ zs,
                          double4 *__restrict__ rec, double2 *__restrict__ vol,
-                         double *__restrict__ snap)
+                         double *__restrict__ snap, int *__restrict__ iperm)
 {
     const int f = *flag;
     // MODE 0 runs first in every kick: it resets the work counter of the brick kernel
@@ -451,6 +451,7 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
     } else {
 #pragma unroll
         for (int d = 0; d < ND; d++) snap[d * nf + s] = x[d];
+        iperm[i] = s;
     }
     }
 }
@@ -496,9 +497,10 @@ __device__ __forceinline__ void adami_finish(const DevParams &P, int w, double m
 // alist / acnt: the wall particles' Verlet lists of fluid candidates (slot-major, row
 // length lw), or nullptr for the cell walk (every wall particle).  With lists only the
 // active wall particles (a non-empty list; active / nactive, built at the rebuild) are
-// visited: the others keep the values the rebuild gave them.  A count of -1 (overflow)
-// walks the cells.
-template <int ND>
+// visited: the others keep the values the rebuild gave them.  LANES lanes share a wall
+// particle (lane q takes list entries q, q + LANES, ...; the partial sums are added at the
+// end).  A count of -1 (overflow) walks the cells on the first lane.
+template <int ND, int LANES>
 __global__ void __launch_bounds__(kBlock)
     k_wall(DevParams P, FluidView F, const double4 *__restrict__ wpos, double *__restrict__ pw,
            double *__restrict__ rhow, double2 *__restrict__ wpv, const int *__restrict__ alist,
@@ -506,22 +508,30 @@ __global__ void __launch_bounds__(kBlock)
            const int *__restrict__ nactive)
 {
     const int na = alist ? *nactive : P.nw;
-    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < na; a += gridDim.x * blockDim.x) {
-        const int w = alist ? active[a] : a;
+    const int q = threadIdx.x % LANES;
+    // the loop bound is uniform per LANES lanes, so the partners shuffle together
+    for (int g = (blockIdx.x * blockDim.x + threadIdx.x) / LANES; g < na;
+         g += gridDim.x * blockDim.x / LANES) {
+        const int w = alist ? active[g] : g;
         const double4 xw = wpos[w];
         double pressure = 0.0, volume = 0.0;
         const int n = alist ? acnt[w] : -1;
         if (n >= 0) {
-            for (int k = 0; k < n; k++)
+            for (int k = q; k < n; k += LANES)
                 adami_term<ND>(P, F, xw, alist[(size_t)k * lw + w], pressure, volume);
-        } else {
+        } else if (q == 0) {
             Pencils<ND> it;
             it.init(P, xw.x, xw.y, xw.z, P.hs_fluid);
             int jb, je;
             while (it.next(P, F.start, jb, je))
                 for (int j = jb; j < je; j++) adami_term<ND>(P, F, xw, j, pressure, volume);
         }
-        adami_finish(P, w, xw.w, pressure, volume, pw, rhow, wpv);
+#pragma unroll
+        for (int o = 1; o < LANES; o <<= 1) {
+            pressure += __shfl_xor(pressure, o);
+            volume += __shfl_xor(volume, o);
+        }
+        if (q == 0) adami_finish(P, w, xw.w, pressure, volume, pw, rhow, wpv);
     }
 }
 
@@ -1652,6 +1662,8 @@ __global__ void __launch_bounds__(kBrickThreads)
 
 // Wall sums of the targets with wall candidates (wtargets, built at the rebuild) after
 // k_fluid_brick: dv = (dv + wall sum) + gravity, the reference's order of the sums.
+// Four lanes share a target: lane q takes list entries q, q + 4, ...; the sums are added
+// across the four lanes before the write.
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
     k_fluid_walls(DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
@@ -1659,11 +1671,13 @@ __global__ void __launch_bounds__(kBlock)
                   const int *__restrict__ wnbr, const int *__restrict__ wcnt, int ls,
                   double *__restrict__ dv)
 {
+    constexpr int kLanes = 4;
     __shared__ int lists[kCap * kBlock];  // only for particles whose wall list overflowed
     int *lst = lists + threadIdx.x;
     const int nt = *nwtargets;
-    for (int base = blockIdx.x * kBlock; base < nt; base += gridDim.x * kBlock) {
-        const int idx = base + threadIdx.x;
+    const int q = threadIdx.x % kLanes;
+    for (int base = blockIdx.x * (kBlock / kLanes); base < nt; base += gridDim.x * (kBlock / kLanes)) {
+        const int idx = base + (int)threadIdx.x / kLanes;
         const bool active = idx < nt;
         const int t = wtargets[active ? idx : base];
         const int s = ord[t];
@@ -1674,23 +1688,24 @@ __global__ void __launch_bounds__(kBlock)
         const int nw = active ? wcnt[t] : 0;
         double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
         const int m = max(nw, 0);
-        int j = (m > 0) ? wnbr[t] : 0;
-        WallNbr cur = load_wall(Wv, j);
-        for (int k = 0; k < m; k++) {
-            const int jn = (k + 1 < m) ? wnbr[(size_t)(k + 1) * ls + t] : j;
-            const WallNbr nxt = load_wall(Wv, jn);
+        for (int k = q; k < m; k += kLanes) {
+            const WallNbr cur = load_wall(Wv, wnbr[(size_t)k * ls + t]);
             const double p0 = xa[0] - cur.a.x, p1 = xa[1] - cur.a.y;
             const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
             const double d2 = dist2<ND>(p0, p1, p2);
             if (d2 <= P.R2 && d2 >= P.az2)
                 pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
-            cur = nxt;
-            j = jn;
         }
         if (__any(nw < 0))
-            neighbour_sweep<ND, true>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0, F, Wv, lst,
-                                      xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
-        if (active) {
+            neighbour_sweep<ND, true>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0 && q == 0, F,
+                                      Wv, lst, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+#pragma unroll
+        for (int o = 1; o < kLanes; o <<= 1) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) accw[d] += __shfl_xor(accw[d], o);
+            drhow += __shfl_xor(drhow, o);
+        }
+        if (active && q == 0) {
             double *o = dv + (size_t)F.perm[s] * (ND + 1);
 #pragma unroll
             for (int d = 0; d < ND; d++) o[d] = (o[d] + accw[d]) + P.g[d];
@@ -1736,11 +1751,19 @@ __global__ void k_drift(int nf, const double *__restrict__ v, double *__restrict
 // before v is updated, as the integrator evaluates kick! and drift! at the same stage state.
 // A block owns kBlock particles and walks their u and then their v entries as flat,
 // contiguous ranges, so every load and store of a wavefront covers whole cache lines.
+// Then, what k_gather<0> does for the next kick: the cell-sorted records of the new state
+// (positions, velocity, density, pressure by the state equation, volume), the skin check
+// and the reset of the brick work counter.  iperm[i] is the sorted index of particle i of
+// the last rebuild.
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
-    k_ck_stage(int nf, double *__restrict__ v, double *__restrict__ u, double *__restrict__ tv,
-               double *__restrict__ tu, const double *__restrict__ dv, double a, double b,
-               double dt, int first)
+    k_ck_stage_gather(int *flag, int *work, DevParams P, int nf, double *__restrict__ v,
+                      double *__restrict__ u, double *__restrict__ tv, double *__restrict__ tu,
+                      const double *__restrict__ dv, double a, double b, double dt, int first,
+                      const int *__restrict__ iperm, const double *__restrict__ m,
+                      double *__restrict__ xs, double *__restrict__ ys, double *__restrict__ zs,
+                      double4 *__restrict__ rec, double2 *__restrict__ vol,
+                      const double *__restrict__ snap)
 {
     const int i0 = blockIdx.x * kBlock;
     const int np = min(kBlock, nf - i0);
@@ -1760,6 +1783,33 @@ __global__ void __launch_bounds__(kBlock)
         tv[k] = t;
         v[k] = fma(b, t, v[k]);
     }
+    __syncthreads();  // the block's new u and v are complete
+    if (work && blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
+    const int i = i0 + (int)threadIdx.x;
+    if (i >= nf) return;
+    const size_t nfs = (size_t)nf;
+    const int s = iperm[i];
+    double x[3] = {0.0, 0.0, 0.0}, vv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+        x[d] = u[(size_t)i * ND + d];
+        vv[d] = v[(size_t)i * (ND + 1) + d];
+    }
+    const double rho = v[(size_t)i * (ND + 1) + ND];
+    const double mass = m[i];
+    xs[s] = x[0];
+    ys[s] = x[1];
+    if constexpr (ND == 3) zs[s] = x[2];
+    rec[2 * (size_t)s] = make_double4(x[0], x[1], x[2], rho);
+    rec[2 * (size_t)s + 1] = make_double4(vv[0], vv[1], vv[2], state_equation(P, rho));
+    vol[s] = make_double2(mass / rho, mass);
+    double e2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+        const double e = x[d] - snap[d * nfs + s];
+        e2 = fma(e, e, e2);
+    }
+    if (!(e2 <= P.half_skin2)) atomicOr(flag, 1);
 }
 
 __global__ void k_rk_stage(int64_t n, double *__restrict__ y, double *__restrict__ tmp,
@@ -1877,6 +1927,7 @@ struct tpsph_handle {
     int *scan_tiles = nullptr;
     double *f_mass = nullptr;  // caller order
     double *xs = nullptr, *ys = nullptr, *zs = nullptr, *snap = nullptr;
+    int *iperm = nullptr;  // sorted index of every caller particle (last rebuild)
     double4 *rec = nullptr;
     double2 *vol = nullptr;
     // walls (static, cell-sorted) + per kick
@@ -2024,14 +2075,15 @@ int build_cells(tpsph_handle *h, const int *flag, int n, const double *x, int *c
 
 // Fluid cell list (rebuilt only when flagged) + cell-sorted gather of (v, u).
 template <int ND>
-int update_fluid(tpsph_handle *h, const double *v, const double *u)
+int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathered)
 {
     const int nf = h->nf;
     if (nf == 0) return TPSPH_OK;
-    {
+    if (!gathered) {  // else the stage kernel wrote the records of (v, u)
         TimedScope t(h, TPSPH_TIME_GATHER);
         hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
-            h->flag, h->bwork, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
+            h->flag, h->bwork, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
+            h->snap, nullptr);
         TRY(launch_check(h, "k_gather<0>"));
     }
     {
@@ -2045,7 +2097,8 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
             TRY(launch_check(h, "k_tiles"));
         }
         hipLaunchKernelGGL((k_gather<ND, 1>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
-            h->flag, nullptr, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
+            h->flag, nullptr, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
+            h->snap, h->iperm);
         if (h->search == TPSPH_SEARCH_LISTS) {
             hipLaunchKernelGGL((k_fill_i32), dim3(1), dim3(kBlock), 0, h->stream, h->flag, h->counters, 0, 3);
             // brick order (f_cell / f_rank are free again after the cell list build)
@@ -2078,15 +2131,20 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u)
 }
 
 template <int ND>
-int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u)
+int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, bool gathered)
 {
     const int nf = h->nf, nw = h->nw;
-    TRY(update_fluid<ND>(h, v, u));
+    TRY(update_fluid<ND>(h, v, u, gathered));
     if (nw > 0) {
         TimedScope t(h, TPSPH_TIME_WALL);
-        hipLaunchKernelGGL((k_wall<ND>), dim3(h->anbr ? gate_blocks(nw) : nblocks(nw)), dim3(kBlock), 0,
-            h->stream, h->P, h->fluid_view(), h->wpos, h->pw, h->rhow, h->wpv, h->anbr, h->acnt,
-            h->lw, h->wactive, h->counters ? h->counters + 1 : nullptr);
+        if (h->anbr)
+            hipLaunchKernelGGL((k_wall<ND, 4>), dim3(gate_blocks(4 * (int64_t)nw)), dim3(kBlock), 0,
+                h->stream, h->P, h->fluid_view(), h->wpos, h->pw, h->rhow, h->wpv, h->anbr, h->acnt,
+                h->lw, h->wactive, h->counters + 1);
+        else
+            hipLaunchKernelGGL((k_wall<ND, 1>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P,
+                h->fluid_view(), h->wpos, h->pw, h->rhow, h->wpv, nullptr, nullptr, h->lw, nullptr,
+                nullptr);
         TRY(launch_check(h, "k_wall"));
     }
     if (nf > 0) {
@@ -2097,7 +2155,7 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u)
                 h->ord, h->bstart, h->nel, h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
                 h->nbr16, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
             if (nw > 0)
-                hipLaunchKernelGGL((k_fluid_walls<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
+                hipLaunchKernelGGL((k_fluid_walls<ND>), dim3(gate_blocks(4 * (int64_t)nf)), dim3(kBlock), 0, h->stream,
                     h->P, h->fluid_view(), h->wall_view(), h->ord, h->wtargets, h->counters + 2,
                     h->wnbr, h->wcnt, h->ls, dv);
         } else if (h->search == TPSPH_SEARCH_LISTS) {
@@ -2119,9 +2177,11 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u)
     return TPSPH_OK;
 }
 
-int kick_dispatch(tpsph_handle *h, double *dv, const double *v, const double *u)
+// gathered: the cell-sorted records already hold (v, u) (written by k_ck_stage_gather)
+int kick_dispatch(tpsph_handle *h, double *dv, const double *v, const double *u,
+                  bool gathered = false)
 {
-    return h->nd == 2 ? kick_impl<2>(h, dv, v, u) : kick_impl<3>(h, dv, v, u);
+    return h->nd == 2 ? kick_impl<2>(h, dv, v, u, gathered) : kick_impl<3>(h, dv, v, u, gathered);
 }
 
 bool is_host(const tpsph_handle *h) { return h->params.memory == TPSPH_MEM_HOST; }
@@ -2412,6 +2472,7 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     CTRY(dalloc(h, &h->ys, (size_t)nf + kPad));
     CTRY(dalloc(h, &h->zs, (size_t)nf + kPad));
     CTRY(dalloc(h, &h->snap, (size_t)nd * nf));
+    CTRY(dalloc(h, &h->iperm, nf));
     CTRY(dalloc(h, &h->rec, 2 * (size_t)nf));
     CTRY(dalloc(h, &h->vol, nf));
     CTRY(dalloc(h, &h->w_start, (size_t)ncells + 1));
@@ -2738,15 +2799,23 @@ int tpsph_solve_ck2n54(tpsph_handle *h, double *v_ode, double *u_ode, double t0,
         TRY(dalloc(h, &h->sol_tv, v_len(h)));
         TRY(dalloc(h, &h->sol_tu, u_len(h)));
     }
+    // every stage kernel also writes the cell-sorted records of the new state, so the kicks
+    // after the first skip the gather
+    bool gathered = false;
     for (int64_t s = 0; s < nsteps; s++) {
         for (int i = 0; i < 5; i++) {
-            TRY(kick_dispatch(h, h->sol_dv, v, u));
+            TRY(kick_dispatch(h, h->sol_dv, v, u, gathered));
             TimedScope ts(h, TPSPH_TIME_STAGE);
             if (h->nd == 2)
-                hipLaunchKernelGGL((k_ck_stage<2>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt, i == 0);
+                hipLaunchKernelGGL((k_ck_stage_gather<2>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
+                    h->flag, h->bwork, h->P, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt,
+                    i == 0, h->iperm, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
             else
-                hipLaunchKernelGGL((k_ck_stage<3>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt, i == 0);
-            TRY(launch_check(h, "k_ck_stage"));
+                hipLaunchKernelGGL((k_ck_stage_gather<3>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
+                    h->flag, h->bwork, h->P, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt,
+                    i == 0, h->iperm, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
+            TRY(launch_check(h, "k_ck_stage_gather"));
+            gathered = true;
         }
     }
     if (is_host(h)) {
@@ -2802,7 +2871,7 @@ void tpsph_destroy(tpsph_handle *h)
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
                     h->tiles, h->ntiles, h->work, h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,
-                    h->wtargets,
+                    h->wtargets, h->iperm,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu};
     for (void *p : ptrs)
