@@ -1612,9 +1612,12 @@ __global__ void __launch_bounds__(kBrickThreads)
                 // four LDS indices per 8-byte load, one group ahead of the pair terms
                 const ushort4 *l4 = reinterpret_cast<const ushort4 *>(nbr16);
                 const int ng = (m + 3) >> 2;
-                ushort4 cur4 = (half < ng) ? l4[(size_t)half * ls + col] : make_ushort4(0, 0, 0, 0);
+                // rows clamped into the list (row 0 always exists) rather than a select between
+                // a load and a register: the select made the compiler keep cur4 in scratch
+                // and issue a flat load, whose wait also held up every LDS read
+                ushort4 cur4 = l4[(size_t)min(half, max(ng - 1, 0)) * ls + col];
                 for (int g = half; g < ng; g += 2) {
-                    const ushort4 nxt4 = (g + 2 < ng) ? l4[(size_t)(g + 2) * ls + col] : cur4;
+                    const ushort4 nxt4 = l4[(size_t)min(g + 2, ng - 1) * ls + col];
                     // the candidates are independent chains without branches: two at a time
                     // interleave within the register budget of four waves per SIMD
 #pragma unroll 1
