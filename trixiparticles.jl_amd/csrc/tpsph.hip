@@ -53,7 +53,8 @@ constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kBrickThreads = 1024;   // threads of the brick kernel (one block per CU), two
                                       // per target
-constexpr int kBrickCap = 2048;       // particles a brick can stage in LDS
+constexpr int kBrickCap = 2048;       // LDS records of a brick: the union plus one far record
+constexpr double kFar = 1e100;        // coordinate of the far record (never within R)
 constexpr int kBrickMaxRows = 256;    // union rows per brick (one scan block)
 constexpr int kGateBlocks = 1024;     // grid-stride kernels behind the rebuild flag: a small
                                       // grid keeps the launch cheap when the flag is down
@@ -555,26 +556,22 @@ __global__ void __launch_bounds__(kBlock)
 // the range scaling: arguments here are normal and positive); results agree with the
 // correctly rounded operations to an ulp or so.
 // ------------------------------------------------------------------------------------------
+// sqrt(x) and 1/x for normal positive x from the hardware approximations v_rsq_f64 and
+// v_rcp_f64 (relative error about 5e-8, measured) refined by one step of Goldschmidt's
+// square-root iteration and of Newton's reciprocal iteration: each step squares the
+// relative error, so the results are within about 5e-15 (measured 4.2e-15 and 2.2e-15 over
+// 2^20 inputs from 2^-41 to 1), far inside the 1e-12 parity bar of the pair terms.
 __device__ __forceinline__ double sqrt_pos(double x)
 {
-    double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, hh = 0.5 * y;
-    const double r = fma(-hh, g, 0.5);
-    g = fma(g, r, g);
-    hh = fma(hh, r, hh);
-    double d = fma(-g, g, x);
-    g = fma(d, hh, g);
-    d = fma(-g, g, x);
-    return fma(d, hh, g);
+    const double y = __builtin_amdgcn_rsq(x);
+    const double g = x * y, hh = 0.5 * y;
+    return fma(g, fma(-hh, g, 0.5), g);
 }
 
 __device__ __forceinline__ double rcp_pos(double x)
 {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
 }
 
 struct FluidNbr {
@@ -1381,9 +1378,10 @@ __global__ void __launch_bounds__(TileCfg<ND>::THREADS)
 // R + skin of the brick box, as contiguous ranges of the cell-sorted arrays.  Every Verlet list
 // entry of a target lies in the union of its brick, so after one coalesced staging pass of the
 // union into LDS the pair terms read every candidate from LDS.  The lists also store the
-// candidate's LDS index (u16, four slots per ushort4).  A brick whose union exceeds kBrickCap,
-// or with a target whose list overflowed, is marked (bntot = INT_MAX) and its targets read
-// their candidates from global memory.
+// candidate's LDS index (u16, four slots per ushort4; the slots past the end of a list hold
+// the index of the far record, kBrickCap - 1, which no target reaches).  A brick whose union
+// does not leave room for the far record, or with a target whose list overflowed, is marked
+// (bntot = INT_MAX) and its targets read their candidates from global memory.
 // ------------------------------------------------------------------------------------------
 
 // Union rows of every brick (one block per brick, grid-stride) and the list of non-empty
@@ -1477,18 +1475,21 @@ __global__ void __launch_bounds__(kBlock)
                 break;
             }
             const int e = lds[r] + (j - gb[r]);
-            if (j < gb[r] || e >= lds[r + 1] || e >= kBrickCap) {
+            if (j < gb[r] || e >= lds[r + 1] || e >= kBrickCap - 1) {
                 ok = false;
                 break;
             }
             nbr16[((size_t)(k >> 2) * ls + t) * 4 + (k & 3)] = (unsigned short)e;
         }
+        for (int k = n; ok && (k & 3); k++)
+            nbr16[((size_t)(k >> 2) * ls + t) * 4 + (k & 3)] = (unsigned short)(kBrickCap - 1);
         if (!ok) bntot[b] = INT_MAX;
     }
 }
 
 // Staged union: one record per particle (AoS, so one address per candidate and paired
-// 8-byte reads): 3D {x, y, z, rho, vx, vy, vz, p, V}, 2D {x, y, rho, vx, vy, p, V}.  With a
+// 8-byte reads): 3D {x, y, z, rho, vx, vy, vz, p, V}, 2D {x, y, rho, vx, vy, p, V}; the last
+// record is the far record that pads the list groups (finite values, zero volume).  With a
 // global brick the same bytes hold the lanes' int lists.
 template <int ND>
 struct BrickSmem {
@@ -1546,12 +1547,15 @@ __global__ void __launch_bounds__(kBrickThreads)
         const int b = nel[item];
         const int t0 = bstart[b], t1 = bstart[b + 1];
         const int ntot = bntot[b];
-        const bool staged = ntot <= kBrickCap;
+        const bool staged = ntot <= kBrickCap - 1;
         if (staged) {
             for (int r = tid; r <= P.maxr; r += kBrickThreads) {
                 if (r < P.maxr) S.row_gb[r] = brow_gb[(size_t)b * P.maxr + r];
                 S.row_lds[r] = brow_lds[(size_t)b * (P.maxr + 1) + r];
             }
+            constexpr int kRec = BrickSmem<ND>::kRec;
+            if (tid < kRec)  // the far record: coordinates kFar, rho 1, the rest 0
+                S.rec[kRec * (kBrickCap - 1) + tid] = (tid < ND) ? kFar : (tid == ND ? 1.0 : 0.0);
             __syncthreads();
             for (int i = tid; i < ntot; i += kBrickThreads) {
                 const int r = bsearch_le(S.row_lds, P.maxr + 1, i);
@@ -1622,13 +1626,11 @@ __global__ void __launch_bounds__(kBrickThreads)
                     // interleave within the register budget of four waves per SIMD
 #pragma unroll 1
                     for (int h2 = 0; h2 < 2; h2++) {
-                        const int k = 4 * g + 2 * h2;
-                        const int ea = h2 ? cur4.z : cur4.x, eb = h2 ? cur4.w : cur4.y;
-                        const bool va0 = k < m, vb0 = k + 1 < m;
-                        const FluidNbr na = lds_fluid<ND>(S, va0 ? ea : 0);
-                        const FluidNbr nb = lds_fluid<ND>(S, vb0 ? eb : 0);
-                        pair_fluid_masked<ND>(P, na, va0, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
-                        pair_fluid_masked<ND>(P, nb, vb0, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                        // slots past the end of the list read the far record: exact zeros
+                        const FluidNbr na = lds_fluid<ND>(S, h2 ? cur4.z : cur4.x);
+                        const FluidNbr nb = lds_fluid<ND>(S, h2 ? cur4.w : cur4.y);
+                        pair_fluid_masked<ND>(P, na, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                        pair_fluid_masked<ND>(P, nb, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
                     }
                     cur4 = nxt4;
                 }
