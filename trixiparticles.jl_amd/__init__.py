@@ -4,15 +4,16 @@ Python host side of the HIP library ``libtpsph.so`` (C ABI in ``include/tpsph.h`
 reference's system/model constructors, setups and its ``Semidiscretization`` /
 ``semidiscretize`` / ``kick!`` / ``drift!`` ODE surface.  The directory name contains a dot,
 so it is imported through :func:`load_package` (registered as ``trixiparticles_jl_amd``)."""
-from .examples import DamBreak, dam_break
+from .examples import DamBreak, dam_break, marrone_dam_break_2d
 from .schemes import (AdamiPressureExtrapolation, ArtificialViscosityMonaghan,
                       BoundaryModelDummyParticles, ContinuityDensity,
                       DensityDiffusionMolteniColagrossi, SchoenbergCubicSplineKernel,
                       StateEquationCole, WallBoundarySystem, WeaklyCompressibleSPHSystem,
                       WendlandC2Kernel)
 from .semidiscretization import (CK2N54_A, CK2N54_B, CK2N54_C, CarpenterKennedy2N54,
-                                 DynamicalODEProblem, Semidiscretization, drift, kick,
-                                 semidiscretize, solve)
+                                 DynamicalODEProblem, Semidiscretization, calculate_dt, drift,
+                                 kick, semidiscretize, solve, solve_tstops)
+from .postprocess import compute_pressure, interpolate_line, interpolate_points, max_x_coord
 from .setups import InitialCondition, RectangularShape, RectangularTank, splitmix64_uniform
 from . import native
 
@@ -21,7 +22,10 @@ __all__ = [
     "ContinuityDensity", "DensityDiffusionMolteniColagrossi", "SchoenbergCubicSplineKernel",
     "StateEquationCole", "WallBoundarySystem", "WeaklyCompressibleSPHSystem", "WendlandC2Kernel",
     "CK2N54_A", "CK2N54_B", "CK2N54_C", "CarpenterKennedy2N54", "DynamicalODEProblem",
-    "Semidiscretization", "drift", "kick", "semidiscretize", "solve", "InitialCondition",
+    "Semidiscretization", "calculate_dt", "drift", "kick", "semidiscretize", "solve",
+    "solve_tstops", "InitialCondition",
     "RectangularShape", "RectangularTank", "splitmix64_uniform", "DamBreak", "dam_break",
+    "marrone_dam_break_2d", "compute_pressure", "interpolate_line", "interpolate_points",
+    "max_x_coord",
     "native",
 ]

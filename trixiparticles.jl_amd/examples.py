@@ -16,7 +16,7 @@ from .schemes import (AdamiPressureExtrapolation, ArtificialViscosityMonaghan,
                       WeaklyCompressibleSPHSystem, WendlandC2Kernel)
 from .setups import RectangularTank
 
-__all__ = ["DamBreak", "dam_break"]
+__all__ = ["DamBreak", "dam_break", "marrone_dam_break_2d"]
 
 
 @dataclass
@@ -66,3 +66,20 @@ def dam_break(ndims=2, particle_spacing=0.01, fluid_size=None, tank_size=None, *
         state_equation=state_equation, clip_negative_pressure=clip_wall_pressure)
     boundary_system = WallBoundarySystem(tank.boundary, boundary_model)
     return DamBreak(tank, fluid_system, boundary_system, state_equation, particle_spacing)
+
+
+def marrone_dam_break_2d(particles_per_height=40, *, alpha=0.01):
+    """The 2D dam-break validation case (validation/dam_break_2d/setup_marrone_2011.jl with
+    examples/fluid/dam_break_2d.jl): water H = 0.6 m high and 2H wide in a closed tank
+    floor(5.366 H / dx) dx wide and 4 m high with 4 wall layers, dx = H /
+    particles_per_height, WendlandC2 with h = 2 dx, StateEquationCole(c = 20 sqrt(g H),
+    exponent 1, no clipping), Monaghan viscosity, Molteni-Colagrossi delta = 0.1, Adami walls
+    with clipping, no jitter.  ``alpha`` defaults to the 0.01 recorded in the metadata of the
+    reference's validation series."""
+    H = 0.6
+    dx = H / particles_per_height
+    tank_size = (math.floor(5.366 * H / dx) * dx, 4.0)
+    return dam_break(2, dx, fluid_size=(2 * H, H), tank_size=tank_size, alpha=alpha, delta=0.1,
+                     n_layers=4, open_top=False, jitter=0.0, smoothing_factor=2.0,
+                     exponent=1.0, water_height=H)
+

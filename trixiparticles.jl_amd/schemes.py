@@ -61,11 +61,34 @@ class WendlandC2Kernel(_Kernel):
             return (7 / (4 * math.pi)) * h_inv * h_inv
         return (21 / (16 * math.pi)) * h_inv * h_inv * h_inv
 
+    def kernel(self, r, h):
+        """``kernel(kernel, r, h)`` (smoothing_kernels.jl:30-34, 436-441), elementwise:
+        ``sigma (1 - q/2)^4 (2q + 1)`` for ``r < 2h``, else 0."""
+        r = np.asarray(r, dtype=np.float64)
+        h_inv = 1 / h
+        q = r * h_inv
+        w = self.normalization_factor(h_inv) * (1 - q / 2) ** 4 * (2 * q + 1)
+        return np.where(r < self.compact_support(h), w, 0.0)
+
 
 class SchoenbergCubicSplineKernel(_Kernel):
     """``SchoenbergCubicSplineKernel{NDIMS}()`` (smoothing_kernels.jl:161-227)."""
 
     kernel_id = 1
+
+    def normalization_factor(self, h_inv):
+        if self.ndims == 2:
+            return 10 / (7 * math.pi) * h_inv * h_inv
+        return 1 / math.pi * h_inv * h_inv * h_inv
+
+    def kernel(self, r, h):
+        """``kernel(kernel, r, h)`` (smoothing_kernels.jl:30-34, 192-199), elementwise:
+        ``sigma ((2 - q)^3 / 4 - (q < 1)(1 - q)^3)`` for ``r < 2h``, else 0."""
+        r = np.asarray(r, dtype=np.float64)
+        h_inv = 1 / h
+        q = r * h_inv
+        w = self.normalization_factor(h_inv) * ((2 - q) ** 3 / 4 - (q < 1) * (1 - q) ** 3)
+        return np.where(r < self.compact_support(h), w, 0.0)
 
 
 class StateEquationCole:

@@ -27,6 +27,7 @@ from . import native
 from .schemes import WallBoundarySystem, WeaklyCompressibleSPHSystem
 
 __all__ = ["Semidiscretization", "semidiscretize", "DynamicalODEProblem", "kick", "drift",
+           "calculate_dt", "solve_tstops",
            "CarpenterKennedy2N54", "solve", "CK2N54_A", "CK2N54_B", "CK2N54_C"]
 
 # Carpenter & Kennedy (1994), NASA TM-109112, five-stage fourth-order 2N-storage scheme.
@@ -344,4 +345,49 @@ def solve(ode: DynamicalODEProblem, alg=None, *, dt, nsteps=None, callback=None)
                              semi.handle)
         t = t0 + (step + 1) * dt
         callback(v, u, t, step + 1)
+    return v, u
+
+
+def calculate_dt(v_ode, u_ode, cfl_number, semi: Semidiscretization):
+    """``calculate_dt(v_ode, u_ode, cfl_number, semi)`` (semidiscretization.jl:510-520) as the
+    ``StepsizeCallback(cfl)`` uses it: the minimum over the systems, a wall system giving
+    ``Inf`` (wall_boundary/system.jl:291-293).  For the fluid (fluid.jl:199-231):
+    ``min(0.125 h^2 / nu, 0.25 sqrt(h / |g|), cfl h / c)`` with the Monaghan kinematic
+    viscosity ``nu = alpha h c / (2 ndims + 4)`` (viscosity.jl:82-87).  None of the terms
+    depends on the state, so the step is constant for this system."""
+    f = semi.fluid
+    h = f.smoothing_length
+    c = f.state_equation.sound_speed
+    dt_viscosity = math.inf
+    if f.viscosity is not None:
+        nu = f.viscosity.alpha * h * c / (2 * f.ndims + 4)
+        dt_viscosity = 0.125 * h ** 2 / nu if nu > 0 else math.inf
+    g = math.sqrt(sum(a * a for a in f.acceleration))
+    dt_acceleration = 0.25 * math.sqrt(h / g) if g > 0 else math.inf
+    dt_sound_speed = cfl_number * h / c
+    return min(dt_viscosity, dt_acceleration, dt_sound_speed)
+
+
+def solve_tstops(ode: DynamicalODEProblem, *, dt, tstops, callback=None):
+    """Fixed-step CarpenterKennedy2N54 through a sorted list of stop times, the way
+    OrdinaryDiffEq steps a non-adaptive integrator with ``tstops`` (the stops the
+    reference's periodic callbacks add): every step is ``min(dt, tstop - t)``, a step that
+    reaches a stop lands on it exactly, and the next one is ``dt`` again.  The steps run on
+    the device (``tpsph_solve_ck2n54``, one step per call).  ``callback(v, u, t)`` is called
+    at every stop.  Returns the final ``(v, u)``."""
+    semi = ode.p["semi"]
+    lib = native.load()
+    v = ode.v0.clone() if hasattr(ode.v0, "clone") else ode.v0.copy()
+    u = ode.u0.clone() if hasattr(ode.u0, "clone") else ode.u0.copy()
+    t = float(ode.tspan[0])
+    for stop in sorted(float(x) for x in tstops):
+        if stop <= t:
+            continue
+        while t < stop:
+            step = min(dt, stop - t)
+            native.check(lib.tpsph_solve_ck2n54(semi.handle, _ptr(v), _ptr(u), t, float(step), 1),
+                         semi.handle)
+            t = stop if step == stop - t else t + step
+        if callback is not None:
+            callback(v, u, t)
     return v, u
