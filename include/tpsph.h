@@ -2,8 +2,9 @@
  * tpsph.h -- C ABI of the MI355X-native WCSPH right-hand side (HIP, gfx950).
  *
  * The library replaces the per-stage work behind TrixiParticles.jl's ODE surface for one
- * fluid system (WeaklyCompressibleSPHSystem, ContinuityDensity, StateEquationCole,
- * ArtificialViscosityMonaghan, DensityDiffusionMolteniColagrossi, WendlandC2Kernel) and one
+ * fluid system (WeaklyCompressibleSPHSystem, ContinuityDensity, StateEquationCole or
+ * StateEquationAdaptiveCole, ArtificialViscosityMonaghan, DensityDiffusionMolteniColagrossi,
+ * WendlandC2Kernel) and one
  * static wall system (WallBoundarySystem + BoundaryModelDummyParticles +
  * AdamiPressureExtrapolation):
  *
@@ -40,7 +41,7 @@
 extern "C" {
 #endif
 
-#define TPSPH_ABI_VERSION 1
+#define TPSPH_ABI_VERSION 2
 
 enum tpsph_status {
     TPSPH_OK = 0,
@@ -109,6 +110,17 @@ enum tpsph_timed_kernel {
  *                                      reference's one whatever the skin.
  *   pair_search                        how the fluid kernel finds candidate pairs
  *                                      (enum tpsph_pair_search); all give the same pairs
+ *   adaptive_sound_speed,              StateEquationAdaptiveCole(; mach_number_target,
+ *   mach_number_target,                min_sound_speed, max_sound_speed)
+ *   min_sound_speed, max_sound_speed   (state_equations.jl:1-76): with adaptive_sound_speed
+ *                                      = 1 every kick first sets the sound speed of the
+ *                                      state equation, the viscosity and the density
+ *                                      diffusion to clamp(max |v| / mach_number_target,
+ *                                      min_sound_speed, max_sound_speed) over the fluid's
+ *                                      current velocities (update_speed_of_sound!,
+ *                                      wcsph/system.jl:307-320); `sound_speed` is then the
+ *                                      value before the first kick (the reference starts
+ *                                      at min_sound_speed)
  */
 typedef struct tpsph_params {
     int32_t ndims;                 /* 2 or 3 */
@@ -135,6 +147,11 @@ typedef struct tpsph_params {
     double domain_min[3];
     double domain_max[3];
     double skin;                   /* < 0 = default (0.04) */
+    int32_t adaptive_sound_speed;  /* 0/1: StateEquationAdaptiveCole */
+    int32_t reserved0;
+    double mach_number_target;
+    double min_sound_speed;
+    double max_sound_speed;
 } tpsph_params;
 
 typedef struct tpsph_handle tpsph_handle;
@@ -170,6 +187,12 @@ int tpsph_drift(tpsph_handle *h, double *du_ode, const double *v_ode, const doub
  * boundary_model.cache.density (n_wall). */
 int tpsph_get_pressures(tpsph_handle *h, double *fluid_pressure, double *wall_pressure,
                         double *wall_density);
+
+/* system_sound_speed(system) (wcsph/system.jl:285) as of the last kick: the sound speed the
+ * state equation, viscosity and density diffusion used (with adaptive_sound_speed the value
+ * update_speed_of_sound! set, else params.sound_speed).  The StepsizeCallback reads it
+ * (fluid.jl:199-231).  Synchronises the handle's stream. */
+int tpsph_get_sound_speed(tpsph_handle *h, double *sound_speed);
 
 /* Neighbour sets of the current positions as CSR (PointNeighbors' foreach_point_neighbor
  * relation, neighborhood_search.jl:5-13: all b with |x_a - x_b|^2 <= (2h)^2, self

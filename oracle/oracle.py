@@ -36,6 +36,11 @@ class OracleParams(C.Structure):
         ("delta", C.c_double),
         ("gravity", C.c_double * 3),
         ("pressure_offset", C.c_double),
+        ("adaptive_sound_speed", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("mach_number_target", C.c_double),
+        ("min_sound_speed", C.c_double),
+        ("max_sound_speed", C.c_double),
     ]
 
 
@@ -79,6 +84,8 @@ def lib():
         L.oracle_interact_fluid_fluid.argtypes = [P, i64, P, P, P, P, P]
         L.oracle_interact_fluid_wall.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P]
         L.oracle_kick.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, P]
+        L.oracle_adaptive_sound_speed.argtypes = [P, i64, P]
+        L.oracle_adaptive_sound_speed.restype = d
         L.oracle_drift.argtypes = [C.c_int, i64, P, P]
         L.oracle_drift.restype = None
         L.oracle_solve_ck2n54.argtypes = [P, i64, P, P, P, i64, P, P, d, i64]
@@ -113,6 +120,11 @@ def params_from_semi(semi) -> OracleParams:
     P.sound_speed = se.sound_speed
     P.exponent = se.exponent
     P.background_pressure = se.background_pressure
+    if hasattr(se, "mach_number_target"):  # StateEquationAdaptiveCole
+        P.adaptive_sound_speed = 1
+        P.mach_number_target = se.mach_number_target
+        P.min_sound_speed = se.min_sound_speed
+        P.max_sound_speed = se.max_sound_speed
     if f.viscosity is not None:
         P.use_viscosity = 1
         P.alpha, P.beta, P.epsilon = f.viscosity.alpha, f.viscosity.beta, f.viscosity.epsilon
@@ -211,6 +223,12 @@ def kick(P, v, u, m, xw=None, mw=None):
                            _p(pf), _p(pw), _p(rw))
     assert rc == 0
     return dv, pf, pw[:nw], rw[:nw]
+
+
+def adaptive_sound_speed(P, v):
+    """The sound speed update_speed_of_sound! sets for the fluid velocities in v."""
+    v = _f64(v)
+    return lib().oracle_adaptive_sound_speed(C.byref(P), v.shape[0], _p(v))
 
 
 def drift(nd, v):

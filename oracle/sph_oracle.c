@@ -58,6 +58,11 @@ typedef struct {
     double delta;
     double gravity[3];
     double pressure_offset;       /* AdamiPressureExtrapolation(pressure_offset=...) */
+    int32_t adaptive_sound_speed; /* StateEquationAdaptiveCole: sound_speed set per kick */
+    int32_t reserved0;
+    double mach_number_target;
+    double min_sound_speed;
+    double max_sound_speed;
 } oracle_params;
 
 /* ---------------------------------------------------------------------------------------
@@ -596,10 +601,35 @@ int oracle_interact_fluid_wall(const oracle_params *P, int64_t nf, const double 
  *   set_zero!(dv); fluid EOS; Adami wall pressure; fluid-fluid; fluid-wall; gravity.
  * Optional outputs (may be NULL): fluid pressure, wall pressure, wall density.
  * ------------------------------------------------------------------------------------- */
-int oracle_kick(const oracle_params *P, int64_t nf, const double *v, const double *x,
+/* update_speed_of_sound! (wcsph/system.jl:307-320): the sound speed of an adaptive state
+ * equation for the fluid velocities v, min(c_max, max(c_min, sqrt(max_a v_a.v_a) / Ma)).
+ * v_a.v_a is summed in component order, without fused multiply-adds. */
+double oracle_adaptive_sound_speed(const oracle_params *P, int64_t nf, const double *v)
+{
+    int nd = P->ndims;
+    double vmax2 = 0.0;
+    for (int64_t a = 0; a < nf; a++) {
+        const double *va = v + a * (nd + 1);
+        double d = va[0] * va[0];
+        for (int k = 1; k < nd; k++) d += va[k] * va[k];
+        if (d > vmax2) vmax2 = d;
+    }
+    double c = sqrt(vmax2) / P->mach_number_target;
+    if (c < P->min_sound_speed) c = P->min_sound_speed;
+    if (c > P->max_sound_speed) c = P->max_sound_speed;
+    return c;
+}
+
+int oracle_kick(const oracle_params *P0, int64_t nf, const double *v, const double *x,
                 const double *m, int64_t nw, const double *xw, const double *mw, double *dv,
                 double *pf_out, double *pw_out, double *rhow_out)
 {
+    /* update_quantities! runs first in the kick (semidiscretization.jl:616-651): the
+     * adaptive sound speed of this kick, used by the state equation (fluid and walls), the
+     * viscosity and the density diffusion alike */
+    oracle_params Q = *P0;
+    if (P0->adaptive_sound_speed) Q.sound_speed = oracle_adaptive_sound_speed(P0, nf, v);
+    const oracle_params *P = &Q;
     int nd = P->ndims;
     int rc = 0;
     double *pf = (double *)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1));

@@ -21,12 +21,13 @@ def test_library_exports_every_declared_symbol(tp):
     for s in syms:
         assert hasattr(lib, s), f"{s} declared in include/tpsph.h but not exported"
     assert set(tp.native.EXPORTED_SYMBOLS) == set(syms)
-    assert lib.tpsph_abi_version() == 1
+    assert lib.tpsph_abi_version() == 2
 
 
 def test_params_struct_layout_matches_header(tp):
     # 10 int32 + 9 doubles + acceleration[3] + offset + domain_min[3] + domain_max[3] + skin
-    assert C.sizeof(tp.native.TpsphParams) == 10 * 4 + (9 + 3 + 1 + 3 + 3 + 1) * 8
+    # + adaptive flag and its padding + 3 adaptive doubles
+    assert C.sizeof(tp.native.TpsphParams) == 10 * 4 + (9 + 3 + 1 + 3 + 3 + 1) * 8 + 2 * 4 + 3 * 8
 
 
 def _create(tp, **overrides):
@@ -63,6 +64,20 @@ def _create(tp, **overrides):
 ])
 def test_create_rejects_invalid_parameters(tp, field, value, msg):
     rc, err = _create(tp, **{field: value})
+    assert rc == tp.native.TPSPH_ERR_ARGUMENT
+    assert msg in err
+
+
+@pytest.mark.parametrize("fields,msg", [
+    ({"mach_number_target": 0.0, "min_sound_speed": 10.0, "max_sound_speed": 100.0},
+     "mach_number_target"),
+    ({"mach_number_target": 0.1, "min_sound_speed": 0.0, "max_sound_speed": 100.0},
+     "min_sound_speed"),
+    ({"mach_number_target": 0.1, "min_sound_speed": 10.0, "max_sound_speed": 5.0},
+     "max_sound_speed"),
+])
+def test_create_rejects_invalid_adaptive_state_equation(tp, fields, msg):
+    rc, err = _create(tp, adaptive_sound_speed=1, **fields)
     assert rc == tp.native.TPSPH_ERR_ARGUMENT
     assert msg in err
 

@@ -91,7 +91,24 @@ struct DevParams {
     // radius (list radius + margin) and cell growth (margin) of the union
     int ur[3], maxr;
     double rl_union2, hs_union;
+    // StateEquationAdaptiveCole: {c, B, av_c, dd2} of the current kick in device memory
+    // (k_sound_speed writes them), or nullptr for the constants above
+    const double *dyn;
+    double mach, c_min, c_max;
 };
+
+// The sound-speed dependent constants of this kick: P's own, or the adaptive ones.  Kernels
+// that evaluate the state equation or the pair terms start with P = with_sound(P).
+__device__ __forceinline__ DevParams with_sound(DevParams P)
+{
+    if (P.dyn) {
+        P.c = P.dyn[0];
+        P.B = P.dyn[1];
+        P.av_c = P.dyn[2];
+        P.dd2 = P.dyn[3];
+    }
+    return P;
+}
 
 // Cell-sorted fluid data.  rec[2s], rec[2s+1] = {x, y, z, rho}, {vx, vy, vz, p} (one 64-byte
 // line per particle, read by the pair phase), vol[s] = {m/rho, m}; x/y/z SoA for the radius
@@ -418,6 +435,7 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
                          double4 *__restrict__ rec, double2 *__restrict__ vol,
                          double *__restrict__ snap, int *__restrict__ iperm)
 {
+    P = with_sound(P);
     const int f = *flag;
     // MODE 0 runs first in every kick: it resets the work counter of the brick kernel
     if (MODE == 0 && work && blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
@@ -508,6 +526,7 @@ __global__ void __launch_bounds__(kBlock)
            const int *__restrict__ acnt, int lw, const int *__restrict__ active,
            const int *__restrict__ nactive)
 {
+    P = with_sound(P);
     const int na = alist ? *nactive : P.nw;
     const int q = threadIdx.x % LANES;
     // the loop bound is uniform per LANES lanes, so the partners shuffle together
@@ -759,6 +778,7 @@ __global__ void __launch_bounds__(kBlock)
     k_fluid(int *flag, DevParams P, FluidView F, WallView Wv,
             const unsigned char *__restrict__ wall_near, double *__restrict__ dv)
 {
+    P = with_sound(P);
     if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;  // the rebuild (if any) is done
     __shared__ int lists[kCap * kBlock];
     int *lst = lists + threadIdx.x;
@@ -887,6 +907,7 @@ __global__ void __launch_bounds__(kBlock)
                  int *__restrict__ active, int *__restrict__ nactive, double *__restrict__ pw,
                  double *__restrict__ rhow, double2 *__restrict__ wpv)
 {
+    P = with_sound(P);
     if (skip(flag)) return;
     for (int w = blockIdx.x * kBlock + threadIdx.x; w < P.nw; w += gridDim.x * kBlock) {
         const double4 xw = wpos[w];
@@ -934,6 +955,7 @@ __global__ void __launch_bounds__(kBlock)
                  const int *__restrict__ wnbr, const int *__restrict__ wcnt, int ls,
                  double *__restrict__ dv)
 {
+    P = with_sound(P);
     if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;  // the rebuild (if any) is done
     // 8 KiB per wavefront: the staging area (64 records x 80 bytes) while the lists are
     // streamed, then the int lists (kCap x 64 lanes) of the cell-walk fallback
@@ -1220,6 +1242,7 @@ __global__ void __launch_bounds__(TileCfg<ND>::THREADS)
                   const int *__restrict__ tiles, const int *__restrict__ ntiles,
                   int *__restrict__ work, int tgx, int tgy, double *__restrict__ dv)
 {
+    P = with_sound(P);
     using C = TileCfg<ND>;
     using SM = TileSmem<ND>;
     __shared__ SM S;
@@ -1536,6 +1559,7 @@ __global__ void __launch_bounds__(kBrickThreads)
                   const int *__restrict__ wnbr, const int *__restrict__ wcnt, int ls,
                   double *__restrict__ dv)
 {
+    P = with_sound(P);
     // cell-walk fallback lists of a global brick: as many entries per lane as the staging
     // area holds (at most kCap)
     constexpr int kFallbackCap = std::min<int>(kCap, sizeof(BrickSmem<ND>) / (sizeof(int) * kBrickThreads));
@@ -1683,6 +1707,7 @@ __global__ void __launch_bounds__(kBlock)
                   const int *__restrict__ wnbr, const int *__restrict__ wcnt, int ls,
                   double *__restrict__ dv)
 {
+    P = with_sound(P);
     constexpr int kLanes = 4;
     __shared__ int lists[kCap * kBlock];  // only for particles whose wall list overflowed
     int *lst = lists + threadIdx.x;
@@ -1777,6 +1802,7 @@ __global__ void __launch_bounds__(kBlock)
                       double4 *__restrict__ rec, double2 *__restrict__ vol,
                       const double *__restrict__ snap)
 {
+    P = with_sound(P);
     const int i0 = blockIdx.x * kBlock;
     const int np = min(kBlock, nf - i0);
     const size_t u0 = (size_t)i0 * ND, v0 = (size_t)i0 * (ND + 1);
@@ -1822,6 +1848,55 @@ __global__ void __launch_bounds__(kBlock)
         e2 = fma(e, e, e2);
     }
     if (!(e2 <= P.half_skin2)) atomicOr(flag, 1);
+}
+
+// ------------------------------------------------------------------------------------------
+// StateEquationAdaptiveCole: update_speed_of_sound! (wcsph/system.jl:307-320) at the start
+// of every kick.  k_vmax: per-block maxima of v.v over the fluid's current velocities (the
+// caller-order v_ode); k_sound_speed: c = min(c_max, max(c_min, sqrt(max v.v) / Ma)) and the
+// constants that depend on it, computed as tpsph_create computes them from a fixed c.
+// ------------------------------------------------------------------------------------------
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_vmax(int nf, const double *__restrict__ v, double *__restrict__ part)
+{
+    __shared__ double s[kBlock];
+    double m = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < nf; i += gridDim.x * kBlock) {
+        double d = v[(size_t)i * (ND + 1)] * v[(size_t)i * (ND + 1)];
+#pragma unroll
+        for (int k = 1; k < ND; k++) d += v[(size_t)i * (ND + 1) + k] * v[(size_t)i * (ND + 1) + k];
+        m = fmax(m, d);
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void __launch_bounds__(kBlock)
+    k_sound_speed(DevParams P, int nparts, const double *__restrict__ part, double *__restrict__ dyn,
+                  double alpha, double delta)
+{
+    __shared__ double s[kBlock];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kBlock) m = fmax(m, part[i]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double c = fmin(P.c_max, fmax(P.c_min, sqrt(s[0]) / P.mach));
+        dyn[0] = c;
+        dyn[1] = P.rho0 * (c * c) / P.gamma;
+        dyn[2] = alpha * c * P.h;
+        dyn[3] = P.use_dd ? 2.0 * delta * P.h * c : 0.0;
+    }
 }
 
 __global__ void k_rk_stage(int64_t n, double *__restrict__ y, double *__restrict__ tmp,
@@ -1958,6 +2033,8 @@ struct tpsph_handle {
     bool bricks = false;
     int *brow_gb = nullptr, *brow_lds = nullptr, *bntot = nullptr, *nel = nullptr, *bwork = nullptr;
     unsigned short *nbr16 = nullptr;
+    double *dyn = nullptr;       // adaptive sound speed: {c, B, av_c, dd2}
+    double *vmax_part = nullptr; // per-block maxima of v.v (kGateBlocks)
     // Verlet lists of the wall particles over the fluid (Adami pass), active wall particles
     int *anbr = nullptr, *acnt = nullptr, *wactive = nullptr;
     int lw = 0, cap_a = 0;
@@ -2146,6 +2223,17 @@ template <int ND>
 int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, bool gathered)
 {
     const int nf = h->nf, nw = h->nw;
+    if (h->dyn && nf > 0) {
+        // the sound speed of this kick comes from its velocities, before any pressure: the
+        // records are then gathered again with it
+        TimedScope t(h, TPSPH_TIME_GATHER);
+        const int nb = gate_blocks(nf);
+        hipLaunchKernelGGL((k_vmax<ND>), dim3(nb), dim3(kBlock), 0, h->stream, nf, v, h->vmax_part);
+        hipLaunchKernelGGL((k_sound_speed), dim3(1), dim3(kBlock), 0, h->stream, h->P, nb,
+                           h->vmax_part, h->dyn, h->params.alpha, h->params.delta);
+        TRY(launch_check(h, "k_vmax/k_sound_speed"));
+        gathered = false;
+    }
     TRY(update_fluid<ND>(h, v, u, gathered));
     if (nw > 0) {
         TimedScope t(h, TPSPH_TIME_WALL);
@@ -2227,6 +2315,12 @@ int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, co
     if (!(p->reference_density > DBL_EPSILON))
         return bad("`reference_density` needs to be positive and larger than eps()");
     if (!(p->sound_speed > 0.0)) return bad("`sound_speed` must be positive");
+    if (p->adaptive_sound_speed) {
+        if (!(p->mach_number_target > 0.0)) return bad("`mach_number_target` must be positive");
+        if (!(p->min_sound_speed > 0.0)) return bad("`min_sound_speed` must be positive");
+        if (!(p->max_sound_speed >= p->min_sound_speed))
+            return bad("`max_sound_speed` must not be below `min_sound_speed`");
+    }
     if (!(p->exponent > 0.0)) return bad("`exponent` must be positive");
     if (p->cell_divisions < 0 || p->cell_divisions > 4) return bad("cell_divisions must be in 0..4");
     if (p->x_cell_divisions < 0 || p->x_cell_divisions > 8)
@@ -2316,6 +2410,10 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     P.dd2 = params->use_density_diffusion ? 2.0 * params->delta * hl * P.c : 0.0;
     for (int d = 0; d < 3; d++) P.g[d] = d < nd ? params->acceleration[d] : 0.0;
     P.p_offset = params->pressure_offset;
+    P.dyn = nullptr;  // set below for an adaptive state equation
+    P.mach = params->mach_number_target;
+    P.c_min = params->min_sound_speed;
+    P.c_max = params->max_sound_speed;
 
     // --- grid: the domain box (or the wall box when unset), one padding cell per side
     double lo[3], hi[3];
@@ -2473,6 +2571,17 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     const int nf = h->nf, nw = h->nw;
     const int tiles = (int)((ncells + 1 + kScanTile - 1) / kScanTile);
     CTRY(dalloc(h, &h->flag, 1));
+    if (params->adaptive_sound_speed) {
+        CTRY(dalloc(h, &h->dyn, 4));
+        CTRY(dalloc(h, &h->vmax_part, kGateBlocks));
+        // until the first kick: the constants of params.sound_speed
+        const double init[4] = {P.c, P.B, P.av_c, P.dd2};
+        if (hipMemcpy(h->dyn, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) {
+            h->last_error = "copy of the sound speed failed";
+            return cleanup_fail(TPSPH_ERR_HIP);
+        }
+        P.dyn = h->dyn;
+    }
     CTRY(dalloc(h, &h->scan_tiles, (size_t)tiles + 1));
     CTRY(dalloc(h, &h->f_cell, nf));
     CTRY(dalloc(h, &h->f_rank, nf));
@@ -2662,6 +2771,20 @@ int tpsph_get_pressures(tpsph_handle *h, double *fluid_pressure, double *wall_pr
     (void)hipFree(tmp);
     (void)hipFree(pf_sorted);
     return rc;
+}
+
+int tpsph_get_sound_speed(tpsph_handle *h, double *sound_speed)
+{
+    if (!h || !sound_speed) return fail(h, TPSPH_ERR_ARGUMENT, "handle or sound_speed is NULL");
+    if (!h->dyn) {
+        *sound_speed = h->P.c;
+        return TPSPH_OK;
+    }
+    if (hipMemcpyAsync(sound_speed, h->dyn, sizeof(double), hipMemcpyDeviceToHost, h->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return fail(h, TPSPH_ERR_HIP, "reading the sound speed failed");
+    return TPSPH_OK;
 }
 
 int tpsph_neighbors_csr(tpsph_handle *h, const double *u_ode, int32_t pair, int64_t *offsets,
@@ -2883,7 +3006,7 @@ void tpsph_destroy(tpsph_handle *h)
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
                     h->tiles, h->ntiles, h->work, h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,
-                    h->wtargets, h->iperm,
+                    h->wtargets, h->iperm, h->dyn, h->vmax_part,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu};
     for (void *p : ptrs)

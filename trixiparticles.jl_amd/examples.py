@@ -12,8 +12,9 @@ from dataclasses import dataclass
 
 from .schemes import (AdamiPressureExtrapolation, ArtificialViscosityMonaghan,
                       BoundaryModelDummyParticles, ContinuityDensity,
-                      DensityDiffusionMolteniColagrossi, StateEquationCole, WallBoundarySystem,
-                      WeaklyCompressibleSPHSystem, WendlandC2Kernel)
+                      DensityDiffusionMolteniColagrossi, StateEquationAdaptiveCole,
+                      StateEquationCole, WallBoundarySystem, WeaklyCompressibleSPHSystem,
+                      WendlandC2Kernel)
 from .setups import RectangularTank
 
 __all__ = ["DamBreak", "dam_break", "marrone_dam_break_2d"]
@@ -39,16 +40,21 @@ class DamBreak:
 def dam_break(ndims=2, particle_spacing=0.01, fluid_size=None, tank_size=None, *,
               alpha=0.02, delta=0.1, gravity=9.81, n_layers=3, open_top=True,
               jitter=0.01, seed=0, smoothing_factor=1.2, exponent=7.0, water_height=0.6,
-              clip_wall_pressure=True):
+              clip_wall_pressure=True, adaptive_sound_speed=False):
     """Dam break with the workload defaults: 2D 1.2 x 0.6 m water in a 3.2 x 1.0 m tank, 3D
-    1.0 x 0.5 x 0.5 m water in a 3.2 x 1.0 x 1.0 m tank."""
+    1.0 x 0.5 x 0.5 m water in a 3.2 x 1.0 x 1.0 m tank.  ``adaptive_sound_speed`` selects
+    the StateEquationAdaptiveCole of examples/fluid/dam_break_3d.jl (its defaults: Mach 0.1,
+    10 to 100 m/s) in place of StateEquationCole with c = 20 sqrt(g H)."""
     if fluid_size is None:
         fluid_size = (1.2, 0.6) if ndims == 2 else (1.0, 0.5, 0.5)
     if tank_size is None:
         tank_size = (3.2, 1.0) if ndims == 2 else (3.2, 1.0, 1.0)
     sound_speed = 20 * math.sqrt(gravity * water_height)
-    state_equation = StateEquationCole(sound_speed=sound_speed, reference_density=1000.0,
-                                       exponent=exponent, clip_negative_pressure=False)
+    if adaptive_sound_speed:
+        state_equation = StateEquationAdaptiveCole(reference_density=1000.0, exponent=exponent)
+    else:
+        state_equation = StateEquationCole(sound_speed=sound_speed, reference_density=1000.0,
+                                           exponent=exponent, clip_negative_pressure=False)
     acceleration = (0.0, -gravity) if ndims == 2 else (0.0, -gravity, 0.0)
     faces = (True, True, True, not open_top) if ndims == 2 else \
         (True, True, True, not open_top, True, True)

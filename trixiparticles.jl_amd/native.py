@@ -39,6 +39,7 @@ TIMED_KERNELS = {"cell_list": 0, "gather": 1, "wall": 2, "fluid": 3, "stage": 4}
 
 EXPORTED_SYMBOLS = (
     "tpsph_abi_version", "tpsph_create", "tpsph_kick", "tpsph_drift", "tpsph_get_pressures",
+    "tpsph_get_sound_speed",
     "tpsph_neighbors_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54", "tpsph_synchronize",
     "tpsph_enable_timing", "tpsph_kernel_time", "tpsph_last_error", "tpsph_destroy",
 )
@@ -72,6 +73,11 @@ class TpsphParams(C.Structure):
         ("domain_min", C.c_double * 3),
         ("domain_max", C.c_double * 3),
         ("skin", C.c_double),
+        ("adaptive_sound_speed", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("mach_number_target", C.c_double),
+        ("min_sound_speed", C.c_double),
+        ("max_sound_speed", C.c_double),
     ]
 
     def __init__(self, *args, **kw):
@@ -106,6 +112,7 @@ def load(path: str | None = None):
     lib.tpsph_kick.argtypes = [P, P, P, P, C.c_double]
     lib.tpsph_drift.argtypes = [P, P, P, P, C.c_double]
     lib.tpsph_get_pressures.argtypes = [P, P, P, P]
+    lib.tpsph_get_sound_speed.argtypes = [P, dp]
     lib.tpsph_neighbors_csr.argtypes = [P, P, C.c_int32, P, P, C.c_int64]
     lib.tpsph_rk_stage.argtypes = [P, P, P, P, C.c_double, C.c_double, C.c_double, C.c_int64,
                                    C.c_int32]

@@ -116,6 +116,33 @@ class StateEquationCole:
         return self.reference_density * tmp ** (1 / self.exponent)
 
 
+class StateEquationAdaptiveCole(StateEquationCole):
+    """``StateEquationAdaptiveCole(; mach_number_target=0.1, min_sound_speed=10.0,
+    reference_density, max_sound_speed=100.0, exponent, background_pressure=0.0,
+    clip_negative_pressure=false)`` (state_equations.jl:1-76).  The sound speed starts at
+    ``min_sound_speed``; every kick sets it to ``clamp(max |v| / mach_number_target,
+    min_sound_speed, max_sound_speed)`` over the fluid's current velocities
+    (``update_speed_of_sound!``, wcsph/system.jl:307-320).  That update runs on the device;
+    ``sound_speed`` here is the host's copy (the start value until
+    ``Semidiscretization.sound_speed()`` reads the current one)."""
+
+    def __init__(self, *, reference_density, exponent, mach_number_target=0.1,
+                 min_sound_speed=10.0, max_sound_speed=100.0, background_pressure=0.0,
+                 clip_negative_pressure=False):
+        super().__init__(sound_speed=min_sound_speed, reference_density=reference_density,
+                         exponent=exponent, background_pressure=background_pressure,
+                         clip_negative_pressure=clip_negative_pressure)
+        self.mach_number_target = float(mach_number_target)
+        self.min_sound_speed = float(min_sound_speed)
+        self.max_sound_speed = float(max_sound_speed)
+
+    def update_sound_speed(self, max_velocity):
+        """The sound speed ``update_speed_of_sound!`` sets for a maximum fluid speed."""
+        self.sound_speed = min(self.max_sound_speed,
+                               max(self.min_sound_speed, max_velocity / self.mach_number_target))
+        return self.sound_speed
+
+
 class ArtificialViscosityMonaghan:
     """``ArtificialViscosityMonaghan(; alpha, beta=0.0, epsilon=0.01)`` (viscosity.jl:68-83)."""
 
@@ -209,7 +236,8 @@ class WeaklyCompressibleSPHSystem:
         if not isinstance(density_calculator, ContinuityDensity):
             raise NotImplementedError("the HIP right-hand side integrates ContinuityDensity")
         if not isinstance(state_equation, StateEquationCole):
-            raise NotImplementedError("the HIP right-hand side implements StateEquationCole")
+            raise NotImplementedError("the HIP right-hand side implements StateEquationCole "
+                                      "and StateEquationAdaptiveCole")
         if viscosity is not None and not isinstance(viscosity, ArtificialViscosityMonaghan):
             raise NotImplementedError("the HIP right-hand side implements "
                                       "ArtificialViscosityMonaghan")

@@ -24,7 +24,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import native
-from .schemes import WallBoundarySystem, WeaklyCompressibleSPHSystem
+from .schemes import StateEquationAdaptiveCole, WallBoundarySystem, WeaklyCompressibleSPHSystem
 
 __all__ = ["Semidiscretization", "semidiscretize", "DynamicalODEProblem", "kick", "drift",
            "calculate_dt", "solve_tstops",
@@ -130,6 +130,11 @@ class Semidiscretization:
         p.sound_speed = se.sound_speed
         p.exponent = se.exponent
         p.background_pressure = se.background_pressure
+        if isinstance(se, StateEquationAdaptiveCole):
+            p.adaptive_sound_speed = 1
+            p.mach_number_target = se.mach_number_target
+            p.min_sound_speed = se.min_sound_speed
+            p.max_sound_speed = se.max_sound_speed
         if f.viscosity is not None:
             p.use_viscosity = 1
             p.alpha, p.beta, p.epsilon = f.viscosity.alpha, f.viscosity.beta, f.viscosity.epsilon
@@ -145,11 +150,28 @@ class Semidiscretization:
             p.clip_wall_pressure = int(bm.clip_negative_pressure)
             p.pressure_offset = bm.density_calculator.pressure_offset
             wse = bm.state_equation
-            if (wse.sound_speed, wse.reference_density, wse.exponent,
+            if isinstance(se, StateEquationAdaptiveCole):
+                # the reference shares one adaptive object between the two systems
+                if wse is not se:
+                    raise NotImplementedError("the wall model must use the fluid's adaptive "
+                                              "state equation object")
+            elif (wse.sound_speed, wse.reference_density, wse.exponent,
                     wse.background_pressure) != (se.sound_speed, se.reference_density,
                                                  se.exponent, se.background_pressure):
                 raise NotImplementedError("the wall model must use the fluid's state equation")
         return p
+
+    def sound_speed(self):
+        """``system_sound_speed`` of the fluid (wcsph/system.jl:285): for an adaptive state
+        equation the value the last kick set (read from the library; the start value
+        before the first kick), copied into the host's state equation object."""
+        se = self.fluid.state_equation
+        if isinstance(se, StateEquationAdaptiveCole) and self._handle is not None:
+            c = C.c_double()
+            native.check(native.load().tpsph_get_sound_speed(self._handle, C.byref(c)),
+                         self._handle)
+            se.sound_speed = c.value
+        return se.sound_speed
 
     # -- array helpers for the configured backend
     def _zeros(self, *shape):
@@ -353,11 +375,12 @@ def calculate_dt(v_ode, u_ode, cfl_number, semi: Semidiscretization):
     ``StepsizeCallback(cfl)`` uses it: the minimum over the systems, a wall system giving
     ``Inf`` (wall_boundary/system.jl:291-293).  For the fluid (fluid.jl:199-231):
     ``min(0.125 h^2 / nu, 0.25 sqrt(h / |g|), cfl h / c)`` with the Monaghan kinematic
-    viscosity ``nu = alpha h c / (2 ndims + 4)`` (viscosity.jl:82-87).  None of the terms
-    depends on the state, so the step is constant for this system."""
+    viscosity ``nu = alpha h c / (2 ndims + 4)`` (viscosity.jl:82-87).  ``c`` is
+    ``system_sound_speed``: constant for StateEquationCole, the value of the last kick for
+    StateEquationAdaptiveCole."""
     f = semi.fluid
     h = f.smoothing_length
-    c = f.state_equation.sound_speed
+    c = semi.sound_speed()
     dt_viscosity = math.inf
     if f.viscosity is not None:
         nu = f.viscosity.alpha * h * c / (2 * f.ndims + 4)
