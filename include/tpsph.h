@@ -121,6 +121,12 @@ enum tpsph_timed_kernel {
  *                                      wcsph/system.jl:307-320); `sound_speed` is then the
  *                                      value before the first kick (the reference starts
  *                                      at min_sound_speed)
+ *   wall_viscosity, wall_alpha,        BoundaryModelDummyParticles(; viscosity): the wall
+ *   wall_beta, wall_epsilon, wall_nu   model's ArtificialViscosityMonaghan(alpha, beta,
+ *                                      epsilon) or ViscosityAdami(nu, epsilon) acting on the
+ *                                      fluid-wall pairs with the no-slip wall velocity the
+ *                                      Adami pass extrapolates (dummy_particles.jl:716-756);
+ *                                      needs the fluid's ArtificialViscosityMonaghan
  */
 typedef struct tpsph_params {
     int32_t ndims;                 /* 2 or 3 */
@@ -152,6 +158,12 @@ typedef struct tpsph_params {
     double mach_number_target;
     double min_sound_speed;
     double max_sound_speed;
+    int32_t wall_viscosity;        /* 0 nothing, 1 ArtificialViscosityMonaghan, 2 ViscosityAdami */
+    int32_t reserved1;
+    double wall_alpha;             /* ArtificialViscosityMonaghan(; alpha, beta) of the walls */
+    double wall_beta;
+    double wall_epsilon;           /* epsilon of the wall model */
+    double wall_nu;                /* ViscosityAdami(; nu) of the walls */
 } tpsph_params;
 
 typedef struct tpsph_handle tpsph_handle;

@@ -41,6 +41,12 @@ class OracleParams(C.Structure):
         ("mach_number_target", C.c_double),
         ("min_sound_speed", C.c_double),
         ("max_sound_speed", C.c_double),
+        ("wall_viscosity", C.c_int32),
+        ("reserved1", C.c_int32),
+        ("wall_alpha", C.c_double),
+        ("wall_beta", C.c_double),
+        ("wall_epsilon", C.c_double),
+        ("wall_nu", C.c_double),
     ]
 
 
@@ -82,7 +88,10 @@ def lib():
         L.oracle_compute_pressure.restype = None
         L.oracle_adami.argtypes = [P, i64, P, i64, P, P, P, P, P, P]
         L.oracle_interact_fluid_fluid.argtypes = [P, i64, P, P, P, P, P]
-        L.oracle_interact_fluid_wall.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P]
+        L.oracle_interact_fluid_wall.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, P, P]
+        L.oracle_wall_velocity.argtypes = [P, i64, P, i64, P, P, P]
+        L.oracle_av_adami.argtypes = [C.c_int, d, d, d, d, P, d, P, d, d, d, d, P, P, P]
+        L.oracle_av_adami.restype = None
         L.oracle_kick.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, P]
         L.oracle_adaptive_sound_speed.argtypes = [P, i64, P]
         L.oracle_adaptive_sound_speed.restype = d
@@ -120,6 +129,13 @@ def params_from_semi(semi) -> OracleParams:
     P.sound_speed = se.sound_speed
     P.exponent = se.exponent
     P.background_pressure = se.background_pressure
+    if semi.wall is not None and semi.wall.boundary_model.viscosity is not None:
+        wv = semi.wall.boundary_model.viscosity
+        P.wall_epsilon = wv.epsilon
+        if hasattr(wv, "nu"):  # ViscosityAdami
+            P.wall_viscosity, P.wall_nu = 2, wv.nu
+        else:
+            P.wall_viscosity, P.wall_alpha, P.wall_beta = 1, wv.alpha, wv.beta
     if hasattr(se, "mach_number_target"):  # StateEquationAdaptiveCole
         P.adaptive_sound_speed = 1
         P.mach_number_target = se.mach_number_target
@@ -199,13 +215,38 @@ def interact_fluid_fluid(P, x, v, m, pf, dv=None):
     return dv
 
 
-def interact_fluid_wall(P, x, v, pf, xw, mw, pw, rhow, dv=None):
+def interact_fluid_wall(P, x, v, pf, xw, mw, pw, rhow, dv=None, m=None, wvel=None):
+    """Fluid-wall interact!; ``m`` (fluid masses) and ``wvel`` (no-slip wall velocities,
+    n_wall x ndims) only with a wall viscosity."""
     x, v, pf, xw, mw, pw, rhow = map(_f64, (x, v, pf, xw, mw, pw, rhow))
+    m = _f64(m) if m is not None else None
+    wvel = _f64(wvel) if wvel is not None else None
     dv = np.zeros_like(v) if dv is None else dv
     rc = lib().oracle_interact_fluid_wall(C.byref(P), x.shape[0], _p(x), _p(v), _p(pf),
                                           xw.shape[0], _p(xw), _p(mw), _p(pw), _p(rhow),
-                                          _p(dv))
+                                          _p(dv), _p(m), _p(wvel))
     assert rc == 0
+    return dv
+
+
+def wall_velocity(P, xw, x, v):
+    """No-slip wall velocities (n_wall x ndims) for the fluid state (x, v)."""
+    xw, x, v = map(_f64, (xw, x, v))
+    out = np.zeros((xw.shape[0], P.ndims))
+    rc = lib().oracle_wall_velocity(C.byref(P), xw.shape[0], _p(xw), x.shape[0], _p(x), _p(v),
+                                    _p(out))
+    assert rc == 0
+    return out
+
+
+def av_adami(nd, nu_a, nu_b, epsilon, h, pos_diff, grad_kernel, m_a, m_b, rho_a, rho_b, v_a,
+             v_b):
+    """ViscosityAdami's pair acceleration (adami_viscosity_force!)."""
+    pos_diff, grad_kernel, v_a, v_b = map(_f64, (pos_diff, grad_kernel, v_a, v_b))
+    dv = np.zeros(nd)
+    lib().oracle_av_adami(nd, nu_a, nu_b, epsilon, h, _p(pos_diff),
+                          float(np.sqrt(pos_diff @ pos_diff)), _p(grad_kernel), m_a, m_b, rho_a,
+                          rho_b, _p(v_a), _p(v_b), _p(dv))
     return dv
 
 

@@ -63,6 +63,12 @@ typedef struct {
     double mach_number_target;
     double min_sound_speed;
     double max_sound_speed;
+    int32_t wall_viscosity;       /* wall model viscosity: 0 nothing, 1 Monaghan, 2 Adami */
+    int32_t reserved1;
+    double wall_alpha;
+    double wall_beta;
+    double wall_epsilon;
+    double wall_nu;
 } oracle_params;
 
 /* ---------------------------------------------------------------------------------------
@@ -199,22 +205,55 @@ static double dot3(int nd, const double *a, const double *b)
 }
 
 /* ArtificialViscosityMonaghan functor, viscosity.jl:96-132.  Adds into dv[0..nd). */
-void oracle_av_monaghan(const oracle_params *P, const double *pos_diff, double distance,
-                        double m_b, double rho_a, double rho_b, const double *v_a,
-                        const double *v_b, const double *grad_kernel, double *dv)
+static void av_monaghan(int nd, double alpha, double beta, double epsilon, double c,
+                        double hl, const double *pos_diff, double distance, double m_b,
+                        double rho_a, double rho_b, const double *v_a, const double *v_b,
+                        const double *grad_kernel, double *dv)
 {
-    int nd = P->ndims;
     double v_diff[3] = {0, 0, 0};
     for (int i = 0; i < nd; i++) v_diff[i] = v_a[i] - v_b[i];
     double vr = dot3(nd, v_diff, pos_diff);
     if (vr < 0.0) {
-        double h = (P->smoothing_length + P->smoothing_length) / 2.0;
+        double h = (hl + hl) / 2.0;
         double rho_mean = (rho_a + rho_b) / 2.0;
-        double mu = (h * vr) / (distance * distance + P->epsilon * (h * h));
-        double c = P->sound_speed;
-        double s = (m_b * P->alpha * c * mu + m_b * P->beta * (mu * mu)) / rho_mean;
+        double mu = (h * vr) / (distance * distance + epsilon * (h * h));
+        double s = (m_b * alpha * c * mu + m_b * beta * (mu * mu)) / rho_mean;
         for (int i = 0; i < nd; i++) dv[i] += 1.0 * (s * grad_kernel[i]);
     }
+}
+
+void oracle_av_monaghan(const oracle_params *P, const double *pos_diff, double distance,
+                        double m_b, double rho_a, double rho_b, const double *v_a,
+                        const double *v_b, const double *grad_kernel, double *dv)
+{
+    av_monaghan(P->ndims, P->alpha, P->beta, P->epsilon, P->sound_speed, P->smoothing_length,
+                pos_diff, distance, m_b, rho_a, rho_b, v_a, v_b, grad_kernel, dv);
+}
+
+/* ViscosityAdami (viscosity.jl:219-280): adami_viscosity_force! with the kinematic
+ * viscosities nu_a, nu_b of the two sides, h the mean smoothing length. */
+void oracle_av_adami(int nd, double nu_a, double nu_b, double epsilon, double h,
+                     const double *pos_diff, double distance, const double *grad_kernel,
+                     double m_a, double m_b, double rho_a, double rho_b, const double *v_a,
+                     const double *v_b, double *dv)
+{
+    double v_diff[3] = {0, 0, 0};
+    for (int i = 0; i < nd; i++) v_diff[i] = v_a[i] - v_b[i];
+    double eta_a = nu_a * rho_a;
+    double eta_b = nu_b * rho_b;
+    double volume_a = m_a / rho_a;
+    double volume_b = m_b / rho_b;
+    double tmp = (2 * eta_a * eta_b) /
+                 ((eta_a + eta_b) * (distance * distance + epsilon * (h * h)) * m_a);
+    double visc = (volume_a * volume_a + volume_b * volume_b) *
+                  dot3(nd, grad_kernel, pos_diff) * tmp;
+    for (int i = 0; i < nd; i++) dv[i] += 1.0 * visc * v_diff[i];
+}
+
+/* kinematic_viscosity(system, ::ArtificialViscosityMonaghan, h, c) (viscosity.jl:82-87) */
+static double kinematic_viscosity_monaghan(const oracle_params *P)
+{
+    return P->alpha * P->smoothing_length * P->sound_speed / (2 * P->ndims + 4);
 }
 
 /* ---------------------------------------------------------------------------------------
@@ -480,6 +519,36 @@ int oracle_adami(const oracle_params *P, int64_t nw, const double *xw, int64_t n
     return 0;
 }
 
+/* No-slip wall velocity of every wall particle (a wall model with a viscosity):
+ * interpolate_fluid_velocity! sums W v_f over the fluid neighbours
+ * (dummy_particles.jl:716-731) and compute_wall_velocity! sets 2 v_wall - sum / volume with
+ * v_wall = 0 where volume > eps (dummy_particles.jl:586-600, 738-756).  wvel is nd x nw. */
+int oracle_wall_velocity(const oracle_params *P, int64_t nw, const double *xw, int64_t nf,
+                         const double *x, const double *v, double *wvel)
+{
+    int nd = P->ndims;
+    double h = P->smoothing_length;
+    double R = 2.0 * h;
+    double lo[3], hi[3];
+    bbox(nd, nw, xw, nf, x, lo, hi);
+    grid_t g;
+    if (grid_build(&g, nd, R, lo, hi, nf, x)) return -1;
+#pragma omp parallel for schedule(static)
+    for (int64_t w = 0; w < nw; w++) {
+        double vol = 0.0, wv[3] = {0, 0, 0};
+        const double *xa = xw + w * nd;
+        FOREACH_NEIGHBOR(&g, xa, x, {
+            double kernel_weight = oracle_kernel(P->kernel, nd, distance, h);
+            vol += kernel_weight;
+            for (int d = 0; d < nd; d++) wv[d] += kernel_weight * v[b * (nd + 1) + d];
+        });
+        for (int d = 0; d < nd; d++)
+            wvel[w * nd + d] = (vol > DBL_EPSILON) ? 2 * 0.0 - wv[d] / vol : wv[d];
+    }
+    grid_free(&g);
+    return 0;
+}
+
 /* ---------------------------------------------------------------------------------------
  * WCSPH interact! (weakly_compressible_sph/rhs.jl:5-127) with ContinuityDensity, no
  * correction, pressure_acceleration_continuity_density (pressure_acceleration.jl:34-41),
@@ -551,12 +620,15 @@ int oracle_interact_fluid_fluid(const oracle_params *P, int64_t nf, const double
 
 /* Fluid-wall: neighbour = WallBoundarySystem with BoundaryModelDummyParticles.  The wall
  * velocity is zero (wall_boundary/system.jl:107-123), rho_b / p_b come from the Adami pass,
- * the viscosity model is the wall model's (`nothing`, fluid.jl:109-112) and density
- * diffusion only acts between identical systems (fluid.jl:179-183). */
+ * the viscosity model is the wall model's (fluid.jl:109-112): nothing, or Monaghan / Adami
+ * on the no-slip velocity wvel (viscous_velocity, wall_boundary/system.jl:148-163; nd x nw,
+ * with the fluid masses m for m_a), and density diffusion only acts between identical
+ * systems (fluid.jl:179-183). */
 int oracle_interact_fluid_wall(const oracle_params *P, int64_t nf, const double *x,
                                const double *v, const double *pf, int64_t nw,
                                const double *xw, const double *mw, const double *pw,
-                               const double *rhow, double *dv)
+                               const double *rhow, double *dv, const double *m,
+                               const double *wvel)
 {
     int nd = P->ndims;
     double h = P->smoothing_length;
@@ -585,6 +657,18 @@ int oracle_interact_fluid_wall(const oracle_params *P, int64_t nf, const double 
             double p_b = pw[b];
             double pfac = -m_b * ((p_a + p_b) / (rho_a * rho_b));
             for (int d = 0; d < nd; d++) dv_particle[d] += (pfac * grad_kernel[d]) * 1.0;
+            if (P->wall_viscosity && wvel) {
+                const double *v_w = wvel + b * nd;
+                if (P->wall_viscosity == 1)
+                    av_monaghan(nd, P->wall_alpha, P->wall_beta, P->wall_epsilon,
+                                P->sound_speed, h, pos_diff, distance, m_b, rho_a, rho_b, v_a,
+                                v_w, grad_kernel, dv_particle);
+                else
+                    oracle_av_adami(nd, kinematic_viscosity_monaghan(P), P->wall_nu,
+                                    P->wall_epsilon, (h + h) / 2, pos_diff, distance,
+                                    grad_kernel, m[a], m_b, rho_a, rho_b, v_a, v_w,
+                                    dv_particle);
+            }
             double v_diff[3] = {0, 0, 0};
             for (int d = 0; d < nd; d++) v_diff[d] = v_a[d] - 0.0;
             drho_particle += (rho_a / rho_b) * m_b * dot3(nd, v_diff, grad_kernel);
@@ -635,13 +719,19 @@ int oracle_kick(const oracle_params *P0, int64_t nf, const double *v, const doub
     double *pf = (double *)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1));
     double *pw = (double *)malloc(sizeof(double) * (size_t)(nw > 0 ? nw : 1));
     double *rhow = (double *)malloc(sizeof(double) * (size_t)(nw > 0 ? nw : 1));
+    double *wvel = NULL;
     if (!pf || !pw || !rhow) { rc = -1; goto out; }
     memset(dv, 0, sizeof(double) * (size_t)(nf * (nd + 1)));
     oracle_compute_pressure(P, nf, v, pf);
     if (nw > 0 && (rc = oracle_adami(P, nw, xw, nf, x, v, pf, pw, rhow, NULL))) goto out;
+    if (nw > 0 && P->wall_viscosity) {
+        wvel = (double *)malloc(sizeof(double) * (size_t)(nw * nd));
+        if (!wvel) { rc = -1; goto out; }
+        if ((rc = oracle_wall_velocity(P, nw, xw, nf, x, v, wvel))) goto out;
+    }
     if ((rc = oracle_interact_fluid_fluid(P, nf, x, v, m, pf, dv))) goto out;
-    if (nw > 0 &&
-        (rc = oracle_interact_fluid_wall(P, nf, x, v, pf, nw, xw, mw, pw, rhow, dv)))
+    if (nw > 0 && (rc = oracle_interact_fluid_wall(P, nf, x, v, pf, nw, xw, mw, pw, rhow, dv,
+                                                   m, wvel)))
         goto out;
     /* add_acceleration!, semidiscretization.jl:720-728 */
     for (int64_t a = 0; a < nf; a++)
@@ -653,6 +743,7 @@ out:
     free(pf);
     free(pw);
     free(rhow);
+    free(wvel);
     return rc;
 }
 

@@ -27,7 +27,9 @@ def test_library_exports_every_declared_symbol(tp):
 def test_params_struct_layout_matches_header(tp):
     # 10 int32 + 9 doubles + acceleration[3] + offset + domain_min[3] + domain_max[3] + skin
     # + adaptive flag and its padding + 3 adaptive doubles
-    assert C.sizeof(tp.native.TpsphParams) == 10 * 4 + (9 + 3 + 1 + 3 + 3 + 1) * 8 + 2 * 4 + 3 * 8
+    # + wall viscosity kind and its padding + 4 wall viscosity doubles
+    assert C.sizeof(tp.native.TpsphParams) == \
+        10 * 4 + (9 + 3 + 1 + 3 + 3 + 1) * 8 + 2 * 4 + 3 * 8 + 2 * 4 + 4 * 8
 
 
 def _create(tp, **overrides):

@@ -8,8 +8,8 @@ from .examples import DamBreak, dam_break, marrone_dam_break_2d
 from .schemes import (AdamiPressureExtrapolation, ArtificialViscosityMonaghan,
                       BoundaryModelDummyParticles, ContinuityDensity,
                       DensityDiffusionMolteniColagrossi, SchoenbergCubicSplineKernel,
-                      StateEquationAdaptiveCole, StateEquationCole, WallBoundarySystem,
-                      WeaklyCompressibleSPHSystem, WendlandC2Kernel)
+                      StateEquationAdaptiveCole, StateEquationCole, ViscosityAdami,
+                      WallBoundarySystem, WeaklyCompressibleSPHSystem, WendlandC2Kernel)
 from .semidiscretization import (CK2N54_A, CK2N54_B, CK2N54_C, CarpenterKennedy2N54,
                                  DynamicalODEProblem, Semidiscretization, calculate_dt, drift,
                                  kick, semidiscretize, solve, solve_tstops)
@@ -20,7 +20,7 @@ from . import native
 __all__ = [
     "AdamiPressureExtrapolation", "ArtificialViscosityMonaghan", "BoundaryModelDummyParticles",
     "ContinuityDensity", "DensityDiffusionMolteniColagrossi", "SchoenbergCubicSplineKernel",
-    "StateEquationAdaptiveCole", "StateEquationCole", "WallBoundarySystem", "WeaklyCompressibleSPHSystem", "WendlandC2Kernel",
+    "StateEquationAdaptiveCole", "StateEquationCole", "ViscosityAdami", "WallBoundarySystem", "WeaklyCompressibleSPHSystem", "WendlandC2Kernel",
     "CK2N54_A", "CK2N54_B", "CK2N54_C", "CarpenterKennedy2N54", "DynamicalODEProblem",
     "Semidiscretization", "calculate_dt", "drift", "kick", "semidiscretize", "solve",
     "solve_tstops", "InitialCondition",

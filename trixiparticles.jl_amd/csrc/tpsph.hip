@@ -95,6 +95,11 @@ struct DevParams {
     // (k_sound_speed writes them), or nullptr for the constants above
     const double *dyn;
     double mach, c_min, c_max;
+    // viscosity of the wall model (no-slip): 0 nothing, 1 ArtificialViscosityMonaghan,
+    // 2 ViscosityAdami; its alpha c h, beta, epsilon h^2 and nu, and the fluid's own
+    // kinematic viscosity alpha h c / (2 ndims + 4) (viscosity.jl:82-87)
+    int visc_w;
+    double w_alpha, w_ac, w_beta, w_eps_h2, w_nu, nu_kin;  // w_ac = w_alpha c
 };
 
 // The sound-speed dependent constants of this kick: P's own, or the adaptive ones.  Kernels
@@ -106,6 +111,8 @@ __device__ __forceinline__ DevParams with_sound(DevParams P)
         P.B = P.dyn[1];
         P.av_c = P.dyn[2];
         P.dd2 = P.dyn[3];
+        P.w_ac = P.w_alpha * P.c;
+        P.nu_kin = P.alpha * P.h * P.c / (2 * P.nd + 4);
     }
     return P;
 }
@@ -126,6 +133,7 @@ struct WallView {
     const double *x, *y, *z;
     const double4 *pos;  // {x, y, z, m}
     const double2 *pv;   // {p_w, m/rho_w} of the current kick
+    const double *vel;   // wall velocity for the viscosity (3 per particle), or nullptr
 };
 
 // ------------------------------------------------------------------------------------------
@@ -480,10 +488,12 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
 // wall-cell order over its fluid neighbours within 2h, then the inverse state equation.
 // ------------------------------------------------------------------------------------------
 // Contribution of fluid particle j (sorted index) to the sums of the wall particle at xw.
+// With a wall viscosity the kernel-weighted fluid velocity is summed into vs
+// (interpolate_fluid_velocity!, dummy_particles.jl:716-731).
 template <int ND>
 __device__ __forceinline__ void adami_term(const DevParams &P, const FluidView &F,
                                            const double4 &xw, int j, double &pressure,
-                                           double &volume)
+                                           double &volume, double *vs)
 {
     const double4 a = F.rec[2 * (size_t)j];
     const double p0 = xw.x - a.x, p1 = xw.y - a.y;
@@ -492,19 +502,33 @@ __device__ __forceinline__ void adami_term(const DevParams &P, const FluidView &
     if (d2 > P.R2) return;
     const double W = wendland_c2(P, sqrt(d2));
     const double rho_f = a.w;
-    const double p_f = F.rec[2 * (size_t)j + 1].w;
+    const double4 b = F.rec[2 * (size_t)j + 1];
+    const double p_f = b.w;
     double hydro = P.g[0] * (rho_f * p0) + P.g[1] * (rho_f * p1);
     if constexpr (ND == 3) hydro += P.g[2] * (rho_f * p2);
     pressure += ((P.p_offset + p_f) + hydro) * W;
     volume += W;
+    if (P.visc_w) {
+        vs[0] += W * b.x;
+        vs[1] += W * b.y;
+        if constexpr (ND == 3) vs[2] += W * b.z;
+    }
 }
 
-// p_w = P_w / V_w (if V_w > eps), optional clipping, the inverse state equation.
+// p_w = P_w / V_w (if V_w > eps), optional clipping, the inverse state equation; with a wall
+// viscosity the no-slip wall velocity 2 v_wall - vs / V_w with v_wall = 0 (if V_w > eps,
+// else vs as summed; compute_wall_velocity!, dummy_particles.jl:738-756).
 __device__ __forceinline__ void adami_finish(const DevParams &P, int w, double mass,
                                              double pressure, double volume,
                                              double *__restrict__ pw, double *__restrict__ rhow,
-                                             double2 *__restrict__ wpv)
+                                             double2 *__restrict__ wpv, const double *vs,
+                                             double *__restrict__ vel)
 {
+    if (P.visc_w) {
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            vel[3 * (size_t)w + d] = (volume > DBL_EPSILON) ? 0.0 - vs[d] / volume : vs[d];
+    }
     if (volume > DBL_EPSILON) pressure /= volume;
     if (P.clip_wall) pressure = julia_max0(pressure);
     const double rho = P.rho0 * pow((pressure - P.p_bg) / P.B + 1.0, P.inv_gamma);
@@ -524,7 +548,7 @@ __global__ void __launch_bounds__(kBlock)
     k_wall(DevParams P, FluidView F, const double4 *__restrict__ wpos, double *__restrict__ pw,
            double *__restrict__ rhow, double2 *__restrict__ wpv, const int *__restrict__ alist,
            const int *__restrict__ acnt, int lw, const int *__restrict__ active,
-           const int *__restrict__ nactive)
+           const int *__restrict__ nactive, double *__restrict__ wvel)
 {
     P = with_sound(P);
     const int na = alist ? *nactive : P.nw;
@@ -534,24 +558,28 @@ __global__ void __launch_bounds__(kBlock)
          g += gridDim.x * blockDim.x / LANES) {
         const int w = alist ? active[g] : g;
         const double4 xw = wpos[w];
-        double pressure = 0.0, volume = 0.0;
+        double pressure = 0.0, volume = 0.0, vs[3] = {0.0, 0.0, 0.0};
         const int n = alist ? acnt[w] : -1;
         if (n >= 0) {
             for (int k = q; k < n; k += LANES)
-                adami_term<ND>(P, F, xw, alist[(size_t)k * lw + w], pressure, volume);
+                adami_term<ND>(P, F, xw, alist[(size_t)k * lw + w], pressure, volume, vs);
         } else if (q == 0) {
             Pencils<ND> it;
             it.init(P, xw.x, xw.y, xw.z, P.hs_fluid);
             int jb, je;
             while (it.next(P, F.start, jb, je))
-                for (int j = jb; j < je; j++) adami_term<ND>(P, F, xw, j, pressure, volume);
+                for (int j = jb; j < je; j++) adami_term<ND>(P, F, xw, j, pressure, volume, vs);
         }
 #pragma unroll
         for (int o = 1; o < LANES; o <<= 1) {
             pressure += __shfl_xor(pressure, o);
             volume += __shfl_xor(volume, o);
+            if (P.visc_w) {
+#pragma unroll
+                for (int d = 0; d < ND; d++) vs[d] += __shfl_xor(vs[d], o);
+            }
         }
-        if (q == 0) adami_finish(P, w, xw.w, pressure, volume, pw, rhow, wpv);
+        if (q == 0) adami_finish(P, w, xw.w, pressure, volume, pw, rhow, wpv, vs, wvel);
     }
 }
 
@@ -601,6 +629,7 @@ struct FluidNbr {
 struct WallNbr {
     double4 a;  // {x, y, z, m}
     double2 pv; // {p, V}
+    double v[3];  // velocity for the viscosity (only with a wall viscosity)
 };
 
 __device__ __forceinline__ FluidNbr load_fluid(const FluidView &F, int j)
@@ -612,7 +641,12 @@ __device__ __forceinline__ FluidNbr load_fluid(const FluidView &F, int j)
 __device__ __forceinline__ WallNbr load_wall(const WallView &Wv, int j)
 {
     const uint32_t o = (uint32_t)j;
-    return {ldo(Wv.pos, o * 32u), ldo(Wv.pv, o * 16u)};
+    WallNbr n{ldo(Wv.pos, o * 32u), ldo(Wv.pv, o * 16u), {0.0, 0.0, 0.0}};
+    if (Wv.vel) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) n.v[d] = Wv.vel[3 * (size_t)j + d];
+    }
+    return n;
 }
 
 template <int ND>
@@ -648,11 +682,16 @@ __device__ __forceinline__ void pair_fluid(const DevParams &P, const FluidNbr &n
     drho = fma(n.v.x * kdr, fma(P.dd2, rho_a - rho_b, rho_a * vr), drho);
 }
 
+// The wall viscosity (none without one) acts along r_ab for ArtificialViscosityMonaghan
+// (viscosity.jl:89-132, approaching pairs only) and along v_ab for ViscosityAdami
+// (viscosity.jl:219-280, with eta_a = nu_a rho_a from the fluid's kinematic viscosity and
+// eta_b = nu rho_b, viscosity_model dispatch of wall_boundary/system.jl:286-289 and
+// fluid.jl:109-112); v_b is the wall's no-slip velocity and rho_b = m_b / V_b.
 template <int ND>
 __device__ __forceinline__ void pair_wall(const DevParams &P, const WallNbr &n,
                                           const double *xa, const double *va, double rho_a,
                                           double p_a, double inv_rho_a, double *acc,
-                                          double &drho)
+                                          double &drho, double m_a)
 {
     const double p0 = xa[0] - n.a.x, p1 = xa[1] - n.a.y, p2 = (ND == 3) ? xa[2] - n.a.z : 0.0;
     const double d2 = dist2<ND>(p0, p1, p2);
@@ -661,7 +700,29 @@ __device__ __forceinline__ void pair_wall(const DevParams &P, const WallNbr &n,
     const double kdr = P.kgrad_c * (w * w * w);
     double vr = va[0] * p0 + va[1] * p1;
     if constexpr (ND == 3) vr += va[2] * p2;
-    const double fac = -n.pv.y * inv_rho_a * (p_a + n.pv.x) * kdr;
+    double fac = -n.pv.y * inv_rho_a * (p_a + n.pv.x) * kdr;
+    if (P.visc_w) {
+        const double v0 = va[0] - n.v[0], v1 = va[1] - n.v[1];
+        const double v2 = (ND == 3) ? va[2] - n.v[2] : 0.0;
+        const double m_b = n.a.w, V_b = n.pv.y, rho_b = m_b / V_b;
+        if (P.visc_w == 1) {
+            double vrw = v0 * p0 + v1 * p1;
+            if constexpr (ND == 3) vrw += v2 * p2;
+            if (vrw < 0.0) {
+                const double mu = P.h * vrw / (d2 + P.w_eps_h2);
+                fac += (m_b * P.w_ac * mu + m_b * P.w_beta * (mu * mu)) /
+                       (0.5 * (rho_a + rho_b)) * kdr;
+            }
+        } else {
+            const double eta_a = P.nu_kin * rho_a, eta_b = P.w_nu * rho_b;
+            const double V_a = m_a / rho_a;
+            const double tmp = 2.0 * eta_a * eta_b / ((eta_a + eta_b) * (d2 + P.w_eps_h2) * m_a);
+            const double visc = (V_a * V_a + V_b * V_b) * (kdr * d2) * tmp;
+            acc[0] = fma(visc, v0, acc[0]);
+            acc[1] = fma(visc, v1, acc[1]);
+            if constexpr (ND == 3) acc[2] = fma(visc, v2, acc[2]);
+        }
+    }
     acc[0] = fma(fac, p0, acc[0]);
     acc[1] = fma(fac, p1, acc[1]);
     if constexpr (ND == 3) acc[2] = fma(fac, p2, acc[2]);
@@ -720,7 +781,8 @@ __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *_
                                                 bool active, const FluidView &F,
                                                 const WallView &Wv, int *lst, const double *xa,
                                                 const double *va, double rho_a, double p_a,
-                                                double inv_rho_a, double *acc, double &drho)
+                                                double inv_rho_a, double *acc, double &drho,
+                                                double m_a = 0.0)
 {
     Pencils<ND> it;
     it.init(P, xa[0], xa[1], (ND == 3) ? xa[2] : 0.0, hs);
@@ -757,7 +819,7 @@ __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *_
                 WallNbr cur = load_wall(Wv, lst[0]);
                 for (int k = 0; k < cnt; k++) {
                     const WallNbr nxt = load_wall(Wv, lst[min(k + 1, cnt - 1) * STRIDE]);
-                    pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                    pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, m_a);
                     cur = nxt;
                 }
             } else {
@@ -812,9 +874,11 @@ __global__ void __launch_bounds__(kBlock)
             const int c2 = (ND == 3) ? cell_1d(xa[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
             near = wall_near[c0 + P.n[0] * (c1 + P.n[1] * c2)] != 0;
         }
-        if (__any(near))
+        if (__any(near)) {
+            const double m_a = (P.visc_w == 2 && active) ? F.vol[s].y : 1.0;
             neighbour_sweep<ND, true>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, near, F, Wv,
-                                      lst, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+                                      lst, xa, va, rho_a, p_a, inv_rho_a, accw, drhow, m_a);
+        }
     }
     if (!active) return;
     const int i = F.perm[s];
@@ -905,7 +969,7 @@ __global__ void __launch_bounds__(kBlock)
     k_wall_lists(const int *flag, DevParams P, FluidView F, const double4 *__restrict__ wpos,
                  int lw, int cap, int *__restrict__ nbr, int *__restrict__ cnt,
                  int *__restrict__ active, int *__restrict__ nactive, double *__restrict__ pw,
-                 double *__restrict__ rhow, double2 *__restrict__ wpv)
+                 double *__restrict__ rhow, double2 *__restrict__ wpv, double *__restrict__ wvel)
 {
     P = with_sound(P);
     if (skip(flag)) return;
@@ -924,8 +988,10 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
         cnt[w] = n <= cap ? n : -1;
-        if (n == 0)  // no fluid within reach until the next rebuild: the values of empty sums
-            adami_finish(P, w, xw.w, 0.0, 0.0, pw, rhow, wpv);
+        if (n == 0) {  // no fluid within reach until the next rebuild: the values of empty sums
+            const double vs0[3] = {0.0, 0.0, 0.0};
+            adami_finish(P, w, xw.w, 0.0, 0.0, pw, rhow, wpv, vs0, wvel);
+        }
         else
             active[atomicAdd(nactive, 1)] = w;
     }
@@ -1034,6 +1100,7 @@ __global__ void __launch_bounds__(kBlock)
 
     double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
     if (__any(nw != 0)) {
+        const double m_a = P.visc_w == 2 ? F.vol[s].y : 1.0;
         const int m = max(nw, 0);
         int j = (m > 0) ? wnbr[col] : 0;
         WallNbr cur = load_wall(Wv, j);
@@ -1044,14 +1111,14 @@ __global__ void __launch_bounds__(kBlock)
             const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
             const double d2 = dist2<ND>(p0, p1, p2);
             if (d2 <= P.R2 && d2 >= P.az2)
-                pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+                pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, accw, drhow, m_a);
             cur = nxt;
             j = jn;
         }
         if (__any(nw < 0))
             neighbour_sweep<ND, true, kCap, 64>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0,
                                                 F, Wv, lst, xa, va, rho_a, p_a, inv_rho_a, accw,
-                                                drhow);
+                                                drhow, m_a);
     }
     if (!active) return;
     const int i = F.perm[s];
@@ -1378,10 +1445,12 @@ __global__ void __launch_bounds__(TileCfg<ND>::THREADS)
                     const int c2 = (ND == 3) ? cell_1d(xa[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
                     near = wall_near[c0 + P.n[0] * (c1 + P.n[1] * c2)] != 0;
                 }
-                if (__any(near))
+                if (__any(near)) {
+                    const double m_a = (P.visc_w == 2 && active) ? F.vol[s].y : 1.0;
                     neighbour_sweep<ND, true, C::CAPL / 2, C::THREADS>(
                         P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, near, F, Wv, (int *)S.lst + tid,
-                        xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+                        xa, va, rho_a, p_a, inv_rho_a, accw, drhow, m_a);
+                }
             }
             if (active) {
                 const int i = F.perm[s];
@@ -1722,6 +1791,7 @@ __global__ void __launch_bounds__(kBlock)
         const double4 c = F.rec[2 * (size_t)s + 1];
         const double xa[3] = {a.x, a.y, a.z}, va[3] = {c.x, c.y, c.z};
         const double rho_a = a.w, p_a = c.w, inv_rho_a = 1.0 / rho_a;
+        const double m_a = P.visc_w == 2 ? F.vol[s].y : 1.0;
         const int nw = active ? wcnt[t] : 0;
         double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
         const int m = max(nw, 0);
@@ -1731,11 +1801,11 @@ __global__ void __launch_bounds__(kBlock)
             const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
             const double d2 = dist2<ND>(p0, p1, p2);
             if (d2 <= P.R2 && d2 >= P.az2)
-                pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+                pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, accw, drhow, m_a);
         }
         if (__any(nw < 0))
             neighbour_sweep<ND, true>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0 && q == 0, F,
-                                      Wv, lst, xa, va, rho_a, p_a, inv_rho_a, accw, drhow);
+                                      Wv, lst, xa, va, rho_a, p_a, inv_rho_a, accw, drhow, m_a);
 #pragma unroll
         for (int o = 1; o < kLanes; o <<= 1) {
 #pragma unroll
@@ -2034,6 +2104,7 @@ struct tpsph_handle {
     int *brow_gb = nullptr, *brow_lds = nullptr, *bntot = nullptr, *nel = nullptr, *bwork = nullptr;
     unsigned short *nbr16 = nullptr;
     double *dyn = nullptr;       // adaptive sound speed: {c, B, av_c, dd2}
+    double *wvel = nullptr;      // no-slip wall velocities (3 per wall particle)
     double *vmax_part = nullptr; // per-block maxima of v.v (kGateBlocks)
     // Verlet lists of the wall particles over the fluid (Adami pass), active wall particles
     int *anbr = nullptr, *acnt = nullptr, *wactive = nullptr;
@@ -2060,7 +2131,7 @@ struct tpsph_handle {
     std::vector<Rec> records;
 
     FluidView fluid_view() const { return {f_start, f_perm, xs, ys, zs, rec, vol}; }
-    WallView wall_view() const { return {w_start, wx, wy, wz, wpos, wpv}; }
+    WallView wall_view() const { return {w_start, wx, wy, wz, wpos, wpv, wvel}; }
 };
 
 namespace {
@@ -2210,7 +2281,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
             if (h->nw > 0)
                 hipLaunchKernelGGL((k_wall_lists<ND>), dim3(gate_blocks(h->nw)), dim3(kBlock), 0, h->stream,
                     h->flag, h->P, h->fluid_view(), h->wpos, h->lw, h->cap_a, h->anbr, h->acnt,
-                    h->wactive, h->counters + 1, h->pw, h->rhow, h->wpv);
+                    h->wactive, h->counters + 1, h->pw, h->rhow, h->wpv, h->wvel);
             TRY(launch_check(h, "k_lists"));
         }
         // the fluid kernel of this kick lowers the flag
@@ -2240,11 +2311,11 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
         if (h->anbr)
             hipLaunchKernelGGL((k_wall<ND, 4>), dim3(gate_blocks(4 * (int64_t)nw)), dim3(kBlock), 0,
                 h->stream, h->P, h->fluid_view(), h->wpos, h->pw, h->rhow, h->wpv, h->anbr, h->acnt,
-                h->lw, h->wactive, h->counters + 1);
+                h->lw, h->wactive, h->counters + 1, h->wvel);
         else
             hipLaunchKernelGGL((k_wall<ND, 1>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P,
                 h->fluid_view(), h->wpos, h->pw, h->rhow, h->wpv, nullptr, nullptr, h->lw, nullptr,
-                nullptr);
+                nullptr, h->wvel);
         TRY(launch_check(h, "k_wall"));
     }
     if (nf > 0) {
@@ -2315,6 +2386,13 @@ int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, co
     if (!(p->reference_density > DBL_EPSILON))
         return bad("`reference_density` needs to be positive and larger than eps()");
     if (!(p->sound_speed > 0.0)) return bad("`sound_speed` must be positive");
+    if (p->wall_viscosity < 0 || p->wall_viscosity > 2)
+        return bad("wall_viscosity must be 0 (nothing), 1 (ArtificialViscosityMonaghan) or 2 "
+                   "(ViscosityAdami)");
+    if (p->wall_viscosity && p->use_viscosity != 1)
+        return bad("a wall viscosity needs the fluid's ArtificialViscosityMonaghan (its "
+                   "kinematic viscosity enters the wall terms)");
+    if (p->wall_viscosity && !(p->wall_epsilon > 0.0)) return bad("`wall_epsilon` must be positive");
     if (p->adaptive_sound_speed) {
         if (!(p->mach_number_target > 0.0)) return bad("`mach_number_target` must be positive");
         if (!(p->min_sound_speed > 0.0)) return bad("`min_sound_speed` must be positive");
@@ -2411,6 +2489,13 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     for (int d = 0; d < 3; d++) P.g[d] = d < nd ? params->acceleration[d] : 0.0;
     P.p_offset = params->pressure_offset;
     P.dyn = nullptr;  // set below for an adaptive state equation
+    P.visc_w = params->wall_viscosity;
+    P.w_alpha = params->wall_alpha;
+    P.w_ac = params->wall_alpha * P.c;
+    P.w_beta = params->wall_beta;
+    P.w_eps_h2 = params->wall_epsilon * hl * hl;
+    P.w_nu = params->wall_nu;
+    P.nu_kin = params->alpha * hl * P.c / (2 * nd + 4);
     P.mach = params->mach_number_target;
     P.c_min = params->min_sound_speed;
     P.c_max = params->max_sound_speed;
@@ -2571,6 +2656,7 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     const int nf = h->nf, nw = h->nw;
     const int tiles = (int)((ncells + 1 + kScanTile - 1) / kScanTile);
     CTRY(dalloc(h, &h->flag, 1));
+    if (params->wall_viscosity && nw > 0) CTRY(dalloc(h, &h->wvel, 3 * (size_t)nw));
     if (params->adaptive_sound_speed) {
         CTRY(dalloc(h, &h->dyn, 4));
         CTRY(dalloc(h, &h->vmax_part, kGateBlocks));
@@ -3006,7 +3092,7 @@ void tpsph_destroy(tpsph_handle *h)
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
                     h->tiles, h->ntiles, h->work, h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,
-                    h->wtargets, h->iperm, h->dyn, h->vmax_part,
+                    h->wtargets, h->iperm, h->dyn, h->vmax_part, h->wvel,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu};
     for (void *p : ptrs)

@@ -40,11 +40,12 @@ class DamBreak:
 def dam_break(ndims=2, particle_spacing=0.01, fluid_size=None, tank_size=None, *,
               alpha=0.02, delta=0.1, gravity=9.81, n_layers=3, open_top=True,
               jitter=0.01, seed=0, smoothing_factor=1.2, exponent=7.0, water_height=0.6,
-              clip_wall_pressure=True, adaptive_sound_speed=False):
+              clip_wall_pressure=True, adaptive_sound_speed=False, wall_viscosity=None):
     """Dam break with the workload defaults: 2D 1.2 x 0.6 m water in a 3.2 x 1.0 m tank, 3D
     1.0 x 0.5 x 0.5 m water in a 3.2 x 1.0 x 1.0 m tank.  ``adaptive_sound_speed`` selects
     the StateEquationAdaptiveCole of examples/fluid/dam_break_3d.jl (its defaults: Mach 0.1,
-    10 to 100 m/s) in place of StateEquationCole with c = 20 sqrt(g H)."""
+    10 to 100 m/s) in place of StateEquationCole with c = 20 sqrt(g H).  ``wall_viscosity``
+    is the wall model's viscosity (no-slip walls; ``None`` = free slip, as in the examples)."""
     if fluid_size is None:
         fluid_size = (1.2, 0.6) if ndims == 2 else (1.0, 0.5, 0.5)
     if tank_size is None:
@@ -69,7 +70,8 @@ def dam_break(ndims=2, particle_spacing=0.01, fluid_size=None, tank_size=None, *
         density_diffusion=DensityDiffusionMolteniColagrossi(delta=delta))
     boundary_model = BoundaryModelDummyParticles(
         tank.boundary.density, tank.boundary.mass, AdamiPressureExtrapolation(), kernel, h,
-        state_equation=state_equation, clip_negative_pressure=clip_wall_pressure)
+        state_equation=state_equation, clip_negative_pressure=clip_wall_pressure,
+        viscosity=wall_viscosity)
     boundary_system = WallBoundarySystem(tank.boundary, boundary_model)
     return DamBreak(tank, fluid_system, boundary_system, state_equation, particle_spacing)
 

@@ -78,6 +78,12 @@ class TpsphParams(C.Structure):
         ("mach_number_target", C.c_double),
         ("min_sound_speed", C.c_double),
         ("max_sound_speed", C.c_double),
+        ("wall_viscosity", C.c_int32),
+        ("reserved1", C.c_int32),
+        ("wall_alpha", C.c_double),
+        ("wall_beta", C.c_double),
+        ("wall_epsilon", C.c_double),
+        ("wall_nu", C.c_double),
     ]
 
     def __init__(self, *args, **kw):

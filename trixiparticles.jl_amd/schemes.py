@@ -152,6 +152,15 @@ class ArtificialViscosityMonaghan:
         self.epsilon = float(epsilon)
 
 
+class ViscosityAdami:
+    """``ViscosityAdami(; nu, epsilon=0.01)`` (viscosity.jl:202-217).  The HIP right-hand side
+    uses it as the viscosity of a wall model (no-slip walls)."""
+
+    def __init__(self, *, nu, epsilon=0.01):
+        self.nu = float(nu)
+        self.epsilon = float(epsilon)
+
+
 class DensityDiffusionMolteniColagrossi:
     """``DensityDiffusionMolteniColagrossi(; delta)`` (density_diffusion.jl:41-47)."""
 
@@ -187,8 +196,10 @@ class BoundaryModelDummyParticles:
             raise ValueError("length(initial_density) != length(hydrodynamic_mass)")
         if not isinstance(density_calculator, AdamiPressureExtrapolation):
             raise NotImplementedError("the HIP wall pass implements AdamiPressureExtrapolation")
-        if viscosity is not None:
-            raise NotImplementedError("wall viscosity (no-slip walls) is not implemented")
+        if viscosity is not None and \
+                not isinstance(viscosity, (ArtificialViscosityMonaghan, ViscosityAdami)):
+            raise NotImplementedError("the HIP wall pass implements ArtificialViscosityMonaghan "
+                                      "and ViscosityAdami as wall viscosity")
         if correction is not None:
             raise NotImplementedError("kernel corrections are not implemented")
         if state_equation is None:

@@ -24,7 +24,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import native
-from .schemes import StateEquationAdaptiveCole, WallBoundarySystem, WeaklyCompressibleSPHSystem
+from .schemes import (StateEquationAdaptiveCole, ViscosityAdami, WallBoundarySystem,
+                      WeaklyCompressibleSPHSystem)
 
 __all__ = ["Semidiscretization", "semidiscretize", "DynamicalODEProblem", "kick", "drift",
            "calculate_dt", "solve_tstops",
@@ -149,6 +150,18 @@ class Semidiscretization:
             bm = self.wall.boundary_model
             p.clip_wall_pressure = int(bm.clip_negative_pressure)
             p.pressure_offset = bm.density_calculator.pressure_offset
+            if bm.viscosity is not None:
+                # the fluid side of the wall terms uses the fluid's kinematic viscosity
+                # (wall_boundary/system.jl:286-289), defined here for Monaghan's model
+                if f.viscosity is None:
+                    raise NotImplementedError("a wall viscosity needs the fluid's "
+                                              "ArtificialViscosityMonaghan")
+                p.wall_epsilon = bm.viscosity.epsilon
+                if isinstance(bm.viscosity, ViscosityAdami):
+                    p.wall_viscosity, p.wall_nu = 2, bm.viscosity.nu
+                else:
+                    p.wall_viscosity = 1
+                    p.wall_alpha, p.wall_beta = bm.viscosity.alpha, bm.viscosity.beta
             wse = bm.state_equation
             if isinstance(se, StateEquationAdaptiveCole):
                 # the reference shares one adaptive object between the two systems
