@@ -394,27 +394,10 @@ __global__ void k_sort_cells(const int *flag, int ncells, const int *__restrict_
 
 __global__ void k_set_i32(int *p, int value) { *p = value; }
 
-// Processing order of the list kernel: the cell-sorted particles grouped by bricks of cells
-// about 2R wide (4R in 2D), so the 64 targets of a wavefront are a compact cluster and the
-// union of their neighbourhoods is small.  The order inside a brick follows the histogram
-// atomics; it only decides which lane handles which particle, never a result.
-template <int ND>
-__global__ void k_brick_count(const int *flag, DevParams P, const double *__restrict__ xs,
-                              const double *__restrict__ ys, const double *__restrict__ zs,
-                              int *__restrict__ bid, int *__restrict__ rank,
-                              int *__restrict__ bcount)
-{
-    if (skip(flag)) return;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < P.nf; s += gridDim.x * blockDim.x) {
-        const int c0 = cell_1d(xs[s], P.lo[0], P.inv_cs[0], P.n[0]);
-        const int c1 = cell_1d(ys[s], P.lo[1], P.inv_cs[1], P.n[1]);
-        const int c2 = (ND == 3) ? cell_1d(zs[s], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
-        const int b = c0 / P.bc[0] + P.nb[0] * (c1 / P.bc[1] + P.nb[1] * (c2 / P.bc[2]));
-        bid[s] = b;
-        rank[s] = atomicAdd(&bcount[b], 1);
-    }
-}
-
+// Processing order of the list kernels: the cell-sorted particles grouped by bricks of cells
+// (counted by k_gather<1>), so the targets of a block are a compact cluster and the union of
+// their neighbourhoods is small.  The order inside a brick follows the histogram atomics; it
+// only decides which lane handles which particle, never a result.
 // ord[t] = sorted index of the t-th particle in brick order; clears the brick histogram.
 __global__ void k_brick_scatter(const int *flag, int n, const int *__restrict__ bid,
                                 const int *__restrict__ rank, const int *__restrict__ bstart,
@@ -435,13 +418,18 @@ __global__ void k_brick_scatter(const int *flag, int n, const int *__restrict__ 
 //           follows gathers everything again.
 //   MODE 1: runs after a rebuild; stores the build positions.
 // ------------------------------------------------------------------------------------------
+// MODE 1 with bcount also counts the particles of every brick of cells (the processing
+// order of the list kernels: see k_brick_scatter), recording each particle's brick and rank,
+// and zeroes the `nzero` counters of the list build.
 template <int ND, int MODE>
 __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restrict__ perm,
                          const double *__restrict__ u, const double *__restrict__ v,
                          const double *__restrict__ m, double *__restrict__ xs,
                          double *__restrict__ ys, double *__restrict__ zs,
                          double4 *__restrict__ rec, double2 *__restrict__ vol,
-                         double *__restrict__ snap, int *__restrict__ iperm)
+                         double *__restrict__ snap, int *__restrict__ iperm,
+                         int *__restrict__ bid, int *__restrict__ rank,
+                         int *__restrict__ bcount, int *__restrict__ zero, int nzero)
 {
     P = with_sound(P);
     const int f = *flag;
@@ -479,8 +467,17 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
 #pragma unroll
         for (int d = 0; d < ND; d++) snap[d * nf + s] = x[d];
         iperm[i] = s;
+        if (bcount) {
+            const int c0 = cell_1d(x[0], P.lo[0], P.inv_cs[0], P.n[0]);
+            const int c1 = cell_1d(x[1], P.lo[1], P.inv_cs[1], P.n[1]);
+            const int c2 = (ND == 3) ? cell_1d(x[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+            const int b = c0 / P.bc[0] + P.nb[0] * (c1 / P.bc[1] + P.nb[1] * (c2 / P.bc[2]));
+            bid[s] = b;
+            rank[s] = atomicAdd(&bcount[b], 1);
+        }
     }
     }
+    if (MODE == 1 && zero && blockIdx.x == 0 && (int)threadIdx.x < nzero) zero[threadIdx.x] = 0;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -945,14 +942,14 @@ __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &
 // Fluid and wall lists of every fluid particle (the wall lists only with walls), stored in
 // column t for the particle ord[t].
 template <int ND>
-__global__ void __launch_bounds__(kBlock)
-    k_lists(const int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
+__device__ __forceinline__ void
+    lists_body(int bx, int gx, const int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
             int ls, int cap_f, int *__restrict__ nbr, int *__restrict__ cnt, int cap_w,
             int *__restrict__ wnbr, int *__restrict__ wcnt, int *__restrict__ wtargets,
             int *__restrict__ nwtargets)
 {
     if (skip(flag)) return;
-    for (int t = blockIdx.x * kBlock + threadIdx.x; t < P.nf; t += gridDim.x * kBlock) {
+    for (int t = bx * kBlock + threadIdx.x; t < P.nf; t += gx * kBlock) {
         const int s = ord[t];
         build_list<ND, false>(P, F, Wv, s, t, ls, cap_f, nbr, cnt);
         if (P.nw > 0) {
@@ -965,15 +962,15 @@ __global__ void __launch_bounds__(kBlock)
 // Verlet lists of the wall particles (wall-sorted order w) over the fluid: candidates within
 // R + skin/2 of the static wall particle, for the Adami pass.
 template <int ND>
-__global__ void __launch_bounds__(kBlock)
-    k_wall_lists(const int *flag, DevParams P, FluidView F, const double4 *__restrict__ wpos,
+__device__ __forceinline__ void
+    wall_lists_body(int bx, int gx, const int *flag, DevParams P, FluidView F, const double4 *__restrict__ wpos,
                  int lw, int cap, int *__restrict__ nbr, int *__restrict__ cnt,
                  int *__restrict__ active, int *__restrict__ nactive, double *__restrict__ pw,
                  double *__restrict__ rhow, double2 *__restrict__ wpv, double *__restrict__ wvel)
 {
     P = with_sound(P);
     if (skip(flag)) return;
-    for (int w = blockIdx.x * kBlock + threadIdx.x; w < P.nw; w += gridDim.x * kBlock) {
+    for (int w = bx * kBlock + threadIdx.x; w < P.nw; w += gx * kBlock) {
         const double4 xw = wpos[w];
         Pencils<ND> it;
         it.init(P, xw.x, xw.y, xw.z, P.hs_list_wall);
@@ -1476,11 +1473,11 @@ __global__ void __launch_bounds__(TileCfg<ND>::THREADS)
 // (bntot = INT_MAX) and its targets read their candidates from global memory.
 // ------------------------------------------------------------------------------------------
 
-// Union rows of every brick (one block per brick, grid-stride) and the list of non-empty
-// bricks.  Row r = (cy - (by bc[1] - ur[1])) + nuy (cz - (bz bc[2] - ur[2])).
+// Union rows of every brick (one block per brick, grid-stride) and, in the first block, the
+// list of non-empty bricks.  Row r = (cy - (by bc[1] - ur[1])) + nuy (cz - (bz bc[2] - ur[2])).
 template <int ND>
-__global__ void __launch_bounds__(kBlock)
-    k_brick_union(const int *flag, DevParams P, const int *__restrict__ start,
+__device__ __forceinline__ void
+    brick_union_body(int bx, int gx, const int *flag, DevParams P, const int *__restrict__ start,
                   const int *__restrict__ bstart, int nbricks, int *__restrict__ brow_gb,
                   int *__restrict__ brow_lds, int *__restrict__ bntot, int *__restrict__ nel,
                   int *__restrict__ nne)
@@ -1488,8 +1485,30 @@ __global__ void __launch_bounds__(kBlock)
     if (skip(flag)) return;
     __shared__ int scan_buf[kBlock];
     const int tid = threadIdx.x;
+    if (bx == 0) {
+        // the non-empty bricks in brick order (the order the brick kernel takes them in;
+        // neighbouring bricks in flight share their unions in L2)
+        int carry = 0;
+        for (int c = 0; c < nbricks; c += kScanTile) {
+            const int b0 = c + tid * kScanItems;
+            int f[kScanItems], n = 0;
+#pragma unroll
+            for (int k = 0; k < kScanItems; k++) {
+                const int b = b0 + k;
+                f[k] = (b < nbricks && bstart[b + 1] > bstart[b]) ? 1 : 0;
+                n += f[k];
+            }
+            int tot;
+            int pos = carry + block_exclusive_scan(n, scan_buf, tot);
+#pragma unroll
+            for (int k = 0; k < kScanItems; k++)
+                if (f[k]) nel[pos++] = b0 + k;
+            carry += tot;
+        }
+        if (tid == 0) *nne = carry;
+    }
     const int nuy = P.bc[1] + 2 * P.ur[1];
-    for (int b = blockIdx.x; b < nbricks; b += gridDim.x) {
+    for (int b = bx; b < nbricks; b += gx) {
         const int bx = b % P.nb[0], by = (b / P.nb[0]) % P.nb[1], bz = b / (P.nb[0] * P.nb[1]);
         const int X0 = bx * P.bc[0], X1 = min(X0 + P.bc[0], P.n[0]) - 1;
         const int Y0 = by * P.bc[1], Y1 = min(Y0 + P.bc[1], P.n[1]) - 1;
@@ -1527,7 +1546,6 @@ __global__ void __launch_bounds__(kBlock)
         if (tid == 0) {
             brow_lds[(size_t)b * (P.maxr + 1) + P.maxr] = tot;
             bntot[b] = tot;
-            if (bstart[b + 1] > bstart[b]) nel[atomicAdd(nne, 1)] = b;
         }
     }
 }
@@ -1535,8 +1553,8 @@ __global__ void __launch_bounds__(kBlock)
 // LDS indices of the fluid lists of every target (the same entries as nbr, in the same
 // order), and the marks of the bricks that cannot stage.
 template <int ND>
-__global__ void __launch_bounds__(kBlock)
-    k_brick_lists(const int *flag, DevParams P, FluidView F, const int *__restrict__ ord,
+__device__ __forceinline__ void
+    brick_lists_body(int bx, int gx, const int *flag, DevParams P, FluidView F, const int *__restrict__ ord,
                   const int *__restrict__ nbr, const int *__restrict__ cnt,
                   const int *__restrict__ wcnt, const int *__restrict__ brow_gb,
                   const int *__restrict__ brow_lds, int *__restrict__ bntot, int ls,
@@ -1544,7 +1562,7 @@ __global__ void __launch_bounds__(kBlock)
 {
     if (skip(flag)) return;
     const int nuy = P.bc[1] + 2 * P.ur[1];
-    for (int t = blockIdx.x * kBlock + threadIdx.x; t < P.nf; t += gridDim.x * kBlock) {
+    for (int t = bx * kBlock + threadIdx.x; t < P.nf; t += gx * kBlock) {
         const int s = ord[t];
         const int c0 = cell_1d(F.x[s], P.lo[0], P.inv_cs[0], P.n[0]);
         const int c1 = cell_1d(F.y[s], P.lo[1], P.inv_cs[1], P.n[1]);
@@ -1577,6 +1595,64 @@ __global__ void __launch_bounds__(kBlock)
             nbr16[((size_t)(k >> 2) * ls + t) * 4 + (k & 3)] = (unsigned short)(kBrickCap - 1);
         if (!ok) bntot[b] = INT_MAX;
     }
+}
+
+// The list builds of a rebuild in two launches: the fluid and wall Verlet lists with the
+// brick unions, then the brick (LDS-index) lists with the Adami lists of the walls.  Each
+// launch gives the first part its own block range and the second part the rest.
+struct ListsArgs {
+    const int *ord;
+    int ls, cap_f;
+    int *nbr, *cnt;
+    int cap_w;
+    int *wnbr, *wcnt, *wtargets, *nwtargets;
+};
+struct UnionArgs {
+    const int *start, *bstart;
+    int nbricks;
+    int *brow_gb, *brow_lds, *bntot, *nel, *nne;
+};
+struct BrickListsArgs {
+    const int *ord, *nbr, *cnt, *wcnt, *brow_gb, *brow_lds;
+    int *bntot;
+    int ls;
+    unsigned short *nbr16;
+};
+struct WallListsArgs {
+    const double4 *wpos;
+    int lw, cap;
+    int *nbr, *cnt, *active, *nactive;
+    double *pw, *rhow;
+    double2 *wpv;
+    double *wvel;
+};
+
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_brick_union_and_lists(const int *flag, DevParams P, FluidView F, WallView Wv, ListsArgs a,
+                            UnionArgs b, int nblocks_lists)
+{
+    if ((int)blockIdx.x < nblocks_lists)
+        lists_body<ND>(blockIdx.x, nblocks_lists, flag, P, F, Wv, a.ord, a.ls, a.cap_f, a.nbr,
+                       a.cnt, a.cap_w, a.wnbr, a.wcnt, a.wtargets, a.nwtargets);
+    else
+        brick_union_body<ND>(blockIdx.x - nblocks_lists, gridDim.x - nblocks_lists, flag, P,
+                             b.start, b.bstart, b.nbricks, b.brow_gb, b.brow_lds, b.bntot, b.nel,
+                             b.nne);
+}
+
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_brick_lists_and_walls(const int *flag, DevParams P, FluidView F, BrickListsArgs a,
+                            WallListsArgs w, int nblocks_bricks)
+{
+    if ((int)blockIdx.x < nblocks_bricks)
+        brick_lists_body<ND>(blockIdx.x, nblocks_bricks, flag, P, F, a.ord, a.nbr, a.cnt, a.wcnt,
+                             a.brow_gb, a.brow_lds, a.bntot, a.ls, a.nbr16);
+    else
+        wall_lists_body<ND>(blockIdx.x - nblocks_bricks, gridDim.x - nblocks_bricks, flag, P, F,
+                            w.wpos, w.lw, w.cap, w.nbr, w.cnt, w.active, w.nactive, w.pw, w.rhow,
+                            w.wpv, w.wvel);
 }
 
 // Staged union: one record per particle as four 16-byte fields and one 8-byte field, each in
@@ -2243,7 +2319,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
         TimedScope t(h, TPSPH_TIME_GATHER);
         hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, h->bwork, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
-            h->snap, nullptr);
+            h->snap, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
         TRY(launch_check(h, "k_gather<0>"));
     }
     {
@@ -2256,32 +2332,36 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
                 h->flag, h->P, h->f_start, h->tgx, h->tgy, h->ntiles_all, h->tiles, h->ntiles);
             TRY(launch_check(h, "k_tiles"));
         }
+        const bool lists = h->search == TPSPH_SEARCH_LISTS;
+        // with lists the gather also counts the bricks (f_cell / f_rank are free again after
+        // the cell list build) and zeroes the list-build counters
         hipLaunchKernelGGL((k_gather<ND, 1>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, nullptr, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
-            h->snap, h->iperm);
-        if (h->search == TPSPH_SEARCH_LISTS) {
-            hipLaunchKernelGGL((k_fill_i32), dim3(1), dim3(kBlock), 0, h->stream, h->flag, h->counters, 0, 3);
-            // brick order (f_cell / f_rank are free again after the cell list build)
-            hipLaunchKernelGGL((k_brick_count<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
-                h->flag, h->P, h->xs, h->ys, h->zs, h->f_cell, h->f_rank, h->bcount);
+            h->snap, h->iperm, lists ? h->f_cell : nullptr, lists ? h->f_rank : nullptr,
+            lists ? h->bcount : nullptr, lists ? h->counters : nullptr, 3);
+        if (lists) {
             TRY(scan(h, h->flag, h->bcount, h->bstart, h->nbricks + 1));
             hipLaunchKernelGGL((k_brick_scatter), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
                 h->flag, nf, h->f_cell, h->f_rank, h->bstart, h->ord, h->bcount);
-            hipLaunchKernelGGL((k_lists<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
-                h->flag, h->P, h->fluid_view(), h->wall_view(), h->ord, h->ls, h->cap_f, h->nbr,
-                h->cnt, h->cap_w, h->wnbr, h->wcnt, h->wtargets, h->counters + 2);
-            if (h->bricks) {
-                hipLaunchKernelGGL((k_brick_union<ND>), dim3(std::max(1, std::min(h->nbricks, kGateBlocks))),
-                    dim3(kBlock), 0, h->stream, h->flag, h->P, h->f_start, h->bstart, h->nbricks,
-                    h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters);
-                hipLaunchKernelGGL((k_brick_lists<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
-                    h->flag, h->P, h->fluid_view(), h->ord, h->nbr, h->cnt, h->wcnt, h->brow_gb,
-                    h->brow_lds, h->bntot, h->ls, h->nbr16);
-            }
-            if (h->nw > 0)
-                hipLaunchKernelGGL((k_wall_lists<ND>), dim3(gate_blocks(h->nw)), dim3(kBlock), 0, h->stream,
-                    h->flag, h->P, h->fluid_view(), h->wpos, h->lw, h->cap_a, h->anbr, h->acnt,
-                    h->wactive, h->counters + 1, h->pw, h->rhow, h->wpv, h->wvel);
+            const int gl = gate_blocks(nf);
+            const int gu = h->bricks ? std::max(1, std::min(h->nbricks, kGateBlocks)) : 0;
+            hipLaunchKernelGGL((k_brick_union_and_lists<ND>), dim3(gl + gu), dim3(kBlock), 0,
+                h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
+                ListsArgs{h->ord, h->ls, h->cap_f, h->nbr, h->cnt, h->cap_w, h->wnbr, h->wcnt,
+                          h->wtargets, h->counters + 2},
+                UnionArgs{h->f_start, h->bstart, h->nbricks, h->brow_gb, h->brow_lds, h->bntot,
+                          h->nel, h->counters},
+                gl);
+            const int gb = h->bricks ? gate_blocks(nf) : 0;
+            const int gw = h->nw > 0 ? gate_blocks(h->nw) : 0;
+            if (gb + gw > 0)
+                hipLaunchKernelGGL((k_brick_lists_and_walls<ND>), dim3(gb + gw), dim3(kBlock), 0,
+                    h->stream, h->flag, h->P, h->fluid_view(),
+                    BrickListsArgs{h->ord, h->nbr, h->cnt, h->wcnt, h->brow_gb, h->brow_lds,
+                                   h->bntot, h->ls, h->nbr16},
+                    WallListsArgs{h->wpos, h->lw, h->cap_a, h->anbr, h->acnt, h->wactive,
+                                  h->counters + 1, h->pw, h->rhow, h->wpv, h->wvel},
+                    gb);
             TRY(launch_check(h, "k_lists"));
         }
         // the fluid kernel of this kick lowers the flag
