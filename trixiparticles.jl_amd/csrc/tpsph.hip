@@ -429,7 +429,8 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
                          double4 *__restrict__ rec, double2 *__restrict__ vol,
                          double *__restrict__ snap, int *__restrict__ iperm,
                          int *__restrict__ bid, int *__restrict__ rank,
-                         int *__restrict__ bcount, int *__restrict__ zero, int nzero)
+                         int *__restrict__ bcount, int *__restrict__ zero, int nzero,
+                         const int *__restrict__ noverflow)
 {
     P = with_sound(P);
     const int f = *flag;
@@ -437,6 +438,9 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
     if (MODE == 0 && work && blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
     if ((MODE == 0) == (f != 0)) return;
     const size_t nf = (size_t)P.nf;
+    // with Verlet lists the SoA coordinates are read between rebuilds only by the cell walk
+    // of overflowed lists (noverflow counts them; nullptr = always needed)
+    const bool soa = !noverflow || *noverflow != 0;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < P.nf; s += gridDim.x * blockDim.x) {
     const int i = perm[s];
     double x[3] = {0.0, 0.0, 0.0}, vv[3] = {0.0, 0.0, 0.0};
@@ -447,9 +451,11 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
     }
     const double rho = v[(size_t)i * (ND + 1) + ND];
     const double mass = m[i];
-    xs[s] = x[0];
-    ys[s] = x[1];
-    if constexpr (ND == 3) zs[s] = x[2];
+    if (MODE == 1 || soa) {
+        xs[s] = x[0];
+        ys[s] = x[1];
+        if constexpr (ND == 3) zs[s] = x[2];
+    }
     rec[2 * (size_t)s] = make_double4(x[0], x[1], x[2], rho);
     rec[2 * (size_t)s + 1] = make_double4(vv[0], vv[1], vv[2], state_equation(P, rho));
     vol[s] = make_double2(mass / rho, mass);
@@ -946,12 +952,14 @@ __device__ __forceinline__ void
     lists_body(int bx, int gx, const int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
             int ls, int cap_f, int *__restrict__ nbr, int *__restrict__ cnt, int cap_w,
             int *__restrict__ wnbr, int *__restrict__ wcnt, int *__restrict__ wtargets,
-            int *__restrict__ nwtargets)
+            int *__restrict__ nwtargets, int *__restrict__ noverflow)
 {
     if (skip(flag)) return;
     for (int t = bx * kBlock + threadIdx.x; t < P.nf; t += gx * kBlock) {
         const int s = ord[t];
         build_list<ND, false>(P, F, Wv, s, t, ls, cap_f, nbr, cnt);
+        // an overflowed list walks the cells, which need the SoA coordinates every kick
+        if (cnt[t] < 0) atomicAdd(noverflow, 1);
         if (P.nw > 0) {
             build_list<ND, true>(P, F, Wv, s, t, ls, cap_w, wnbr, wcnt);
             if (wtargets && wcnt[t] != 0) wtargets[atomicAdd(nwtargets, 1)] = t;
@@ -1605,7 +1613,7 @@ struct ListsArgs {
     int ls, cap_f;
     int *nbr, *cnt;
     int cap_w;
-    int *wnbr, *wcnt, *wtargets, *nwtargets;
+    int *wnbr, *wcnt, *wtargets, *nwtargets, *noverflow;
 };
 struct UnionArgs {
     const int *start, *bstart;
@@ -1634,7 +1642,7 @@ __global__ void __launch_bounds__(kBlock)
 {
     if ((int)blockIdx.x < nblocks_lists)
         lists_body<ND>(blockIdx.x, nblocks_lists, flag, P, F, Wv, a.ord, a.ls, a.cap_f, a.nbr,
-                       a.cnt, a.cap_w, a.wnbr, a.wcnt, a.wtargets, a.nwtargets);
+                       a.cnt, a.cap_w, a.wnbr, a.wcnt, a.wtargets, a.nwtargets, a.noverflow);
     else
         brick_union_body<ND>(blockIdx.x - nblocks_lists, gridDim.x - nblocks_lists, flag, P,
                              b.start, b.bstart, b.nbricks, b.brow_gb, b.brow_lds, b.bntot, b.nel,
@@ -1946,7 +1954,7 @@ __global__ void __launch_bounds__(kBlock)
                       const int *__restrict__ iperm, const double *__restrict__ m,
                       double *__restrict__ xs, double *__restrict__ ys, double *__restrict__ zs,
                       double4 *__restrict__ rec, double2 *__restrict__ vol,
-                      const double *__restrict__ snap)
+                      const double *__restrict__ snap, const int *__restrict__ noverflow)
 {
     P = with_sound(P);
     const int i0 = blockIdx.x * kBlock;
@@ -1981,9 +1989,11 @@ __global__ void __launch_bounds__(kBlock)
     }
     const double rho = v[(size_t)i * (ND + 1) + ND];
     const double mass = m[i];
-    xs[s] = x[0];
-    ys[s] = x[1];
-    if constexpr (ND == 3) zs[s] = x[2];
+    if (!noverflow || *noverflow != 0) {  // see k_gather
+        xs[s] = x[0];
+        ys[s] = x[1];
+        if constexpr (ND == 3) zs[s] = x[2];
+    }
     rec[2 * (size_t)s] = make_double4(x[0], x[1], x[2], rho);
     rec[2 * (size_t)s + 1] = make_double4(vv[0], vv[1], vv[2], state_equation(P, rho));
     vol[s] = make_double2(mass / rho, mass);
@@ -2319,7 +2329,8 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
         TimedScope t(h, TPSPH_TIME_GATHER);
         hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, h->bwork, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
-            h->snap, nullptr, nullptr, nullptr, nullptr, nullptr, 0);
+            h->snap, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+            h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr);
         TRY(launch_check(h, "k_gather<0>"));
     }
     {
@@ -2338,7 +2349,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
         hipLaunchKernelGGL((k_gather<ND, 1>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, nullptr, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
             h->snap, h->iperm, lists ? h->f_cell : nullptr, lists ? h->f_rank : nullptr,
-            lists ? h->bcount : nullptr, lists ? h->counters : nullptr, 3);
+            lists ? h->bcount : nullptr, lists ? h->counters : nullptr, 4, nullptr);
         if (lists) {
             TRY(scan(h, h->flag, h->bcount, h->bstart, h->nbricks + 1));
             hipLaunchKernelGGL((k_brick_scatter), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
@@ -2348,7 +2359,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
             hipLaunchKernelGGL((k_brick_union_and_lists<ND>), dim3(gl + gu), dim3(kBlock), 0,
                 h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
                 ListsArgs{h->ord, h->ls, h->cap_f, h->nbr, h->cnt, h->cap_w, h->wnbr, h->wcnt,
-                          h->wtargets, h->counters + 2},
+                          h->wtargets, h->counters + 2, h->counters + 3},
                 UnionArgs{h->f_start, h->bstart, h->nbricks, h->brow_gb, h->brow_lds, h->bntot,
                           h->nel, h->counters},
                 gl);
@@ -3110,11 +3121,13 @@ int tpsph_solve_ck2n54(tpsph_handle *h, double *v_ode, double *u_ode, double t0,
             if (h->nd == 2)
                 hipLaunchKernelGGL((k_ck_stage_gather<2>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
                     h->flag, h->bwork, h->P, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt,
-                    i == 0, h->iperm, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
+                    i == 0, h->iperm, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap,
+                    h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr);
             else
                 hipLaunchKernelGGL((k_ck_stage_gather<3>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
                     h->flag, h->bwork, h->P, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt,
-                    i == 0, h->iperm, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap);
+                    i == 0, h->iperm, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap,
+                    h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr);
             TRY(launch_check(h, "k_ck_stage_gather"));
             gathered = true;
         }
