@@ -70,6 +70,13 @@ enum tpsph_pair_search {
     TPSPH_SEARCH_TILES = 3   /* cell walk with LDS-staged tiles (default cell divisions) */
 };
 
+/* Density diffusion of the fluid (tpsph_params.use_density_diffusion). */
+enum tpsph_density_diffusion {
+    TPSPH_DIFFUSION_NONE = 0,      /* nothing                                              */
+    TPSPH_DIFFUSION_MOLTENI = 1,   /* DensityDiffusionMolteniColagrossi(; delta)           */
+    TPSPH_DIFFUSION_ANTUONO = 2    /* DensityDiffusionAntuono(; delta), not with TILES     */
+};
+
 /* Kernel identifiers for tpsph_kernel_time. */
 enum tpsph_timed_kernel {
     TPSPH_TIME_CELL_LIST = 0,   /* fluid cell list (counting sort) and Verlet lists        */
@@ -90,7 +97,14 @@ enum tpsph_timed_kernel {
  *   clip_fluid_eos
  *   use_viscosity, alpha, beta,        ArtificialViscosityMonaghan (viscosity.jl:68-83)
  *   epsilon
- *   use_density_diffusion, delta       DensityDiffusionMolteniColagrossi (density_diffusion.jl:41-47)
+ *   use_density_diffusion, delta       enum tpsph_density_diffusion with its delta:
+ *                                      DensityDiffusionMolteniColagrossi
+ *                                      (density_diffusion.jl:41-55) or DensityDiffusionAntuono
+ *                                      (density_diffusion.jl:89-211), whose renormalised
+ *                                      density gradients every kick computes in a pass of its
+ *                                      own; Antuono runs with the Verlet lists (without the
+ *                                      brick kernel) or the cell walk, not with
+ *                                      TPSPH_SEARCH_TILES
  *   acceleration                       `acceleration` of the fluid system (gravity)
  *   clip_wall_pressure                 BoundaryModelDummyParticles(; clip_negative_pressure)
  *   pressure_offset                    AdamiPressureExtrapolation(; pressure_offset)
@@ -135,7 +149,7 @@ typedef struct tpsph_params {
     int32_t clip_fluid_eos;        /* 0/1 */
     int32_t clip_wall_pressure;    /* 0/1 */
     int32_t use_viscosity;         /* 0/1 */
-    int32_t use_density_diffusion; /* 0/1 */
+    int32_t use_density_diffusion; /* enum tpsph_density_diffusion */
     int32_t cell_divisions;        /* 0 = default (2) */
     int32_t x_cell_divisions;      /* 0 = default (2 * cell_divisions) */
     int32_t pair_search;           /* enum tpsph_pair_search, 0 = auto */

@@ -87,7 +87,8 @@ def lib():
         L.oracle_compute_pressure.argtypes = [P, i64, P, P]
         L.oracle_compute_pressure.restype = None
         L.oracle_adami.argtypes = [P, i64, P, i64, P, P, P, P, P, P]
-        L.oracle_interact_fluid_fluid.argtypes = [P, i64, P, P, P, P, P]
+        L.oracle_interact_fluid_fluid.argtypes = [P, i64, P, P, P, P, P, P]
+        L.oracle_antuono_gradient.argtypes = [P, i64, P, P, P, P, P]
         L.oracle_interact_fluid_wall.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, P, P]
         L.oracle_wall_velocity.argtypes = [P, i64, P, i64, P, P, P]
         L.oracle_av_adami.argtypes = [C.c_int, d, d, d, d, P, d, P, d, d, d, d, P, P, P]
@@ -145,7 +146,8 @@ def params_from_semi(semi) -> OracleParams:
         P.use_viscosity = 1
         P.alpha, P.beta, P.epsilon = f.viscosity.alpha, f.viscosity.beta, f.viscosity.epsilon
     if f.density_diffusion is not None:
-        P.use_density_diffusion = 1
+        # DensityDiffusionAntuono carries the correction-matrix pass (kind 2)
+        P.use_density_diffusion = getattr(f.density_diffusion, "kind", 1)
         P.delta = f.density_diffusion.delta
     for d in range(f.ndims):
         P.gravity[d] = f.acceleration[d]
@@ -206,13 +208,31 @@ def adami(P, xw, x, v, pf):
     return pw, rw, vol
 
 
-def interact_fluid_fluid(P, x, v, m, pf, dv=None):
+def interact_fluid_fluid(P, x, v, m, pf, dv=None, ddgrad=None):
+    """Fluid-fluid terms; ``ddgrad`` (nf x nd) are the DensityDiffusionAntuono gradients,
+    needed when ``P.use_density_diffusion == 2``."""
     x, v, m, pf = _f64(x), _f64(v), _f64(m), _f64(pf)
     dv = np.zeros_like(v) if dv is None else dv
+    if P.use_density_diffusion == 2:
+        assert ddgrad is not None, "DensityDiffusionAntuono needs its density gradients"
+        ddgrad = _f64(ddgrad)
     rc = lib().oracle_interact_fluid_fluid(C.byref(P), x.shape[0], _p(x), _p(v), _p(m),
-                                           _p(pf), _p(dv))
+                                           _p(pf), _p(dv), _p(ddgrad))
     assert rc == 0
     return dv
+
+
+def antuono_gradient(P, x, v, m):
+    """DensityDiffusionAntuono update!: (normalised density gradients nf x nd, inverted
+    correction matrices nf x nd x nd with [a, i, j] = L_a[i, j])."""
+    x, v, m = _f64(x), _f64(v), _f64(m)
+    nf, nd = x.shape[0], P.ndims
+    grad = np.zeros((nf, nd))
+    corr = np.zeros((nf, nd, nd))  # column-major per particle: [a, j, i] in C order
+    rc = lib().oracle_antuono_gradient(C.byref(P), nf, _p(x), _p(v), _p(m), _p(grad),
+                                       _p(corr))
+    assert rc == 0
+    return grad, np.ascontiguousarray(corr.transpose(0, 2, 1))
 
 
 def interact_fluid_wall(P, x, v, pf, xw, mw, pw, rhow, dv=None, m=None, wvel=None):

@@ -46,7 +46,8 @@ typedef struct {
     int32_t clip_fluid_eos;       /* StateEquationCole(clip_negative_pressure=...) */
     int32_t clip_wall_pressure;   /* BoundaryModelDummyParticles(clip_negative_pressure=...) */
     int32_t use_viscosity;        /* fluid viscosity = ArtificialViscosityMonaghan, else nothing */
-    int32_t use_density_diffusion;/* DensityDiffusionMolteniColagrossi, else nothing */
+    int32_t use_density_diffusion;/* 0 nothing, 1 DensityDiffusionMolteniColagrossi,
+                                     2 DensityDiffusionAntuono */
     double smoothing_length;
     double reference_density;
     double sound_speed;
@@ -550,6 +551,119 @@ int oracle_wall_velocity(const oracle_params *P, int64_t nw, const double *xw, i
 }
 
 /* ---------------------------------------------------------------------------------------
+ * DensityDiffusionAntuono's update! (density_diffusion.jl:176-211) over the fluid's own
+ * neighbours:
+ *   1. compute_gradient_correction_matrix! (corrections.jl:329-354): per particle a,
+ *      corr_a -= (V_b grad W_ab) * r_ab^T, V_b = m_b / rho_b, skipping zero gradients;
+ *   2. correction_matrix_inversion_step! (corrections.jl:425-457): corr_a = inv(corr_a), or
+ *      the identity where |det(corr_a)| < 1f-9;
+ *   3. the normalised density gradient, summed pair by pair as the reference does:
+ *      grad_a += ((-(rho_a - rho_b)) * L_a) * grad W_ab * V_b.
+ * grad W is smoothing_kernel_grad (abstract_system.jl:142-155): zero at distance >= R and at
+ * distance^2 < eps(h^2).  grad (nd x nf) receives the gradients and corr (nd*nd per
+ * particle, column-major), if not NULL, the inverted matrices.
+ * ------------------------------------------------------------------------------------- */
+static void smoothing_kernel_grad(const oracle_params *P, int nd, const double *pos_diff,
+                                  double distance, double *grad_kernel)
+{
+    double h = P->smoothing_length;
+    for (int d = 0; d < 3; d++) grad_kernel[d] = 0.0;
+    if (distance >= 2.0 * h || distance * distance < julia_eps(h * h)) return;
+    double kdr = deriv_div_r_unsafe(P->kernel, nd, distance, h);
+    for (int d = 0; d < nd; d++) grad_kernel[d] = kdr * pos_diff[d];
+}
+
+/* inv(L) of a 2x2 or 3x3 matrix (column-major) by its adjugate over the determinant,
+ * which is returned (the caller tests it against the singularity bound). */
+static double invert_small(int nd, const double *L, double *inv)
+{
+    if (nd == 2) {
+        double det = L[0] * L[3] - L[2] * L[1];
+        inv[0] = L[3] / det;
+        inv[1] = -L[1] / det;
+        inv[2] = -L[2] / det;
+        inv[3] = L[0] / det;
+        return det;
+    }
+    /* columns x0, x1, x2; det = x0 . (x1 x x2); the rows of the inverse are
+     * (x1 x x2, x2 x x0, x0 x x1) / det */
+    const double *x0 = L, *x1 = L + 3, *x2 = L + 6;
+    double y0[3] = {x1[1] * x2[2] - x1[2] * x2[1], x1[2] * x2[0] - x1[0] * x2[2],
+                    x1[0] * x2[1] - x1[1] * x2[0]};
+    double y1[3] = {x2[1] * x0[2] - x2[2] * x0[1], x2[2] * x0[0] - x2[0] * x0[2],
+                    x2[0] * x0[1] - x2[1] * x0[0]};
+    double y2[3] = {x0[1] * x1[2] - x0[2] * x1[1], x0[2] * x1[0] - x0[0] * x1[2],
+                    x0[0] * x1[1] - x0[1] * x1[0]};
+    double det = (x0[0] * y0[0] + x0[1] * y0[1]) + x0[2] * y0[2];
+    for (int j = 0; j < 3; j++) {
+        inv[0 + 3 * j] = y0[j] / det;
+        inv[1 + 3 * j] = y1[j] / det;
+        inv[2 + 3 * j] = y2[j] / det;
+    }
+    return det;
+}
+
+int oracle_antuono_gradient(const oracle_params *P, int64_t nf, const double *x,
+                            const double *v, const double *m, double *grad, double *corr_out)
+{
+    int nd = P->ndims, nn = nd * nd;
+    double R = 2.0 * P->smoothing_length;
+    double lo[3], hi[3];
+    bbox(nd, nf, x, 0, NULL, lo, hi);
+    grid_t g;
+    if (grid_build(&g, nd, R, lo, hi, nf, x)) return -1;
+    double *corr = corr_out ? corr_out : (double *)malloc(sizeof(double) * (size_t)(nf * nn + 1));
+    if (!corr) {
+        grid_free(&g);
+        return -1;
+    }
+    const double singular = (double)1.0e-9f; /* the Float32 literal 1.0f-9 */
+#pragma omp parallel for schedule(static)
+    for (int64_t a = 0; a < nf; a++) {
+        const double *xa = x + a * nd;
+        double L[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        FOREACH_NEIGHBOR(&g, xa, x, {
+            double grad_kernel[3];
+            smoothing_kernel_grad(P, nd, pos_diff, distance, grad_kernel);
+            int zero = 1;
+            for (int d = 0; d < nd; d++) zero &= grad_kernel[d] == 0.0;
+            if (zero) continue;
+            double volume = m[b] / v[b * (nd + 1) + nd];
+            for (int j = 0; j < nd; j++)
+                for (int i = 0; i < nd; i++)
+                    L[i + nd * j] -= (volume * grad_kernel[i]) * pos_diff[j];
+        });
+        double inv[9];
+        double det = invert_small(nd, L, inv);
+        for (int k = 0; k < nn; k++)
+            corr[a * nn + k] = fabs(det) < singular ? (k % (nd + 1) == 0 ? 1.0 : 0.0) : inv[k];
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t a = 0; a < nf; a++) {
+        const double *xa = x + a * nd;
+        const double *La = corr + a * nn;
+        double rho_a = v[a * (nd + 1) + nd];
+        double acc[3] = {0, 0, 0};
+        FOREACH_NEIGHBOR(&g, xa, x, {
+            double grad_kernel[3];
+            smoothing_kernel_grad(P, nd, pos_diff, distance, grad_kernel);
+            double rho_b = v[b * (nd + 1) + nd];
+            double volume_b = m[b] / rho_b;
+            double s = -(rho_a - rho_b);
+            for (int i = 0; i < nd; i++) {
+                double t = (s * La[i]) * grad_kernel[0];
+                for (int j = 1; j < nd; j++) t = t + (s * La[i + nd * j]) * grad_kernel[j];
+                acc[i] += t * volume_b;
+            }
+        });
+        for (int d = 0; d < nd; d++) grad[a * nd + d] = acc[d];
+    }
+    if (!corr_out) free(corr);
+    grid_free(&g);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------------------
  * WCSPH interact! (weakly_compressible_sph/rhs.jl:5-127) with ContinuityDensity, no
  * correction, pressure_acceleration_continuity_density (pressure_acceleration.jl:34-41),
  * ArtificialViscosityMonaghan (viscosity.jl:96-132), continuity_equation! (fluid.jl:160-186),
@@ -559,7 +673,7 @@ int oracle_wall_velocity(const oracle_params *P, int64_t nw, const double *xw, i
  * ------------------------------------------------------------------------------------- */
 int oracle_interact_fluid_fluid(const oracle_params *P, int64_t nf, const double *x,
                                 const double *v, const double *m, const double *pf,
-                                double *dv)
+                                double *dv, const double *ddgrad)
 {
     int nd = P->ndims;
     double h = P->smoothing_length;
@@ -604,6 +718,14 @@ int oracle_interact_fluid_fluid(const oracle_params *P, int64_t nf, const double
             if (P->use_density_diffusion) {
                 double volume_b = m_b / rho_b;
                 double psi_s = (2.0 * (rho_a - rho_b)) / (distance * distance);
+                if (P->use_density_diffusion == 2) {
+                    /* DensityDiffusionAntuono psi (density_diffusion.jl:155-174) */
+                    double result = 2.0 * (rho_a - rho_b);
+                    double gsum[3] = {0, 0, 0};
+                    for (int d = 0; d < nd; d++) gsum[d] = ddgrad[a * nd + d] + ddgrad[b * nd + d];
+                    result -= dot3(nd, gsum, pos_diff);
+                    psi_s = result / (distance * distance);
+                }
                 double psi[3] = {0, 0, 0};
                 for (int d = 0; d < nd; d++) psi[d] = psi_s * pos_diff[d];
                 double term = volume_b * dot3(nd, psi, grad_kernel);
@@ -719,7 +841,7 @@ int oracle_kick(const oracle_params *P0, int64_t nf, const double *v, const doub
     double *pf = (double *)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1));
     double *pw = (double *)malloc(sizeof(double) * (size_t)(nw > 0 ? nw : 1));
     double *rhow = (double *)malloc(sizeof(double) * (size_t)(nw > 0 ? nw : 1));
-    double *wvel = NULL;
+    double *wvel = NULL, *ddgrad = NULL;
     if (!pf || !pw || !rhow) { rc = -1; goto out; }
     memset(dv, 0, sizeof(double) * (size_t)(nf * (nd + 1)));
     oracle_compute_pressure(P, nf, v, pf);
@@ -729,7 +851,13 @@ int oracle_kick(const oracle_params *P0, int64_t nf, const double *v, const doub
         if (!wvel) { rc = -1; goto out; }
         if ((rc = oracle_wall_velocity(P, nw, xw, nf, x, v, wvel))) goto out;
     }
-    if ((rc = oracle_interact_fluid_fluid(P, nf, x, v, m, pf, dv))) goto out;
+    if (P->use_density_diffusion == 2) {
+        /* update_quantities! -> update!(::DensityDiffusionAntuono) */
+        ddgrad = (double *)malloc(sizeof(double) * (size_t)(nf * nd + 1));
+        if (!ddgrad) { rc = -1; goto out; }
+        if ((rc = oracle_antuono_gradient(P, nf, x, v, m, ddgrad, NULL))) goto out;
+    }
+    if ((rc = oracle_interact_fluid_fluid(P, nf, x, v, m, pf, dv, ddgrad))) goto out;
     if (nw > 0 && (rc = oracle_interact_fluid_wall(P, nf, x, v, pf, nw, xw, mw, pw, rhow, dv,
                                                    m, wvel)))
         goto out;
@@ -744,6 +872,7 @@ out:
     free(pw);
     free(rhow);
     free(wvel);
+    free(ddgrad);
     return rc;
 }
 

@@ -7,7 +7,8 @@ so it is imported through :func:`load_package` (registered as ``trixiparticles_j
 from .examples import DamBreak, dam_break, marrone_dam_break_2d
 from .schemes import (AdamiPressureExtrapolation, ArtificialViscosityMonaghan,
                       BoundaryModelDummyParticles, ContinuityDensity,
-                      DensityDiffusionMolteniColagrossi, SchoenbergCubicSplineKernel,
+                      DensityDiffusionAntuono, DensityDiffusionMolteniColagrossi,
+                      SchoenbergCubicSplineKernel,
                       StateEquationAdaptiveCole, StateEquationCole, ViscosityAdami,
                       WallBoundarySystem, WeaklyCompressibleSPHSystem, WendlandC2Kernel)
 from .semidiscretization import (CK2N54_A, CK2N54_B, CK2N54_C, CarpenterKennedy2N54,
@@ -19,7 +20,8 @@ from . import native
 
 __all__ = [
     "AdamiPressureExtrapolation", "ArtificialViscosityMonaghan", "BoundaryModelDummyParticles",
-    "ContinuityDensity", "DensityDiffusionMolteniColagrossi", "SchoenbergCubicSplineKernel",
+    "ContinuityDensity", "DensityDiffusionAntuono", "DensityDiffusionMolteniColagrossi",
+    "SchoenbergCubicSplineKernel",
     "StateEquationAdaptiveCole", "StateEquationCole", "ViscosityAdami", "WallBoundarySystem", "WeaklyCompressibleSPHSystem", "WendlandC2Kernel",
     "CK2N54_A", "CK2N54_B", "CK2N54_C", "CarpenterKennedy2N54", "DynamicalODEProblem",
     "Semidiscretization", "calculate_dt", "drift", "kick", "semidiscretize", "solve",

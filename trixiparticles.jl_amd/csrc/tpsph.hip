@@ -100,6 +100,12 @@ struct DevParams {
     // kinematic viscosity alpha h c / (2 ndims + 4) (viscosity.jl:82-87)
     int visc_w;
     double w_alpha, w_ac, w_beta, w_eps_h2, w_nu, nu_kin;  // w_ac = w_alpha c
+    // DensityDiffusionAntuono: the renormalised density gradient of every fluid particle
+    // (cell-sorted, ND per particle, written by k_antuono_gradient each kick) or nullptr
+    // for DensityDiffusionMolteniColagrossi; eps(h^2), the zero-distance bound of the
+    // gradient-correction pass (smoothing_kernel_grad, abstract_system.jl:142-155)
+    double *ddgrad;
+    double eps_hh;
 };
 
 // The sound-speed dependent constants of this kick: P's own, or the adaptive ones.  Kernels
@@ -627,6 +633,7 @@ __device__ __forceinline__ double rcp_pos(double x)
 struct FluidNbr {
     double4 a, b;  // {x, y, z, rho}, {vx, vy, vz, p}
     double2 v;     // {V, m}
+    int j;         // cell-sorted index (read only by the DensityDiffusionAntuono term)
 };
 
 struct WallNbr {
@@ -638,7 +645,7 @@ struct WallNbr {
 __device__ __forceinline__ FluidNbr load_fluid(const FluidView &F, int j)
 {
     const uint32_t o = (uint32_t)j;
-    return {ldo(F.rec, o * 64u), ldo(F.rec, o * 64u + 32u), ldo(F.vol, o * 16u)};
+    return {ldo(F.rec, o * 64u), ldo(F.rec, o * 64u + 32u), ldo(F.vol, o * 16u), j};
 }
 
 __device__ __forceinline__ WallNbr load_wall(const WallView &Wv, int j)
@@ -652,11 +659,15 @@ __device__ __forceinline__ WallNbr load_wall(const WallView &Wv, int j)
     return n;
 }
 
+// ga: the target's renormalised density gradient under DensityDiffusionAntuono, whose
+// psi_ab = 2 (rho_a - rho_b - (grad rho_a + grad rho_b).r_ab / 2) r_ab / r^2
+// (density_diffusion.jl:155-174) replaces the Molteni-Colagrossi 2 (rho_a - rho_b);
+// nullptr for Molteni-Colagrossi (or no density diffusion).
 template <int ND>
 __device__ __forceinline__ void pair_fluid(const DevParams &P, const FluidNbr &n,
                                            const double *xa, const double *va, double rho_a,
                                            double p_a, double inv_rho_a, double *acc,
-                                           double &drho)
+                                           double &drho, const double *ga = nullptr)
 {
     const double p0 = xa[0] - n.a.x, p1 = xa[1] - n.a.y, p2 = (ND == 3) ? xa[2] - n.a.z : 0.0;
     const double d2 = dist2<ND>(p0, p1, p2);
@@ -682,7 +693,14 @@ __device__ __forceinline__ void pair_fluid(const DevParams &P, const FluidNbr &n
     acc[0] = fma(fac, p0, acc[0]);
     acc[1] = fma(fac, p1, acc[1]);
     if constexpr (ND == 3) acc[2] = fma(fac, p2, acc[2]);
-    drho = fma(n.v.x * kdr, fma(P.dd2, rho_a - rho_b, rho_a * vr), drho);
+    double drr = rho_a - rho_b;
+    if (ga) {
+        const double *gb = P.ddgrad + (size_t)n.j * ND;
+        double gr = (ga[0] + gb[0]) * p0 + (ga[1] + gb[1]) * p1;
+        if constexpr (ND == 3) gr += (ga[2] + gb[2]) * p2;
+        drr = fma(-0.5, gr, drr);
+    }
+    drho = fma(n.v.x * kdr, fma(P.dd2, drr, rho_a * vr), drho);
 }
 
 // The wall viscosity (none without one) acts along r_ab for ArtificialViscosityMonaghan
@@ -785,7 +803,7 @@ __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *_
                                                 const WallView &Wv, int *lst, const double *xa,
                                                 const double *va, double rho_a, double p_a,
                                                 double inv_rho_a, double *acc, double &drho,
-                                                double m_a = 0.0)
+                                                double m_a = 0.0, const double *ga = nullptr)
 {
     Pencils<ND> it;
     it.init(P, xa[0], xa[1], (ND == 3) ? xa[2] : 0.0, hs);
@@ -829,7 +847,7 @@ __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *_
                 FluidNbr cur = load_fluid(F, lst[0]);
                 for (int k = 0; k < cnt; k++) {
                     const FluidNbr nxt = load_fluid(F, lst[min(k + 1, cnt - 1) * STRIDE]);
-                    pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                    pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, ga);
                     cur = nxt;
                 }
             }
@@ -863,10 +881,16 @@ __global__ void __launch_bounds__(kBlock)
         p_a = b.w;
     }
     const double inv_rho_a = 1.0 / rho_a;
+    double ga[3] = {0.0, 0.0, 0.0};  // DensityDiffusionAntuono: this particle's gradient
+    if (P.ddgrad && active) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) ga[d] = P.ddgrad[(size_t)s * ND + d];
+    }
 
     double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
     neighbour_sweep<ND, false>(P, F.start, F.x, F.y, F.z, P.hs_fluid, active, F, Wv, lst, xa,
-                               va, rho_a, p_a, inv_rho_a, acc, drho);
+                               va, rho_a, p_a, inv_rho_a, acc, drho, 0.0,
+                               P.ddgrad ? ga : nullptr);
     double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
     if (P.nw > 0) {
         // walls are static: only cells flagged at create can have wall neighbours
@@ -1058,6 +1082,12 @@ __global__ void __launch_bounds__(kBlock)
         if (P.nw > 0) nw = wcnt[col];
     }
     const double inv_rho_a = 1.0 / rho_a;
+    double ga[3] = {0.0, 0.0, 0.0};  // DensityDiffusionAntuono: this particle's gradient
+    if (P.ddgrad && active) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) ga[d] = P.ddgrad[(size_t)s * ND + d];
+    }
+    const double *gap = P.ddgrad ? ga : nullptr;
 
     double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
     {
@@ -1076,7 +1106,8 @@ __global__ void __launch_bounds__(kBlock)
             c3 = ldo(rec2, (uint32_t)quad_bcast<3>(j) * 64u + r16);
             vb = ldo(F.vol, (uint32_t)j * 16u);
         };
-        if (mw > 0) issue((m > 0) ? nbr[col] : sl);
+        int jc = (m > 0) ? nbr[col] : sl;  // candidate whose record is in flight
+        if (mw > 0) issue(jc);
         int jn = (1 < m) ? nbr[(size_t)ls + col] : sl;  // candidate of the next slot
         for (int k = 0; k < mw; k++) {
             mine[0] = c0;
@@ -1086,22 +1117,24 @@ __global__ void __launch_bounds__(kBlock)
             __builtin_amdgcn_wave_barrier();
             const double2 a0 = own[0], a1 = own[1], b0 = own[2], b1 = own[3];
             const FluidNbr cur = {make_double4(a0.x, a0.y, a1.x, a1.y),
-                                  make_double4(b0.x, b0.y, b1.x, b1.y), vb};
+                                  make_double4(b0.x, b0.y, b1.x, b1.y), vb, jc};
             __builtin_amdgcn_wave_barrier();
             if (k + 1 < mw) issue(jn);
+            jc = jn;
             jn = (k + 2 < m) ? nbr[(size_t)(k + 2) * ls + col] : sl;
             if (k < m) {
                 const double p0 = xa[0] - cur.a.x, p1 = xa[1] - cur.a.y;
                 const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
                 const double d2 = dist2<ND>(p0, p1, p2);
                 if (d2 <= P.R2 && d2 >= P.az2)
-                    pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                    pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, gap);
             }
         }
     }
     if (__any(n < 0))
         neighbour_sweep<ND, false, kCap, 64>(P, F.start, F.x, F.y, F.z, P.hs_fluid, n < 0, F,
-                                             Wv, lst, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                                             Wv, lst, xa, va, rho_a, p_a, inv_rho_a, acc, drho,
+                                             0.0, gap);
 
     double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
     if (__any(nw != 0)) {
@@ -1905,6 +1938,125 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// DensityDiffusionAntuono's update! (density_diffusion.jl:176-211), one lane per fluid
+// particle a in cell-sorted order, over its fluid neighbours b:
+//   M_a = -sum_b V_b grad W_ab (x) r_ab        (compute_gradient_correction_matrix!,
+//                                               corrections.jl:329-354)
+//   L_a = M_a^-1, or the identity where |det M_a| < 1f-9
+//                                              (correction_matrix_inversion_step!,
+//                                               corrections.jl:425-457)
+//   grad rho_a = -sum_b (rho_a - rho_b) V_b L_a grad W_ab = -L_a s_a,
+//   s_a = sum_b (rho_a - rho_b) V_b grad W_ab  (L_a is the same for every pair of a, so it
+//                                               multiplies the sum once)
+// with V_b = m_b / rho_b and grad W zero at distance >= R and at distance^2 < eps(h^2)
+// (smoothing_kernel_grad, abstract_system.jl:142-155).  M_a is symmetric: its upper
+// triangle is summed.  Four lanes share a particle and take every fourth candidate; their
+// partial sums are added with shuffles.  With Verlet lists (nbr != nullptr) group t takes
+// the particle ord[t] and its list; a particle whose list overflowed, and every particle of
+// the cell search, walks the cells over the SoA coordinates, which are current whenever
+// they are walked.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_antuono_gradient(DevParams P, FluidView F, const int *__restrict__ ord,
+                       const int *__restrict__ nbr, const int *__restrict__ cnt, int ls)
+{
+    constexpr int kLanes = 4;
+    const int q = threadIdx.x % kLanes;
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
+    const bool active = t < P.nf;
+    const int tc = active ? t : P.nf - 1;  // an in-bounds particle for the idle lanes
+    const int s = nbr ? ord[tc] : tc;
+    const double4 a = F.rec[2 * (size_t)s];
+    const double xa[3] = {a.x, a.y, a.z};
+    const double rho_a = a.w;
+    double m[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // M_00 M_01 M_02 M_11 M_12 M_22
+    double sv[3] = {0.0, 0.0, 0.0};
+    auto term = [&](int j, double p0, double p1, double p2, double d2, double rho_b) {
+        if (!(d2 < P.R2 && d2 >= P.eps_hh)) return;
+        const double r = sqrt_pos(d2);
+        const double w = fma(-0.5 * P.h_inv, r, 1.0);
+        const double kdr = P.kgrad_c * (w * w * w);
+        const double vk = F.vol[j].x * kdr;  // V_b |grad W| / r
+        const double f = (rho_a - rho_b) * vk;
+        m[0] -= vk * p0 * p0;
+        m[1] -= vk * p0 * p1;
+        m[3] -= vk * p1 * p1;
+        sv[0] += f * p0;
+        sv[1] += f * p1;
+        if constexpr (ND == 3) {
+            m[2] -= vk * p0 * p2;
+            m[4] -= vk * p1 * p2;
+            m[5] -= vk * p2 * p2;
+            sv[2] += f * p2;
+        }
+    };
+    const int n = !active ? 0 : (nbr ? cnt[t] : -1);
+    if (n >= 0) {
+        for (int k = q; k < n; k += kLanes) {
+            const int j = nbr[(size_t)k * ls + t];
+            const double4 b = F.rec[2 * (size_t)j];
+            const double p0 = xa[0] - b.x, p1 = xa[1] - b.y;
+            const double p2 = (ND == 3) ? xa[2] - b.z : 0.0;
+            term(j, p0, p1, p2, dist2<ND>(p0, p1, p2), b.w);
+        }
+    } else {
+        Pencils<ND> it;
+        it.init(P, xa[0], xa[1], xa[2], P.hs_fluid);
+        int jb, je;
+        while (it.next(P, F.start, jb, je)) {
+            for (int j = jb + q; j < je; j += kLanes) {
+                const double p0 = xa[0] - F.x[j], p1 = xa[1] - F.y[j];
+                const double p2 = (ND == 3) ? xa[2] - F.z[j] : 0.0;
+                const double d2 = dist2<ND>(p0, p1, p2);
+                if (d2 < P.R2) term(j, p0, p1, p2, d2, F.rec[2 * (size_t)j].w);
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < kLanes; o <<= 1) {
+#pragma unroll
+        for (int i = 0; i < 6; i++)
+            if (ND == 3 || i == 0 || i == 1 || i == 3) m[i] += __shfl_xor(m[i], o);
+#pragma unroll
+        for (int i = 0; i < ND; i++) sv[i] += __shfl_xor(sv[i], o);
+    }
+    if (!active || q != 0) return;
+    constexpr double kSingular = (double)1.0e-9f;  // the Float32 literal 1.0f-9
+    double g[3];
+    if constexpr (ND == 2) {
+        const double det = m[0] * m[3] - m[1] * m[1];
+        if (fabs(det) < kSingular) {
+            g[0] = -sv[0];
+            g[1] = -sv[1];
+        } else {
+            const double id = 1.0 / det;
+            g[0] = -(m[3] * sv[0] - m[1] * sv[1]) * id;
+            g[1] = -(m[0] * sv[1] - m[1] * sv[0]) * id;
+        }
+    } else {
+        // cofactors of the symmetric M (its adjugate is symmetric too)
+        const double c00 = m[3] * m[5] - m[4] * m[4];
+        const double c01 = m[2] * m[4] - m[1] * m[5];
+        const double c02 = m[1] * m[4] - m[2] * m[3];
+        const double c11 = m[0] * m[5] - m[2] * m[2];
+        const double c12 = m[1] * m[2] - m[0] * m[4];
+        const double c22 = m[0] * m[3] - m[1] * m[1];
+        const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+        if (fabs(det) < kSingular) {
+            g[0] = -sv[0];
+            g[1] = -sv[1];
+            g[2] = -sv[2];
+        } else {
+            const double id = 1.0 / det;
+            g[0] = -(c00 * sv[0] + c01 * sv[1] + c02 * sv[2]) * id;
+            g[1] = -(c01 * sv[0] + c11 * sv[1] + c12 * sv[2]) * id;
+            g[2] = -(c02 * sv[0] + c12 * sv[1] + c22 * sv[2]) * id;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < ND; d++) P.ddgrad[(size_t)s * ND + d] = g[d];
+}
+
 // Active tiles (at least one fluid particle), listed after every rebuild of the cell list.
 template <int ND>
 __global__ void k_tiles(const int *flag, DevParams P, const int *__restrict__ start, int tgx,
@@ -2191,6 +2343,7 @@ struct tpsph_handle {
     unsigned short *nbr16 = nullptr;
     double *dyn = nullptr;       // adaptive sound speed: {c, B, av_c, dd2}
     double *wvel = nullptr;      // no-slip wall velocities (3 per wall particle)
+    double *ddgrad = nullptr;    // DensityDiffusionAntuono: renormalised density gradients
     double *vmax_part = nullptr; // per-block maxima of v.v (kGateBlocks)
     // Verlet lists of the wall particles over the fluid (Adami pass), active wall particles
     int *anbr = nullptr, *acnt = nullptr, *wactive = nullptr;
@@ -2411,6 +2564,13 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
     }
     if (nf > 0) {
         TimedScope t(h, TPSPH_TIME_FLUID);
+        if (h->P.ddgrad) {
+            const bool lists = h->search == TPSPH_SEARCH_LISTS;
+            hipLaunchKernelGGL((k_antuono_gradient<ND>), dim3(nblocks(4 * (int64_t)nf)), dim3(kBlock), 0,
+                               h->stream, h->P, h->fluid_view(), lists ? h->ord : nullptr,
+                               lists ? h->nbr : nullptr, lists ? h->cnt : nullptr, h->ls);
+            TRY(launch_check(h, "k_antuono_gradient"));
+        }
         if (h->search == TPSPH_SEARCH_LISTS && h->bricks) {
             hipLaunchKernelGGL((k_fluid_brick<ND>), dim3(std::max(1, std::min(h->nbricks, h->n_cu))),
                 dim3(kBrickThreads), 0, h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
@@ -2496,6 +2656,12 @@ int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, co
         return bad("x_cell_divisions must be in 0..8");
     if (p->pair_search < TPSPH_SEARCH_AUTO || p->pair_search > TPSPH_SEARCH_TILES)
         return bad("pair_search must be one of enum tpsph_pair_search");
+    if (p->use_density_diffusion < TPSPH_DIFFUSION_NONE ||
+        p->use_density_diffusion > TPSPH_DIFFUSION_ANTUONO)
+        return bad("use_density_diffusion must be one of enum tpsph_density_diffusion");
+    if (p->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO && p->pair_search == TPSPH_SEARCH_TILES)
+        return bad("DensityDiffusionAntuono runs with the Verlet lists or the cell walk, not "
+                   "pair_search TPSPH_SEARCH_TILES");
     if (!(p->skin < 1.0)) return bad("skin must be below 1 (a fraction of the compact support)");
     if (nf > 0 && !fm) return bad("fluid_mass is NULL");
     if (nw > 0 && (!wx || !wm)) return bad("wall_coordinates / wall_mass is NULL");
@@ -2577,6 +2743,8 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     P.eps_h2 = params->epsilon * (hl * hl);
     P.av_c = params->alpha * P.c * hl;
     P.dd2 = params->use_density_diffusion ? 2.0 * params->delta * hl * P.c : 0.0;
+    P.ddgrad = nullptr;  // set below for DensityDiffusionAntuono
+    P.eps_hh = std::nextafter(hl * hl, INFINITY) - hl * hl;
     for (int d = 0; d < 3; d++) P.g[d] = d < nd ? params->acceleration[d] : 0.0;
     P.p_offset = params->pressure_offset;
     P.dyn = nullptr;  // set below for an adaptive state equation
@@ -2748,6 +2916,10 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     const int tiles = (int)((ncells + 1 + kScanTile - 1) / kScanTile);
     CTRY(dalloc(h, &h->flag, 1));
     if (params->wall_viscosity && nw > 0) CTRY(dalloc(h, &h->wvel, 3 * (size_t)nw));
+    if (params->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO && nf > 0) {
+        CTRY(dalloc(h, &h->ddgrad, (size_t)nd * nf));
+        P.ddgrad = h->ddgrad;
+    }
     if (params->adaptive_sound_speed) {
         CTRY(dalloc(h, &h->dyn, 4));
         CTRY(dalloc(h, &h->vmax_part, kGateBlocks));
@@ -2784,7 +2956,10 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         CTRY(dalloc(h, &h->cnt, nf));
         CTRY(dalloc(h, &h->ord, nf));
         CTRY(dalloc(h, &h->counters, 4));
-        h->bricks = P.maxr <= kBrickMaxRows;
+        // the brick kernel's LDS records carry no density gradients: DensityDiffusionAntuono
+        // takes the list kernel
+        h->bricks = P.maxr <= kBrickMaxRows &&
+                    params->use_density_diffusion != TPSPH_DIFFUSION_ANTUONO;
         if (h->bricks) {
             CTRY(dalloc(h, &h->brow_gb, (size_t)h->nbricks * P.maxr));
             CTRY(dalloc(h, &h->brow_lds, (size_t)h->nbricks * (P.maxr + 1)));
@@ -3185,7 +3360,7 @@ void tpsph_destroy(tpsph_handle *h)
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
                     h->tiles, h->ntiles, h->work, h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,
-                    h->wtargets, h->iperm, h->dyn, h->vmax_part, h->wvel,
+                    h->wtargets, h->iperm, h->dyn, h->vmax_part, h->wvel, h->ddgrad,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu};
     for (void *p : ptrs)

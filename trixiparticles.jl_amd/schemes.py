@@ -9,6 +9,7 @@ Reference constructors mirrored here:
 * ``StateEquationCole`` -- ``src/schemes/fluid/weakly_compressible_sph/state_equations.jl:100-139``;
 * ``ArtificialViscosityMonaghan`` -- ``src/schemes/fluid/viscosity.jl:68-132``;
 * ``DensityDiffusionMolteniColagrossi`` -- ``weakly_compressible_sph/density_diffusion.jl:41-55``;
+* ``DensityDiffusionAntuono`` -- ``weakly_compressible_sph/density_diffusion.jl:89-211``;
 * ``ContinuityDensity`` -- ``src/general/density_calculators.jl:188``;
 * ``WeaklyCompressibleSPHSystem`` -- ``weakly_compressible_sph/system.jl:65-174``;
 * ``AdamiPressureExtrapolation`` / ``BoundaryModelDummyParticles`` --
@@ -30,7 +31,8 @@ from .setups import InitialCondition
 
 __all__ = [
     "WendlandC2Kernel", "SchoenbergCubicSplineKernel", "StateEquationCole",
-    "ArtificialViscosityMonaghan", "DensityDiffusionMolteniColagrossi", "ContinuityDensity",
+    "ArtificialViscosityMonaghan", "DensityDiffusionMolteniColagrossi",
+    "DensityDiffusionAntuono", "ContinuityDensity",
     "WeaklyCompressibleSPHSystem", "AdamiPressureExtrapolation",
     "BoundaryModelDummyParticles", "WallBoundarySystem",
 ]
@@ -164,8 +166,33 @@ class ViscosityAdami:
 class DensityDiffusionMolteniColagrossi:
     """``DensityDiffusionMolteniColagrossi(; delta)`` (density_diffusion.jl:41-47)."""
 
+    kind = 1  # enum tpsph_density_diffusion
+
     def __init__(self, *, delta):
         self.delta = float(delta)
+
+
+class DensityDiffusionAntuono:
+    """``DensityDiffusionAntuono(; delta)`` (density_diffusion.jl:89-125): the
+    Molteni-Colagrossi term with the renormalised density gradients of both particles
+    subtracted, which every kick computes in a pass of its own (``update!``,
+    density_diffusion.jl:176-211).  It runs with the Verlet lists or the cell walk, not
+    with ``pair_search="tiles"``."""
+
+    kind = 2  # enum tpsph_density_diffusion
+
+    def __init__(self, *, delta):
+        self.delta = float(delta)
+
+    def __repr__(self):
+        # Base.show, density_diffusion.jl:143-149
+        return f"DensityDiffusionAntuono({_julia_float(self.delta)})"
+
+
+def _julia_float(x: float) -> str:
+    """Julia's shortest round-trip text of a Float64 (``repr(0.1) == "0.1"``)."""
+    r = repr(float(x))
+    return r if ("e" in r or "." in r or "n" in r) else r + ".0"
 
 
 class ContinuityDensity:
@@ -252,10 +279,11 @@ class WeaklyCompressibleSPHSystem:
         if viscosity is not None and not isinstance(viscosity, ArtificialViscosityMonaghan):
             raise NotImplementedError("the HIP right-hand side implements "
                                       "ArtificialViscosityMonaghan")
-        if density_diffusion is not None and \
-                not isinstance(density_diffusion, DensityDiffusionMolteniColagrossi):
+        if density_diffusion is not None and not isinstance(
+                density_diffusion, (DensityDiffusionMolteniColagrossi, DensityDiffusionAntuono)):
             raise NotImplementedError("the HIP right-hand side implements "
-                                      "DensityDiffusionMolteniColagrossi")
+                                      "DensityDiffusionMolteniColagrossi and "
+                                      "DensityDiffusionAntuono")
         for name, opt in (("correction", correction),
                           ("pressure_acceleration", pressure_acceleration),
                           ("surface_tension", surface_tension),

@@ -140,7 +140,7 @@ class Semidiscretization:
             p.use_viscosity = 1
             p.alpha, p.beta, p.epsilon = f.viscosity.alpha, f.viscosity.beta, f.viscosity.epsilon
         if f.density_diffusion is not None:
-            p.use_density_diffusion = 1
+            p.use_density_diffusion = f.density_diffusion.kind
             p.delta = f.density_diffusion.delta
         for d in range(f.ndims):
             p.acceleration[d] = f.acceleration[d]
