@@ -447,47 +447,75 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
     // with Verlet lists the SoA coordinates are read between rebuilds only by the cell walk
     // of overflowed lists (noverflow counts them; nullptr = always needed)
     const bool soa = !noverflow || *noverflow != 0;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < P.nf; s += gridDim.x * blockDim.x) {
-    const int i = perm[s];
-    double x[3] = {0.0, 0.0, 0.0}, vv[3] = {0.0, 0.0, 0.0};
+    // the loop bound is uniform per wavefront (blockDim is a multiple of 64): the brick
+    // counting below works on whole wavefronts
+    for (int s0 = blockIdx.x * blockDim.x; s0 < P.nf; s0 += gridDim.x * blockDim.x) {
+        const int s = s0 + (int)threadIdx.x;
+        const bool in = s < P.nf;
+        int b = -1;          // brick of the particle (MODE 1 with bcount)
+        bool moved = false;  // left its skin (MODE 0)
+        if (in) {
+            const int i = perm[s];
+            double x[3] = {0.0, 0.0, 0.0}, vv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int d = 0; d < ND; d++) {
-        x[d] = u[(size_t)i * ND + d];
-        vv[d] = v[(size_t)i * (ND + 1) + d];
-    }
-    const double rho = v[(size_t)i * (ND + 1) + ND];
-    const double mass = m[i];
-    if (MODE == 1 || soa) {
-        xs[s] = x[0];
-        ys[s] = x[1];
-        if constexpr (ND == 3) zs[s] = x[2];
-    }
-    rec[2 * (size_t)s] = make_double4(x[0], x[1], x[2], rho);
-    rec[2 * (size_t)s + 1] = make_double4(vv[0], vv[1], vv[2], state_equation(P, rho));
-    vol[s] = make_double2(mass / rho, mass);
-    if constexpr (MODE == 0) {
-        // Euclidean displacement since the build (bounds the per-axis drift the grown cells
-        // allow for, and the drift the Verlet lists allow for)
-        double e2 = 0.0;
+            for (int d = 0; d < ND; d++) {
+                x[d] = u[(size_t)i * ND + d];
+                vv[d] = v[(size_t)i * (ND + 1) + d];
+            }
+            const double rho = v[(size_t)i * (ND + 1) + ND];
+            const double mass = m[i];
+            if (MODE == 1 || soa) {
+                xs[s] = x[0];
+                ys[s] = x[1];
+                if constexpr (ND == 3) zs[s] = x[2];
+            }
+            rec[2 * (size_t)s] = make_double4(x[0], x[1], x[2], rho);
+            rec[2 * (size_t)s + 1] = make_double4(vv[0], vv[1], vv[2], state_equation(P, rho));
+            vol[s] = make_double2(mass / rho, mass);
+            if constexpr (MODE == 0) {
+                // Euclidean displacement since the build (bounds the per-axis drift the grown
+                // cells allow for, and the drift the Verlet lists allow for)
+                double e2 = 0.0;
 #pragma unroll
-        for (int d = 0; d < ND; d++) {
-            const double e = x[d] - snap[d * nf + s];
-            e2 = fma(e, e, e2);
+                for (int d = 0; d < ND; d++) {
+                    const double e = x[d] - snap[d * nf + s];
+                    e2 = fma(e, e, e2);
+                }
+                moved = !(e2 <= P.half_skin2);
+            } else {
+#pragma unroll
+                for (int d = 0; d < ND; d++) snap[d * nf + s] = x[d];
+                iperm[i] = s;
+                if (bcount) {
+                    const int c0 = cell_1d(x[0], P.lo[0], P.inv_cs[0], P.n[0]);
+                    const int c1 = cell_1d(x[1], P.lo[1], P.inv_cs[1], P.n[1]);
+                    const int c2 = (ND == 3) ? cell_1d(x[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+                    b = c0 / P.bc[0] + P.nb[0] * (c1 / P.bc[1] + P.nb[1] * (c2 / P.bc[2]));
+                }
+            }
         }
-        if (!(e2 <= P.half_skin2)) atomicOr(flag, 1);
-    } else {
-#pragma unroll
-        for (int d = 0; d < ND; d++) snap[d * nf + s] = x[d];
-        iperm[i] = s;
-        if (bcount) {
-            const int c0 = cell_1d(x[0], P.lo[0], P.inv_cs[0], P.n[0]);
-            const int c1 = cell_1d(x[1], P.lo[1], P.inv_cs[1], P.n[1]);
-            const int c2 = (ND == 3) ? cell_1d(x[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
-            const int b = c0 / P.bc[0] + P.nb[0] * (c1 / P.bc[1] + P.nb[1] * (c2 / P.bc[2]));
-            bid[s] = b;
-            rank[s] = atomicAdd(&bcount[b], 1);
+        // one atomic per wavefront that has a particle out of its skin
+        if (MODE == 0 && __any(moved) && ((int)threadIdx.x & 63) == 0) atomicOr(flag, 1);
+        if (MODE == 1 && bcount) {
+            // a wavefront holds consecutive cell-sorted particles, so equal bricks come in runs:
+            // the first lane of a run takes ranks for the whole run with one atomic, and lane l
+            // of the run gets base + l (a rank only decides which lane handles which particle)
+            const int lane = (int)threadIdx.x & 63;
+            const int prev = __shfl_up(b, 1);
+            const bool head = lane == 0 || b != prev;
+            const unsigned long long heads = __ballot(head);
+            const unsigned long long upto = (2ull << lane) - 1;  // lanes 0..lane (all for 63)
+            const int first = 63 - __clzll((long long)(heads & upto));
+            const unsigned long long after = heads & ~upto;
+            const int end = after ? __ffsll((long long)after) - 1 : 64;
+            int base = 0;
+            if (head && b >= 0) base = atomicAdd(&bcount[b], end - first);
+            base = __shfl(base, first);
+            if (b >= 0) {
+                bid[s] = b;
+                rank[s] = base + (lane - first);
+            }
         }
-    }
     }
     if (MODE == 1 && zero && blockIdx.x == 0 && (int)threadIdx.x < nzero) zero[threadIdx.x] = 0;
 }
@@ -958,11 +986,21 @@ __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &
     it.init(P, xa, ya, za, WALL ? P.hs_list_wall : P.hs_list_fluid);
     int n = 0, j, je;
     while (it.next(P, start, j, je)) {
-        for (; j < je; j++) {
-            const double d2 = dist2<ND>(xa - cx[j], ya - cy[j], (ND == 3) ? za - cz[j] : 0.0);
-            if (d2 <= rl2 && (WALL || j != s)) {
-                if (n < cap) nbr[(size_t)n * ls + col] = j;
-                n++;
+        // four candidates per step, their loads in flight together: the SoA arrays carry
+        // kPad entries past their end, so the loads stay in bounds, and j + k < je masks the
+        // candidates past the range (the list order is the candidates' order)
+        for (; j < je; j += 4) {
+            double d2[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                d2[k] = dist2<ND>(xa - cx[j + k], ya - cy[j + k],
+                                  (ND == 3) ? za - cz[j + k] : 0.0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (j + k < je && d2[k] <= rl2 && (WALL || j + k != s)) {
+                    if (n < cap) nbr[(size_t)n * ls + col] = j + k;
+                    n++;
+                }
             }
         }
     }
@@ -1615,25 +1653,36 @@ __device__ __forceinline__ void
         const int uy0 = by * P.bc[1] - P.ur[1], uz0 = (ND == 3) ? bz * P.bc[2] - P.ur[2] : 0;
         const int *gb = brow_gb + (size_t)b * P.maxr;
         const int *lds = brow_lds + (size_t)b * (P.maxr + 1);
-        for (int k = 0; ok && k < n; k++) {
-            const int j = nbr[(size_t)k * ls + t];
-            const int cy = cell_1d(F.y[j], P.lo[1], P.inv_cs[1], P.n[1]);
-            const int cz = (ND == 3) ? cell_1d(F.z[j], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
-            const int ry = cy - uy0, rz = cz - uz0;
-            const int r = ry + nuy * rz;
-            if (ry < 0 || ry >= nuy || rz < 0 || r >= P.maxr) {
-                ok = false;
-                break;
+        // one group of four slots per step (one 8-byte store); the slots past the end of the
+        // list point at the far record
+        ushort4 *l4 = reinterpret_cast<ushort4 *>(nbr16);
+        const int ng = (max(n, 0) + 3) >> 2;
+        for (int g = 0; ok && g < ng; g++) {
+            int j[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) j[k] = 4 * g + k < n ? nbr[(size_t)(4 * g + k) * ls + t] : -1;
+            unsigned short e16[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                e16[k] = (unsigned short)(kBrickCap - 1);
+                if (j[k] < 0) continue;
+                const int cy = cell_1d(F.y[j[k]], P.lo[1], P.inv_cs[1], P.n[1]);
+                const int cz = (ND == 3) ? cell_1d(F.z[j[k]], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+                const int ry = cy - uy0, rz = cz - uz0;
+                const int r = ry + nuy * rz;
+                if (ry < 0 || ry >= nuy || rz < 0 || r >= P.maxr) {
+                    ok = false;
+                    continue;
+                }
+                const int e = lds[r] + (j[k] - gb[r]);
+                if (j[k] < gb[r] || e >= lds[r + 1] || e >= kBrickCap - 1) {
+                    ok = false;
+                    continue;
+                }
+                e16[k] = (unsigned short)e;
             }
-            const int e = lds[r] + (j - gb[r]);
-            if (j < gb[r] || e >= lds[r + 1] || e >= kBrickCap - 1) {
-                ok = false;
-                break;
-            }
-            nbr16[((size_t)(k >> 2) * ls + t) * 4 + (k & 3)] = (unsigned short)e;
+            if (ok) l4[(size_t)g * ls + t] = make_ushort4(e16[0], e16[1], e16[2], e16[3]);
         }
-        for (int k = n; ok && (k & 3); k++)
-            nbr16[((size_t)(k >> 2) * ls + t) * 4 + (k & 3)] = (unsigned short)(kBrickCap - 1);
         if (!ok) bntot[b] = INT_MAX;
     }
 }
@@ -2130,32 +2179,36 @@ __global__ void __launch_bounds__(kBlock)
     __syncthreads();  // the block's new u and v are complete
     if (work && blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
     const int i = i0 + (int)threadIdx.x;
-    if (i >= nf) return;
-    const size_t nfs = (size_t)nf;
-    const int s = iperm[i];
-    double x[3] = {0.0, 0.0, 0.0}, vv[3] = {0.0, 0.0, 0.0};
+    bool moved = false;  // left its skin
+    if (i < nf) {
+        const size_t nfs = (size_t)nf;
+        const int s = iperm[i];
+        double x[3] = {0.0, 0.0, 0.0}, vv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int d = 0; d < ND; d++) {
-        x[d] = u[(size_t)i * ND + d];
-        vv[d] = v[(size_t)i * (ND + 1) + d];
-    }
-    const double rho = v[(size_t)i * (ND + 1) + ND];
-    const double mass = m[i];
-    if (!noverflow || *noverflow != 0) {  // see k_gather
-        xs[s] = x[0];
-        ys[s] = x[1];
-        if constexpr (ND == 3) zs[s] = x[2];
-    }
-    rec[2 * (size_t)s] = make_double4(x[0], x[1], x[2], rho);
-    rec[2 * (size_t)s + 1] = make_double4(vv[0], vv[1], vv[2], state_equation(P, rho));
-    vol[s] = make_double2(mass / rho, mass);
-    double e2 = 0.0;
+        for (int d = 0; d < ND; d++) {
+            x[d] = u[(size_t)i * ND + d];
+            vv[d] = v[(size_t)i * (ND + 1) + d];
+        }
+        const double rho = v[(size_t)i * (ND + 1) + ND];
+        const double mass = m[i];
+        if (!noverflow || *noverflow != 0) {  // see k_gather
+            xs[s] = x[0];
+            ys[s] = x[1];
+            if constexpr (ND == 3) zs[s] = x[2];
+        }
+        rec[2 * (size_t)s] = make_double4(x[0], x[1], x[2], rho);
+        rec[2 * (size_t)s + 1] = make_double4(vv[0], vv[1], vv[2], state_equation(P, rho));
+        vol[s] = make_double2(mass / rho, mass);
+        double e2 = 0.0;
 #pragma unroll
-    for (int d = 0; d < ND; d++) {
-        const double e = x[d] - snap[d * nfs + s];
-        e2 = fma(e, e, e2);
+        for (int d = 0; d < ND; d++) {
+            const double e = x[d] - snap[d * nfs + s];
+            e2 = fma(e, e, e2);
+        }
+        moved = !(e2 <= P.half_skin2);
     }
-    if (!(e2 <= P.half_skin2)) atomicOr(flag, 1);
+    // one atomic per wavefront that has a particle out of its skin
+    if (__any(moved) && ((int)threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
 
 // ------------------------------------------------------------------------------------------
