@@ -63,6 +63,8 @@ def _create(tp, **overrides):
     ("pair_search", -1, "pair_search"),
     ("skin", 1.0, "skin"),
     ("cell_divisions", 5, "cell_divisions"),
+    ("use_density_diffusion", 3, "use_density_diffusion"),
+    ("use_density_diffusion", -1, "use_density_diffusion"),
 ])
 def test_create_rejects_invalid_parameters(tp, field, value, msg):
     rc, err = _create(tp, **{field: value})
