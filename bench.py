@@ -4,6 +4,7 @@ One step = one CarpenterKennedy2N54 step (5 x kick! + drift! + stage update), ru
 on the device (tpsph_solve_ck2n54) with the state resident in HBM.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--dx 6.25e-3] [--cpu-steps S]
+  python bench.py --rebuild-kicks N    # cost of a rebuild (separate JSON line, no bench)
 
 Roofline of the dominant kernel (the fused fluid interaction): its algorithmic HBM bytes per
 launch are the compulsory reads and writes of one evaluation,
@@ -46,10 +47,41 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="CK2N54 steps of the oracle in the CPU-baseline sample (0: skip)")
     ap.add_argument("--pair-search", default="auto", choices=["auto", "lists", "cells", "tiles"])
+    ap.add_argument("--rebuild-kicks", type=int, default=0, metavar="N",
+                    help="instead of the bench, time N kicks that each rebuild the cell list "
+                         "and the Verlet lists")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the run, copy MIB MiB device to device once (a plain copy "
                          "of known size that calibrates FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def rebuild_kicks(tp, lib, semi, v, u, args, nf, nw):
+    """Kicks that all rebuild: the positions alternate between u and u shifted by 0.06 dx
+    along x, beyond half the default skin, so every kick rebuilds the cell-sorted order, the
+    Verlet lists and the brick tables.  Prints one JSON line."""
+    import ctypes as C
+    import torch
+    u2 = u.clone()
+    u2.view(-1, 3)[:, 0] += 0.06 * args.dx
+    dv = torch.empty_like(v)
+    p = {"semi": semi}
+    for k in range(4):
+        tp.kick(dv, v, u if k % 2 == 0 else u2, p, 0.0)
+    torch.cuda.synchronize()
+    lib.tpsph_enable_timing(semi.handle, 1)
+    t0 = time.perf_counter()
+    for k in range(args.rebuild_kicks):
+        tp.kick(dv, v, u if k % 2 == 0 else u2, p, 0.0)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n, ms = C.c_int64(), C.c_double()
+    lib.tpsph_kernel_time(semi.handle, 0, C.byref(n), C.byref(ms))  # TPSPH_TIME_CELL_LIST
+    print(json.dumps({
+        "rebuilding_kicks": args.rebuild_kicks,
+        "ms_per_kick": elapsed * 1e3 / args.rebuild_kicks,
+        "rebuild_chain_ms": ms.value / max(n.value, 1),
+        "n_fluid": nf, "n_wall": nw}), flush=True)
 
 
 def main():
@@ -74,6 +106,9 @@ def main():
     v, u = ode.v0.clone(), ode.u0.clone()
     h = semi.handle
     nf, nw = db.n_fluid, db.n_wall
+    if args.rebuild_kicks > 0:
+        rebuild_kicks(tp, lib, semi, v, u, args, nf, nw)
+        return
 
     def run(nsteps):
         native.check(lib.tpsph_solve_ck2n54(h, v.data_ptr(), u.data_ptr(), 0.0, args.dt,
