@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--dt", type=float, default=1e-5)
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="CK2N54 steps of the oracle in the CPU-baseline sample (0: skip)")
-    ap.add_argument("--pair-search", default="auto", choices=["auto", "lists", "cells", "tiles"])
+    ap.add_argument("--pair-search", default="auto", choices=["auto", "lists", "cells"])
     ap.add_argument("--rebuild-kicks", type=int, default=0, metavar="N",
                     help="instead of the bench, time N kicks that each rebuild the cell list "
                          "and the Verlet lists")
@@ -144,7 +144,7 @@ def main():
     if search == "auto":
         search = "lists" if semi.skin != 0.0 else "cells"
     # with Verlet lists and the default cell divisions the fluid kernel stages bricks in LDS
-    fluid_kernel = {"lists": "k_fluid_brick", "cells": "k_fluid", "tiles": "k_fluid_tiled"}[search]
+    fluid_kernel = {"lists": "k_fluid_brick", "cells": "k_fluid"}[search]
     nd = 3
     fluid_bytes = nf * (8 * nd + 8 * nd + 8 + 8 + 8 + 8 * (nd + 1))
     wall_bytes = nw * (8 * nd + 8 + 8 + 8)

@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define TPSPH_ABI_VERSION 2
+#define TPSPH_ABI_VERSION 3
 
 enum tpsph_status {
     TPSPH_OK = 0,
@@ -66,15 +66,14 @@ enum tpsph_pair {
 enum tpsph_pair_search {
     TPSPH_SEARCH_AUTO = 0,   /* Verlet lists when skin > 0, else the cell walk           */
     TPSPH_SEARCH_LISTS = 1,  /* Verlet lists rebuilt with the cell list                  */
-    TPSPH_SEARCH_CELLS = 2,  /* cell walk on every kick                                  */
-    TPSPH_SEARCH_TILES = 3   /* cell walk with LDS-staged tiles (default cell divisions) */
+    TPSPH_SEARCH_CELLS = 2   /* cell walk on every kick                                  */
 };
 
 /* Density diffusion of the fluid (tpsph_params.use_density_diffusion). */
 enum tpsph_density_diffusion {
     TPSPH_DIFFUSION_NONE = 0,      /* nothing                                              */
     TPSPH_DIFFUSION_MOLTENI = 1,   /* DensityDiffusionMolteniColagrossi(; delta)           */
-    TPSPH_DIFFUSION_ANTUONO = 2    /* DensityDiffusionAntuono(; delta), not with TILES     */
+    TPSPH_DIFFUSION_ANTUONO = 2    /* DensityDiffusionAntuono(; delta)                     */
 };
 
 /* Kernel identifiers for tpsph_kernel_time. */
@@ -103,8 +102,7 @@ enum tpsph_timed_kernel {
  *                                      (density_diffusion.jl:89-211), whose renormalised
  *                                      density gradients every kick computes in a pass of its
  *                                      own; Antuono runs with the Verlet lists (without the
- *                                      brick kernel) or the cell walk, not with
- *                                      TPSPH_SEARCH_TILES
+ *                                      brick kernel) or the cell walk
  *   acceleration                       `acceleration` of the fluid system (gravity)
  *   clip_wall_pressure                 BoundaryModelDummyParticles(; clip_negative_pressure)
  *   pressure_offset                    AdamiPressureExtrapolation(; pressure_offset)

@@ -21,7 +21,7 @@ def test_library_exports_every_declared_symbol(tp):
     for s in syms:
         assert hasattr(lib, s), f"{s} declared in include/tpsph.h but not exported"
     assert set(tp.native.EXPORTED_SYMBOLS) == set(syms)
-    assert lib.tpsph_abi_version() == 2
+    assert lib.tpsph_abi_version() == 3
 
 
 def test_params_struct_layout_matches_header(tp):

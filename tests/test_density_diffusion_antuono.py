@@ -107,11 +107,12 @@ def test_singular_correction_matrix_falls_back_to_identity(orc):
 def test_antuono_options_are_validated(tp):
     db = tp.dam_break(2, 0.05, density_diffusion=tp.DensityDiffusionAntuono(delta=0.1))
     semi = tp.Semidiscretization(db.fluid_system, db.boundary_system,
-                                 parallelization_backend="hip_host", pair_search="tiles")
+                                 parallelization_backend="hip_host", pair_search="lists")
     p = semi.params()
     assert p.use_density_diffusion == 2 and p.delta == 0.1
-    with pytest.raises(Exception, match="DensityDiffusionAntuono"):
-        semi.handle
+    with pytest.raises(ValueError, match="pair_search"):
+        tp.Semidiscretization(db.fluid_system, db.boundary_system,
+                              parallelization_backend="hip_host", pair_search="tiles")
 
 
 def rel_err(got, ref):

@@ -93,10 +93,10 @@ def test_kick_matches_oracle(tp, orc, nd, dx, grid):
     assert rel_err(rw, rw_ref) <= RTOL
 
 
-@pytest.mark.parametrize("search", ["lists", "cells", "tiles"])
+@pytest.mark.parametrize("search", ["lists", "cells"])
 @pytest.mark.parametrize("nd,dx", [(2, 0.01), (3, 0.02)])
 def test_kick_matches_oracle_for_every_pair_search(tp, orc, nd, dx, search):
-    """Verlet lists, the per-kick cell walk and the LDS-tiled cell walk give the oracle's
+    """The Verlet lists (brick kernel) and the per-kick cell walk give the oracle's
     right-hand side."""
     db, semi = setup_case(tp, nd, dx, grid={"pair_search": search})
     v0, u0 = semi.initial_state()

@@ -12,6 +12,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libtpsph.so")
 SOURCES = [os.path.join(PKG_DIR, "csrc", "tpsph.hip")]
+HEADERS = [os.path.join(PKG_DIR, "csrc", "tpsph_internal.h")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -11,8 +11,9 @@
 //      Verlet lists, k_lists (fluid and wall candidates within R + skin);
 //   3. k_wall        Adami pressure extrapolation + inverse state equation per wall particle;
 //   4. the fused fluid-fluid + fluid-wall interaction + gravity, results scattered back to
-//      the caller's particle order: k_fluid_list (Verlet lists), k_fluid (cell walk) or
-//      k_fluid_tiled (cell walk over LDS-staged tiles).
+//      the caller's particle order: k_fluid_brick + k_fluid_walls (Verlet lists with the
+//      candidates staged brick by brick in LDS), k_fluid_list (Verlet lists) or k_fluid
+//      (cell walk).
 // The wall cell list is static and built once at create.
 //
 // Neighbour search.  Cells have edge R/k (k = cell_divisions, R = compact support 2h).  For
@@ -31,249 +32,20 @@
 // has exhausted its pencils, so any neighbour count is handled.  k_fluid_list streams the
 // Verlet lists instead (about 1.1x the neighbour count per particle, read as contiguous
 // 256-byte rows per wavefront) and applies the same exact test to every entry.
-#include <hip/hip_runtime.h>
+#include "tpsph_internal.h"
 
-#include <algorithm>
-#include <cfloat>
-#include <climits>
-#include <cmath>
-#include <cstdint>
-#include <cstring>
-#include <new>
-#include <string>
-#include <vector>
+namespace tpsph {
+thread_local std::string g_create_error;
+}  // namespace tpsph
 
-#include "tpsph.h"
+using namespace tpsph;
 
 namespace {
 
-constexpr int kBlock = 256;  // 4 wavefronts of 64
-constexpr int kCap = 32;     // neighbour-list entries per lane and phase
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kBlock * kScanItems;
-constexpr int kBrickThreads = 1024;   // threads of the brick kernel (one block per CU), two
-                                      // per target
-constexpr int kBrickCap = 2048;       // LDS records of a brick: the union plus one far record
-constexpr double kFar = 1e100;        // coordinate of the far record (never within R)
-constexpr int kBrickMaxRows = 256;    // union rows per brick (one scan block)
-constexpr int kGateBlocks = 1024;     // grid-stride kernels behind the rebuild flag: a small
-                                      // grid keeps the launch cheap when the flag is down
-constexpr int kPad = 4;               // NaN entries after the SoA coordinate arrays
-
-thread_local std::string g_create_error;
-
-struct DevParams {
-    int nd;
-    int nf, nw;
-    int use_visc, use_dd, clip_fluid, clip_wall;
-    double h, h_inv, R, R2, two_h, az2;
-    double kval_c;   // kernel normalisation sigma * h^-d
-    double kgrad_c;  // -5 sigma h^-d h^-2: grad W = kgrad_c (1 - q/2)^3 r_ab
-    double rho0, gamma, inv_gamma, p_bg, B, c;
-    double alpha, beta, eps_h2;  // epsilon h^2
-    double av_c;                 // alpha c h
-    double dd2;                  // 2 delta h c (0 without density diffusion)
-    double g[3];
-    double p_offset;
-    // grid
-    double lo[3];
-    double cs[3], inv_cs[3];  // cell edge per axis
-    int n[3];
-    int ncells;
-    double hs_fluid;  // half skin + rounding margin (fluid cells)
-    double hs_wall;   // rounding margin (static wall cells)
-    double half_skin, half_skin2;
-    // Verlet lists: squared list radii and the cell growth that makes the pencils reach them
-    double rl2_fluid, rl2_wall, hs_list_fluid, hs_list_wall;
-    // bricks of cells (processing order of the list kernels): cells per brick, bricks per axis
-    int bc[3], nb[3];
-    // union rows of a brick: cells beyond the brick along y and z, rows per brick, and the
-    // radius (list radius + margin) and cell growth (margin) of the union
-    int ur[3], maxr;
-    double rl_union2, hs_union;
-    // StateEquationAdaptiveCole: {c, B, av_c, dd2} of the current kick in device memory
-    // (k_sound_speed writes them), or nullptr for the constants above
-    const double *dyn;
-    double mach, c_min, c_max;
-    // viscosity of the wall model (no-slip): 0 nothing, 1 ArtificialViscosityMonaghan,
-    // 2 ViscosityAdami; its alpha c h, beta, epsilon h^2 and nu, and the fluid's own
-    // kinematic viscosity alpha h c / (2 ndims + 4) (viscosity.jl:82-87)
-    int visc_w;
-    double w_alpha, w_ac, w_beta, w_eps_h2, w_nu, nu_kin;  // w_ac = w_alpha c
-    // DensityDiffusionAntuono: the renormalised density gradient of every fluid particle
-    // (cell-sorted, ND per particle, written by k_antuono_gradient each kick) or nullptr
-    // for DensityDiffusionMolteniColagrossi; eps(h^2), the zero-distance bound of the
-    // gradient-correction pass (smoothing_kernel_grad, abstract_system.jl:142-155)
-    double *ddgrad;
-    double eps_hh;
-};
-
-// The sound-speed dependent constants of this kick: P's own, or the adaptive ones.  Kernels
-// that evaluate the state equation or the pair terms start with P = with_sound(P).
-__device__ __forceinline__ DevParams with_sound(DevParams P)
-{
-    if (P.dyn) {
-        P.c = P.dyn[0];
-        P.B = P.dyn[1];
-        P.av_c = P.dyn[2];
-        P.dd2 = P.dyn[3];
-        P.w_ac = P.w_alpha * P.c;
-        P.nu_kin = P.alpha * P.h * P.c / (2 * P.nd + 4);
-    }
-    return P;
-}
-
-// Cell-sorted fluid data.  rec[2s], rec[2s+1] = {x, y, z, rho}, {vx, vy, vz, p} (one 64-byte
-// line per particle, read by the pair phase), vol[s] = {m/rho, m}; x/y/z SoA for the radius
-// test.
-struct FluidView {
-    const int *start;
-    const int *perm;
-    const double *x, *y, *z;
-    const double4 *rec;
-    const double2 *vol;
-};
-
-struct WallView {
-    const int *start;
-    const double *x, *y, *z;
-    const double4 *pos;  // {x, y, z, m}
-    const double2 *pv;   // {p_w, m/rho_w} of the current kick
-    const double *vel;   // wall velocity for the viscosity (3 per particle), or nullptr
-};
-
-// ------------------------------------------------------------------------------------------
-// device helpers
-// ------------------------------------------------------------------------------------------
-// Julia's max(x, 0) for Float64: NaN propagates, -0.0 becomes +0.0.
-__device__ __forceinline__ double julia_max0(double x) { return (x != x) ? x : (x > 0.0 ? x : 0.0); }
-
-template <int ND>
-__device__ __forceinline__ double dist2(double a, double b, double c)
-{
-#pragma clang fp contract(off)
-    double s = a * a + b * b;
-    if constexpr (ND == 3) s = s + c * c;
-    return s;
-}
-
-__device__ __forceinline__ int cell_1d(double x, double lo, double inv, int n)
-{
-    double f = floor((x - lo) * inv);
-    f = fmin(fmax(f, 0.0), (double)(n - 1));  // also maps NaN into the grid
-    return (int)f;
-}
-
-__device__ __forceinline__ double state_equation(const DevParams &P, double rho)
-{
-    double p = P.B * (pow(rho / P.rho0, P.gamma) - 1.0) + P.p_bg;
-    return P.clip_fluid ? julia_max0(p) : p;
-}
-
-// WendlandC2 kernel value with the r < 2h guard (smoothing_kernels.jl:30-34, 436-441).
-__device__ __forceinline__ double wendland_c2(const DevParams &P, double r)
-{
-    double q = r * P.h_inv;
-    double w = 1.0 - q * 0.5;
-    double w2 = w * w;
-    return r < P.two_h ? P.kval_c * (w2 * w2) * (2.0 * q + 1.0) : 0.0;
-}
-
-// Distance from coordinate y to the interval of cell c along axis d, grown by hs; border
-// cells reach to infinity outwards.
-__device__ __forceinline__ double transverse(const DevParams &P, int d, int c, double y, double hs)
-{
-    double lower = (c == 0) ? -INFINITY : P.lo[d] + c * P.cs[d] - hs;
-    double upper = (c == P.n[d] - 1) ? INFINITY : P.lo[d] + (c + 1) * P.cs[d] + hs;
-    return fmax(0.0, fmax(lower - y, y - upper));
-}
-
-// Byte-offset loads: base pointer in SGPRs, 32-bit offset in a VGPR (global_load saddr form).
-template <typename T>
-__device__ __forceinline__ T ldo(const T *__restrict__ base, uint32_t byte_off)
-{
-    return *(const T *)((const char *)base + byte_off);
-}
-
-// Enumerates the candidate ranges (pencils) around one point, z-outer, y-inner.
-template <int ND>
-struct Pencils {
-    double xa, ya, za, hs, dz2;
-    int y0, y1, z1, iy, iz, rowz;
-
-    __device__ __forceinline__ void init(const DevParams &P, double x, double y, double z,
-                                         double hs_)
-    {
-        xa = x;
-        ya = y;
-        za = z;
-        hs = hs_;
-        dz2 = 0.0;
-        rowz = 0;
-        const double reach = P.R + hs;
-        y0 = cell_1d(ya - reach, P.lo[1], P.inv_cs[1], P.n[1]);
-        y1 = cell_1d(ya + reach, P.lo[1], P.inv_cs[1], P.n[1]);
-        if constexpr (ND == 3) {
-            iz = cell_1d(za - reach, P.lo[2], P.inv_cs[2], P.n[2]) - 1;
-            z1 = cell_1d(za + reach, P.lo[2], P.inv_cs[2], P.n[2]);
-        } else {
-            iz = -1;
-            z1 = 0;
-        }
-        iy = y1 + 1;  // the first call advances to the first z plane
-    }
-
-    // next non-empty candidate range [jb, je); false when exhausted
-    __device__ __forceinline__ bool next(const DevParams &P, const int *__restrict__ start,
-                                         int &jb, int &je)
-    {
-        while (true) {
-            if (iy > y1) {
-                if (++iz > z1) return false;
-                iy = y0;
-                if constexpr (ND == 3) {
-                    const double dz = transverse(P, 2, iz, za, hs);
-                    dz2 = dz * dz;
-                }
-                rowz = P.n[0] * P.n[1] * iz;
-            }
-            const int cy = iy++;
-            const double dy = transverse(P, 1, cy, ya, hs);
-            const double r2 = P.R2 - fma(dy, dy, dz2);
-            if (r2 < 0.0) continue;
-            // conservative half-width: FP32 sqrt, enlarged well past its rounding error
-            const double rx = (double)(__builtin_amdgcn_sqrtf((float)r2) * 1.000001f) + hs;
-            const int x0 = cell_1d(xa - rx, P.lo[0], P.inv_cs[0], P.n[0]);
-            const int x1 = cell_1d(xa + rx, P.lo[0], P.inv_cs[0], P.n[0]);
-            const uint32_t row = (uint32_t)(rowz + P.n[0] * cy);
-            jb = ldo(start, (row + (uint32_t)x0) * 4u);
-            je = ldo(start, (row + (uint32_t)x1 + 1u) * 4u);
-            if (jb < je) return true;
-        }
-    }
-};
-
-// Map block b to a tile so that each XCD (blocks are dealt round-robin over 8 XCDs) works
-// on one contiguous part of the cell-sorted particles and its L2 keeps the neighbours.
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-    constexpr int kXcd = 8;
-    const int per = nb / kXcd, extra = nb % kXcd;
-    const int x = b % kXcd, k = b / kXcd;
-    return (x < extra) ? x * (per + 1) + k : extra * (per + 1) + (x - extra) * per + k;
-}
 
 // ------------------------------------------------------------------------------------------
 // cell list (counting sort), every kernel gated by the rebuild flag (nullptr = always)
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ bool skip(const int *flag) { return flag && *flag == 0; }
-
-__global__ void k_fill_i32(const int *flag, int *p, int value, int n)
-{
-    if (skip(flag)) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        p[i] = value;
-}
 
 template <int ND>
 __global__ void k_cells(const int *flag, DevParams P, int n, const double *__restrict__ x,
@@ -520,59 +292,6 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
     if (MODE == 1 && zero && blockIdx.x == 0 && (int)threadIdx.x < nzero) zero[threadIdx.x] = 0;
 }
 
-// ------------------------------------------------------------------------------------------
-// Adami pressure extrapolation (dummy_particles.jl:489-569): one thread per wall particle in
-// wall-cell order over its fluid neighbours within 2h, then the inverse state equation.
-// ------------------------------------------------------------------------------------------
-// Contribution of fluid particle j (sorted index) to the sums of the wall particle at xw.
-// With a wall viscosity the kernel-weighted fluid velocity is summed into vs
-// (interpolate_fluid_velocity!, dummy_particles.jl:716-731).
-template <int ND>
-__device__ __forceinline__ void adami_term(const DevParams &P, const FluidView &F,
-                                           const double4 &xw, int j, double &pressure,
-                                           double &volume, double *vs)
-{
-    const double4 a = F.rec[2 * (size_t)j];
-    const double p0 = xw.x - a.x, p1 = xw.y - a.y;
-    const double p2 = (ND == 3) ? xw.z - a.z : 0.0;
-    const double d2 = dist2<ND>(p0, p1, p2);
-    if (d2 > P.R2) return;
-    const double W = wendland_c2(P, sqrt(d2));
-    const double rho_f = a.w;
-    const double4 b = F.rec[2 * (size_t)j + 1];
-    const double p_f = b.w;
-    double hydro = P.g[0] * (rho_f * p0) + P.g[1] * (rho_f * p1);
-    if constexpr (ND == 3) hydro += P.g[2] * (rho_f * p2);
-    pressure += ((P.p_offset + p_f) + hydro) * W;
-    volume += W;
-    if (P.visc_w) {
-        vs[0] += W * b.x;
-        vs[1] += W * b.y;
-        if constexpr (ND == 3) vs[2] += W * b.z;
-    }
-}
-
-// p_w = P_w / V_w (if V_w > eps), optional clipping, the inverse state equation; with a wall
-// viscosity the no-slip wall velocity 2 v_wall - vs / V_w with v_wall = 0 (if V_w > eps,
-// else vs as summed; compute_wall_velocity!, dummy_particles.jl:738-756).
-__device__ __forceinline__ void adami_finish(const DevParams &P, int w, double mass,
-                                             double pressure, double volume,
-                                             double *__restrict__ pw, double *__restrict__ rhow,
-                                             double2 *__restrict__ wpv, const double *vs,
-                                             double *__restrict__ vel)
-{
-    if (P.visc_w) {
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-            vel[3 * (size_t)w + d] = (volume > DBL_EPSILON) ? 0.0 - vs[d] / volume : vs[d];
-    }
-    if (volume > DBL_EPSILON) pressure /= volume;
-    if (P.clip_wall) pressure = julia_max0(pressure);
-    const double rho = P.rho0 * pow((pressure - P.p_bg) / P.B + 1.0, P.inv_gamma);
-    pw[w] = pressure;
-    rhow[w] = rho;
-    wpv[w] = make_double2(pressure, mass / rho);
-}
 
 // alist / acnt: the wall particles' Verlet lists of fluid candidates (slot-major, row
 // length lw), or nullptr for the cell walk (every wall particle).  With lists only the
@@ -620,202 +339,6 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// ------------------------------------------------------------------------------------------
-// Fused WCSPH interaction (wcsph/rhs.jl:5-127) for fluid-fluid and fluid-wall neighbours plus
-// gravity (semidiscretization.jl:720-728), one lane per fluid particle in cell-sorted order.
-//
-// With grad W = kdr r_ab, kdr = -5 sigma h^-2 (1 - q/2)^3, V_b = m_b / rho_b:
-//   pressure   -m_b (p_a + p_b)/(rho_a rho_b) grad W = -V_b (p_a + p_b)/rho_a grad W
-//   viscosity  (m_b alpha c mu + m_b beta mu^2)/rho_mean grad W  when v_ab.r_ab < 0,
-//              mu = h v_ab.r_ab / (r^2 + eps h^2)                     (fluid neighbours)
-//   continuity rho_a/rho_b m_b v_ab.grad W = rho_a V_b kdr v_ab.r_ab
-//   diffusion  delta h c V_b psi.grad W = 2 delta h c V_b (rho_a - rho_b) kdr
-//              (psi.grad W = 2 (rho_a - rho_b) kdr r_ab.r_ab / r^2)   (fluid neighbours)
-// Wall neighbours: v_b = 0, p_b and rho_b from the Adami pass, no viscosity (the wall model's
-// viscosity is `nothing`) and no density diffusion.  Per particle the fluid sum, the wall sum
-// and gravity are added in this order, as the reference adds them into dv.
-//
-// sqrt and the viscosity division use the hardware rsq / rcp refined by Goldschmidt and
-// Newton-Raphson steps (the refinement LLVM emits for IEEE f64 sqrt and division, without
-// the range scaling: arguments here are normal and positive); results agree with the
-// correctly rounded operations to an ulp or so.
-// ------------------------------------------------------------------------------------------
-// sqrt(x) and 1/x for normal positive x from the hardware approximations v_rsq_f64 and
-// v_rcp_f64 (relative error about 5e-8, measured) refined by one step of Goldschmidt's
-// square-root iteration and of Newton's reciprocal iteration: each step squares the
-// relative error, so the results are within about 5e-15 (measured 4.2e-15 and 2.2e-15 over
-// 2^20 inputs from 2^-41 to 1), far inside the 1e-12 parity bar of the pair terms.
-__device__ __forceinline__ double sqrt_pos(double x)
-{
-    const double y = __builtin_amdgcn_rsq(x);
-    const double g = x * y, hh = 0.5 * y;
-    return fma(g, fma(-hh, g, 0.5), g);
-}
-
-__device__ __forceinline__ double rcp_pos(double x)
-{
-    const double r = __builtin_amdgcn_rcp(x);
-    return fma(r, fma(-x, r, 1.0), r);
-}
-
-struct FluidNbr {
-    double4 a, b;  // {x, y, z, rho}, {vx, vy, vz, p}
-    double2 v;     // {V, m}
-    int j;         // cell-sorted index (read only by the DensityDiffusionAntuono term)
-};
-
-struct WallNbr {
-    double4 a;  // {x, y, z, m}
-    double2 pv; // {p, V}
-    double v[3];  // velocity for the viscosity (only with a wall viscosity)
-};
-
-__device__ __forceinline__ FluidNbr load_fluid(const FluidView &F, int j)
-{
-    const uint32_t o = (uint32_t)j;
-    return {ldo(F.rec, o * 64u), ldo(F.rec, o * 64u + 32u), ldo(F.vol, o * 16u), j};
-}
-
-__device__ __forceinline__ WallNbr load_wall(const WallView &Wv, int j)
-{
-    const uint32_t o = (uint32_t)j;
-    WallNbr n{ldo(Wv.pos, o * 32u), ldo(Wv.pv, o * 16u), {0.0, 0.0, 0.0}};
-    if (Wv.vel) {
-#pragma unroll
-        for (int d = 0; d < 3; d++) n.v[d] = Wv.vel[3 * (size_t)j + d];
-    }
-    return n;
-}
-
-// ga: the target's renormalised density gradient under DensityDiffusionAntuono, whose
-// psi_ab = 2 (rho_a - rho_b - (grad rho_a + grad rho_b).r_ab / 2) r_ab / r^2
-// (density_diffusion.jl:155-174) replaces the Molteni-Colagrossi 2 (rho_a - rho_b);
-// nullptr for Molteni-Colagrossi (or no density diffusion).
-template <int ND>
-__device__ __forceinline__ void pair_fluid(const DevParams &P, const FluidNbr &n,
-                                           const double *xa, const double *va, double rho_a,
-                                           double p_a, double inv_rho_a, double *acc,
-                                           double &drho, const double *ga = nullptr)
-{
-    const double p0 = xa[0] - n.a.x, p1 = xa[1] - n.a.y, p2 = (ND == 3) ? xa[2] - n.a.z : 0.0;
-    const double d2 = dist2<ND>(p0, p1, p2);
-    const double r = sqrt_pos(d2);
-    const double w = fma(-0.5 * P.h_inv, r, 1.0);
-    const double kdr = P.kgrad_c * (w * w * w);
-    const double v0 = va[0] - n.b.x, v1 = va[1] - n.b.y, v2 = (ND == 3) ? va[2] - n.b.z : 0.0;
-    double vr = v0 * p0 + v1 * p1;
-    if constexpr (ND == 3) vr += v2 * p2;
-    const double rho_b = n.a.w;
-    double fac = -n.v.x * inv_rho_a * (p_a + n.b.w);
-    if (P.use_visc && vr < 0.0) {
-        // (m_b alpha c mu + m_b beta mu^2) / rho_mean, rho_mean = (rho_a + rho_b) / 2
-        const double q = d2 + P.eps_h2;
-        const double sum_rho = rho_a + rho_b;
-        fac = fma(2.0 * P.av_c * n.v.y * vr, rcp_pos(q * sum_rho), fac);
-        if (P.beta != 0.0) {
-            const double mu = P.h * vr * rcp_pos(q);
-            fac += 2.0 * n.v.y * P.beta * (mu * mu) * rcp_pos(sum_rho);
-        }
-    }
-    fac *= kdr;
-    acc[0] = fma(fac, p0, acc[0]);
-    acc[1] = fma(fac, p1, acc[1]);
-    if constexpr (ND == 3) acc[2] = fma(fac, p2, acc[2]);
-    double drr = rho_a - rho_b;
-    if (ga) {
-        const double *gb = P.ddgrad + (size_t)n.j * ND;
-        double gr = (ga[0] + gb[0]) * p0 + (ga[1] + gb[1]) * p1;
-        if constexpr (ND == 3) gr += (ga[2] + gb[2]) * p2;
-        drr = fma(-0.5, gr, drr);
-    }
-    drho = fma(n.v.x * kdr, fma(P.dd2, drr, rho_a * vr), drho);
-}
-
-// The wall viscosity (none without one) acts along r_ab for ArtificialViscosityMonaghan
-// (viscosity.jl:89-132, approaching pairs only) and along v_ab for ViscosityAdami
-// (viscosity.jl:219-280, with eta_a = nu_a rho_a from the fluid's kinematic viscosity and
-// eta_b = nu rho_b, viscosity_model dispatch of wall_boundary/system.jl:286-289 and
-// fluid.jl:109-112); v_b is the wall's no-slip velocity and rho_b = m_b / V_b.
-template <int ND>
-__device__ __forceinline__ void pair_wall(const DevParams &P, const WallNbr &n,
-                                          const double *xa, const double *va, double rho_a,
-                                          double p_a, double inv_rho_a, double *acc,
-                                          double &drho, double m_a)
-{
-    const double p0 = xa[0] - n.a.x, p1 = xa[1] - n.a.y, p2 = (ND == 3) ? xa[2] - n.a.z : 0.0;
-    const double d2 = dist2<ND>(p0, p1, p2);
-    const double r = sqrt_pos(d2);
-    const double w = fma(-0.5 * P.h_inv, r, 1.0);
-    const double kdr = P.kgrad_c * (w * w * w);
-    double vr = va[0] * p0 + va[1] * p1;
-    if constexpr (ND == 3) vr += va[2] * p2;
-    double fac = -n.pv.y * inv_rho_a * (p_a + n.pv.x) * kdr;
-    if (P.visc_w) {
-        const double v0 = va[0] - n.v[0], v1 = va[1] - n.v[1];
-        const double v2 = (ND == 3) ? va[2] - n.v[2] : 0.0;
-        const double m_b = n.a.w, V_b = n.pv.y, rho_b = m_b / V_b;
-        if (P.visc_w == 1) {
-            double vrw = v0 * p0 + v1 * p1;
-            if constexpr (ND == 3) vrw += v2 * p2;
-            if (vrw < 0.0) {
-                const double mu = P.h * vrw / (d2 + P.w_eps_h2);
-                fac += (m_b * P.w_ac * mu + m_b * P.w_beta * (mu * mu)) /
-                       (0.5 * (rho_a + rho_b)) * kdr;
-            }
-        } else {
-            const double eta_a = P.nu_kin * rho_a, eta_b = P.w_nu * rho_b;
-            const double V_a = m_a / rho_a;
-            const double tmp = 2.0 * eta_a * eta_b / ((eta_a + eta_b) * (d2 + P.w_eps_h2) * m_a);
-            const double visc = (V_a * V_a + V_b * V_b) * (kdr * d2) * tmp;
-            acc[0] = fma(visc, v0, acc[0]);
-            acc[1] = fma(visc, v1, acc[1]);
-            if constexpr (ND == 3) acc[2] = fma(visc, v2, acc[2]);
-        }
-    }
-    acc[0] = fma(fac, p0, acc[0]);
-    acc[1] = fma(fac, p1, acc[1]);
-    if constexpr (ND == 3) acc[2] = fma(fac, p2, acc[2]);
-    drho = fma(n.pv.y * kdr, rho_a * vr, drho);
-}
-
-// pair_fluid without branches, for candidates evaluated in groups: an out-of-range (or
-// invalid) candidate gets kdr = 0, so its terms are exact zeros, and the viscosity term is
-// selected by the sign of v_ab.r_ab.  For an in-range candidate every operation is the one
-// of pair_fluid, so the sums agree bit for bit.
-template <int ND>
-__device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const FluidNbr &n,
-                                                  bool valid, const double *xa,
-                                                  const double *va, double rho_a, double p_a,
-                                                  double inv_rho_a, double *acc, double &drho)
-{
-    const double p0 = xa[0] - n.a.x, p1 = xa[1] - n.a.y, p2 = (ND == 3) ? xa[2] - n.a.z : 0.0;
-    const double d2 = dist2<ND>(p0, p1, p2);
-    const bool in = valid & (d2 <= P.R2) & (d2 >= P.az2);
-    const double d2s = in ? d2 : P.R2;  // a normal argument for the square root
-    const double r = sqrt_pos(d2s);
-    const double w = fma(-0.5 * P.h_inv, r, 1.0);
-    const double kdr = in ? P.kgrad_c * (w * w * w) : 0.0;
-    const double v0 = va[0] - n.b.x, v1 = va[1] - n.b.y, v2 = (ND == 3) ? va[2] - n.b.z : 0.0;
-    double vr = v0 * p0 + v1 * p1;
-    if constexpr (ND == 3) vr += v2 * p2;
-    const double rho_b = n.a.w;
-    double fac = -n.v.x * inv_rho_a * (p_a + n.b.w);
-    if (P.use_visc) {
-        const double q = d2s + P.eps_h2;
-        const double sum_rho = rho_a + rho_b;
-        double fv = fma(2.0 * P.av_c * n.v.y * vr, rcp_pos(q * sum_rho), fac);
-        if (P.beta != 0.0) {
-            const double mu = P.h * vr * rcp_pos(q);
-            fv += 2.0 * n.v.y * P.beta * (mu * mu) * rcp_pos(sum_rho);
-        }
-        fac = vr < 0.0 ? fv : fac;
-    }
-    fac *= kdr;
-    acc[0] = fma(fac, p0, acc[0]);
-    acc[1] = fma(fac, p1, acc[1]);
-    if constexpr (ND == 3) acc[2] = fma(fac, p2, acc[2]);
-    drho = fma(n.v.x * kdr, fma(P.dd2, rho_a - rho_b, rho_a * vr), drho);
-}
 
 // Two-phase sweep over the neighbours of one target (see the file header).  lst points at
 // this lane's column of the block's LDS lists (stride kBlock).  Phase 1 tests four
@@ -1203,341 +726,6 @@ __global__ void __launch_bounds__(kBlock)
     dv[(size_t)i * (ND + 1) + ND] = drho + drhow;
 }
 
-// ------------------------------------------------------------------------------------------
-// Tiled fluid interaction.  The cell grid is cut into tiles of TX x TY x TZ cells; each block
-// takes tiles from a work counter.  For a tile it stages in LDS, with coalesced loads, every
-// fluid particle that can be a neighbour of one of the tile's particles: the cell rows
-// (y, z) within reach of the tile box, each trimmed in x to sqrt(R^2 - d^2) + skin, where d
-// is the row's transverse distance from the box (both grown by half the skin).  Staged per
-// candidate: x, y, z, rho, V and the global index.  The tile's particles then run the
-// two-phase sweep of the global kernel with positions, rho and V read from LDS and only
-// {v, p} gathered from global memory.  A tile whose halo exceeds the LDS capacity runs the
-// global sweep instead.  Wall neighbours always come from the global wall sweep.
-//
-// The halo widths (cells) follow from the cell sizes: a cell is (R + skin/2 + margin) / k
-// along an axis with k divisions, so the reach of any point stays within k cells.
-// ------------------------------------------------------------------------------------------
-template <int ND>
-struct TileCfg;
-template <>
-struct TileCfg<3> {
-    static constexpr int TX = 8, TY = 4, TZ = 8, HX = 4, HY = 2, HZ = 2;
-    static constexpr int THREADS = 256, CAPC = 1280, CAPL = 24;
-};
-template <>
-struct TileCfg<2> {
-    static constexpr int TX = 24, TY = 12, TZ = 1, HX = 4, HY = 2, HZ = 0;
-    static constexpr int THREADS = 256, CAPC = 1280, CAPL = 24;
-};
-
-template <int ND>
-struct TileSmem {
-    using C = TileCfg<ND>;
-    static constexpr int URY = C::TY + 2 * C::HY, URZ = C::TZ + 2 * C::HZ;
-    static constexpr int MAXR = URY * URZ;
-    static constexpr int MAXCS = MAXR * (C::TX + 2 * C::HX + 1);
-    static constexpr int TROWS = C::TY * C::TZ;
-    double x[C::CAPC + 4], y[C::CAPC + 4], z[ND == 3 ? C::CAPC + 4 : 1];
-    double rho[C::CAPC], vol[C::CAPC];
-    int g[C::CAPC];
-    int row_gb[MAXR], row_x0[MAXR], row_nx[MAXR], row_lds[MAXR + 1], row_cs[MAXR + 1];
-    int cstart[MAXCS];
-    int trow_gb[TROWS], trow_pre[TROWS + 1];
-    // neighbour lists: u16 LDS indices in the tiled sweep, reused as int lists (half the
-    // entries) by the global sweeps; see l16 for the lane-disjoint layout
-    unsigned short lst[C::CAPL * C::THREADS];
-    int tile, ntot, ncs, ntargets, uy0, nuy, uz0, nuz;
-};
-
-// largest r in [0, n) with pre[r] <= i (pre ascending, pre[0] = 0)
-__device__ __forceinline__ int bsearch_le(const int *pre, int n, int i)
-{
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= i)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
-// bounds of cell c along axis d grown by hs; border cells reach to infinity outwards
-__device__ __forceinline__ double cell_lo(const DevParams &P, int d, int c, double hs)
-{
-    return (c == 0) ? -INFINITY : P.lo[d] + c * P.cs[d] - hs;
-}
-__device__ __forceinline__ double cell_hi(const DevParams &P, int d, int c, double hs)
-{
-    return (c == P.n[d] - 1) ? INFINITY : P.lo[d] + (c + 1) * P.cs[d] + hs;
-}
-
-// Lane lists in the tiled kernel.  The same LDS area serves as u16 lists (tiled sweep) and
-// as int lists with stride THREADS (global sweeps), and waves of a block switch between the
-// two without a barrier.  Entry k of lane t therefore sits in half (k & 1) of the lane's int
-// slot t + THREADS (k >> 1): both views keep every lane inside its own 4-byte slots.
-// lst points at the u16 pair of slot t (S.lst + 2 t).
-template <int THREADS>
-__device__ __forceinline__ int l16(int k)
-{
-    return 2 * THREADS * (k >> 1) + (k & 1);
-}
-
-// Two-phase sweep over the staged fluid candidates of one target (positions, rho, V from
-// LDS, {v, p} from global memory).
-template <int ND>
-__device__ __forceinline__ void tile_sweep(const DevParams &P, const FluidView &F,
-                                           const TileSmem<ND> &S, unsigned short *lst,
-                                           bool active, const double *xa, const double *va,
-                                           double rho_a, double p_a, double inv_rho_a,
-                                           double *acc, double &drho)
-{
-    using C = TileCfg<ND>;
-    const double hs = P.hs_fluid;
-    const double reach = P.R + hs;
-    const int y0 = cell_1d(xa[1] - reach, P.lo[1], P.inv_cs[1], P.n[1]);
-    const int y1 = cell_1d(xa[1] + reach, P.lo[1], P.inv_cs[1], P.n[1]);
-    int z0 = 0, z1 = 0;
-    if constexpr (ND == 3) {
-        z0 = cell_1d(xa[2] - reach, P.lo[2], P.inv_cs[2], P.n[2]);
-        z1 = cell_1d(xa[2] + reach, P.lo[2], P.inv_cs[2], P.n[2]);
-    }
-    int iy = y1 + 1, iz = z0 - 1;
-    double dz2 = 0.0;
-    bool done = !active;
-    int j = 0, je = 0;
-    while (true) {
-        int cnt = 0;
-        while (!done && cnt <= C::CAPL - 4) {
-            if (j >= je) {
-                // next pencil (union row) of this target
-                if (iy > y1) {
-                    if (++iz > z1) {
-                        done = true;
-                        continue;
-                    }
-                    iy = y0;
-                    if constexpr (ND == 3) {
-                        const double dz = transverse(P, 2, iz, xa[2], hs);
-                        dz2 = dz * dz;
-                    }
-                }
-                const int cy = iy++;
-                // rows outside the staged union can not hold neighbours (the union covers
-                // the reach of every point of the grown tile box); skip them defensively
-                if (cy < S.uy0 || cy >= S.uy0 + S.nuy || iz < S.uz0 || iz >= S.uz0 + S.nuz) continue;
-                const int ur = (cy - S.uy0) + S.nuy * (iz - S.uz0);
-                const int nx = S.row_nx[ur];
-                if (nx == 0) continue;
-                const double dy = transverse(P, 1, cy, xa[1], hs);
-                const double r2 = P.R2 - fma(dy, dy, dz2);
-                if (r2 < 0.0) continue;
-                const double rx = (double)(__builtin_amdgcn_sqrtf((float)r2) * 1.000001f) + hs;
-                const int rx0 = S.row_x0[ur];
-                const int x0 = min(max(cell_1d(xa[0] - rx, P.lo[0], P.inv_cs[0], P.n[0]) - rx0, 0), nx);
-                const int x1 = max(min(cell_1d(xa[0] + rx, P.lo[0], P.inv_cs[0], P.n[0]) - rx0, nx - 1), -1);
-                if (x0 > x1) continue;
-                const int cs = S.row_cs[ur], base = S.row_lds[ur] - S.row_gb[ur];
-                j = base + S.cstart[cs + x0];
-                je = base + S.cstart[cs + x1 + 1];
-                continue;
-            }
-            double d2[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const double p0 = xa[0] - S.x[j + k], p1 = xa[1] - S.y[j + k];
-                const double p2 = (ND == 3) ? xa[2] - S.z[j + k] : 0.0;
-                d2[k] = dist2<ND>(p0, p1, p2);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const bool hit = (j + k < je) & (d2[k] <= P.R2) & (d2[k] >= P.az2);
-                lst[l16<C::THREADS>(cnt)] = (unsigned short)(j + k);
-                cnt += hit ? 1 : 0;
-            }
-            j += 4;
-        }
-        // phase 2
-        if (cnt > 0) {
-            int jl = lst[0];
-            double4 vp = F.rec[2 * (size_t)S.g[jl] + 1];
-            for (int k = 0; k < cnt; k++) {
-                const int jn = lst[l16<C::THREADS>(min(k + 1, cnt - 1))];
-                const double4 vpn = F.rec[2 * (size_t)S.g[jn] + 1];
-                FluidNbr n;
-                n.a = make_double4(S.x[jl], S.y[jl], (ND == 3) ? S.z[jl] : 0.0, S.rho[jl]);
-                n.b = vp;
-                n.v = make_double2(S.vol[jl], S.vol[jl] * S.rho[jl]);
-                pair_fluid<ND>(P, n, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
-                jl = jn;
-                vp = vpn;
-            }
-        }
-        if (!__any(!done)) break;
-    }
-}
-
-template <int ND>
-__global__ void __launch_bounds__(TileCfg<ND>::THREADS)
-    k_fluid_tiled(int *flag, DevParams P, FluidView F, WallView Wv,
-                  const unsigned char *__restrict__ wall_near,
-                  const int *__restrict__ tiles, const int *__restrict__ ntiles,
-                  int *__restrict__ work, int tgx, int tgy, double *__restrict__ dv)
-{
-    P = with_sound(P);
-    using C = TileCfg<ND>;
-    using SM = TileSmem<ND>;
-    __shared__ SM S;
-    const int tid = threadIdx.x;
-    if (blockIdx.x == 0 && tid == 0) *flag = 0;  // the rebuild (if any) is done
-    const int nt = *ntiles;
-    while (true) {
-        if (tid == 0) S.tile = atomicAdd(work, 1);
-        __syncthreads();
-        const int tile_slot = S.tile;
-        if (tile_slot >= nt) break;
-        const int tile = tiles[tile_slot];
-        const int tx = tile % tgx, ty = (tile / tgx) % tgy, tz = tile / (tgx * tgy);
-        const int X0 = tx * C::TX, X1 = min(X0 + C::TX, P.n[0]) - 1;
-        const int Y0 = ty * C::TY, Y1 = min(Y0 + C::TY, P.n[1]) - 1;
-        const int Z0 = (ND == 3) ? tz * C::TZ : 0;
-        const int Z1 = (ND == 3) ? min(Z0 + C::TZ, P.n[2]) - 1 : 0;
-        const int uy0 = max(Y0 - C::HY, 0), uy1 = min(Y1 + C::HY, P.n[1] - 1);
-        const int uz0 = (ND == 3) ? max(Z0 - C::HZ, 0) : 0;
-        const int uz1 = (ND == 3) ? min(Z1 + C::HZ, P.n[2] - 1) : 0;
-        const int nuy = uy1 - uy0 + 1, nrows = nuy * (uz1 - uz0 + 1);
-        const double hs = P.hs_fluid;
-        // tile box grown by half the skin (targets may have moved that far)
-        const double bylo = cell_lo(P, 1, Y0, hs), byhi = cell_hi(P, 1, Y1, hs);
-        const double bzlo = (ND == 3) ? cell_lo(P, 2, Z0, hs) : 0.0;
-        const double bzhi = (ND == 3) ? cell_hi(P, 2, Z1, hs) : 0.0;
-        const double bxlo = cell_lo(P, 0, X0, hs), bxhi = cell_hi(P, 0, X1, hs);
-
-        // --- union rows: prune, trim, count
-        int my_cnt = 0, my_cs = 0;
-        if (tid < nrows) {
-            const int cy = uy0 + tid % nuy, cz = uz0 + tid / nuy;
-            const double dy = fmax(0.0, fmax(cell_lo(P, 1, cy, hs) - byhi, bylo - cell_hi(P, 1, cy, hs)));
-            double dz = 0.0;
-            if constexpr (ND == 3)
-                dz = fmax(0.0, fmax(cell_lo(P, 2, cz, hs) - bzhi, bzlo - cell_hi(P, 2, cz, hs)));
-            const double r2 = P.R2 - (dy * dy + dz * dz);
-            int nx = 0, x0 = 0, gb = 0;
-            if (r2 >= 0.0) {
-                const double rx = (double)(__builtin_amdgcn_sqrtf((float)r2) * 1.000001f) + hs;
-                x0 = cell_1d(bxlo - rx, P.lo[0], P.inv_cs[0], P.n[0]);
-                const int x1 = cell_1d(bxhi + rx, P.lo[0], P.inv_cs[0], P.n[0]);
-                const int row = P.n[0] * (cy + P.n[1] * cz);
-                gb = F.start[row + x0];
-                my_cnt = F.start[row + x1 + 1] - gb;
-                nx = x1 - x0 + 1;
-                my_cs = nx + 1;
-            }
-            S.row_nx[tid] = nx;
-            S.row_x0[tid] = x0;
-            S.row_gb[tid] = gb;
-        }
-        // --- target rows
-        int my_t = 0;
-        if (tid < (Y1 - Y0 + 1) * (Z1 - Z0 + 1)) {
-            const int ny = Y1 - Y0 + 1;
-            const int cy = Y0 + tid % ny, cz = Z0 + tid / ny;
-            const int row = P.n[0] * (cy + P.n[1] * cz);
-            const int gb = F.start[row + X0];
-            S.trow_gb[tid] = gb;
-            my_t = F.start[row + X1 + 1] - gb;
-        }
-        int tot;
-        const int lds0 = block_exclusive_scan(my_cnt, (int *)S.lst, tot);
-        if (tid < nrows) S.row_lds[tid] = lds0;
-        if (tid == 0) S.ntot = tot;
-        const int cs0 = block_exclusive_scan(my_cs, (int *)S.lst, tot);
-        if (tid < nrows) S.row_cs[tid] = cs0;
-        if (tid == 0) S.ncs = tot;
-        const int tr0 = block_exclusive_scan(my_t, (int *)S.lst, tot);
-        const int ntrows = (Y1 - Y0 + 1) * (Z1 - Z0 + 1);
-        if (tid < ntrows) S.trow_pre[tid] = tr0;
-        if (tid == 0) {
-            S.ntargets = tot;
-            S.trow_pre[ntrows] = tot;
-            S.row_lds[nrows] = S.ntot;
-            S.row_cs[nrows] = S.ncs;
-            S.uy0 = uy0;
-            S.nuy = nuy;
-            S.uz0 = uz0;
-            S.nuz = uz1 - uz0 + 1;
-        }
-        __syncthreads();
-        const int ntot = S.ntot, ntargets = S.ntargets;
-        const bool staged = ntot <= C::CAPC && S.ncs <= SM::MAXCS;
-        if (staged) {
-            for (int i = tid; i < S.ncs; i += C::THREADS) {
-                const int r = bsearch_le(S.row_cs, nrows, i);
-                const int cy = uy0 + r % nuy, cz = uz0 + r / nuy;
-                S.cstart[i] = F.start[P.n[0] * (cy + P.n[1] * cz) + S.row_x0[r] + (i - S.row_cs[r])];
-            }
-            for (int i = tid; i < ntot; i += C::THREADS) {
-                const int r = bsearch_le(S.row_lds, nrows, i);
-                const int g = S.row_gb[r] + (i - S.row_lds[r]);
-                S.x[i] = F.x[g];
-                S.y[i] = F.y[g];
-                if constexpr (ND == 3) S.z[i] = F.z[g];
-                S.rho[i] = F.rec[2 * (size_t)g].w;
-                S.vol[i] = F.vol[g].x;
-                S.g[i] = g;
-            }
-        }
-        __syncthreads();
-
-        // --- the tile's particles
-        for (int base = 0; base < ntargets; base += C::THREADS) {
-            const int t = base + tid;
-            const bool active = t < ntargets;
-            int s = 0;
-            double xa[3] = {0.0, 0.0, 0.0}, va[3] = {0.0, 0.0, 0.0}, rho_a = 1.0, p_a = 0.0;
-            if (active) {
-                const int r = bsearch_le(S.trow_pre, ntrows, t);
-                s = S.trow_gb[r] + (t - S.trow_pre[r]);
-                const double4 a = F.rec[2 * (size_t)s];
-                const double4 b = F.rec[2 * (size_t)s + 1];
-                xa[0] = a.x; xa[1] = a.y; xa[2] = a.z; rho_a = a.w;
-                va[0] = b.x; va[1] = b.y; va[2] = b.z; p_a = b.w;
-            }
-            const double inv_rho_a = 1.0 / rho_a;
-            double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
-            if (staged)
-                tile_sweep<ND>(P, F, S, S.lst + 2 * tid, active, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
-            else
-                neighbour_sweep<ND, false, C::CAPL / 2, C::THREADS>(
-                    P, F.start, F.x, F.y, F.z, hs, active, F, Wv, (int *)S.lst + tid, xa, va,
-                    rho_a, p_a, inv_rho_a, acc, drho);
-            double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
-            if (P.nw > 0) {
-                bool near = false;
-                if (active) {
-                    const int c0 = cell_1d(xa[0], P.lo[0], P.inv_cs[0], P.n[0]);
-                    const int c1 = cell_1d(xa[1], P.lo[1], P.inv_cs[1], P.n[1]);
-                    const int c2 = (ND == 3) ? cell_1d(xa[2], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
-                    near = wall_near[c0 + P.n[0] * (c1 + P.n[1] * c2)] != 0;
-                }
-                if (__any(near)) {
-                    const double m_a = (P.visc_w == 2 && active) ? F.vol[s].y : 1.0;
-                    neighbour_sweep<ND, true, C::CAPL / 2, C::THREADS>(
-                        P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, near, F, Wv, (int *)S.lst + tid,
-                        xa, va, rho_a, p_a, inv_rho_a, accw, drhow, m_a);
-                }
-            }
-            if (active) {
-                const int i = F.perm[s];
-#pragma unroll
-                for (int d = 0; d < ND; d++) dv[(size_t)i * (ND + 1) + d] = (acc[d] + accw[d]) + P.g[d];
-                dv[(size_t)i * (ND + 1) + ND] = drho + drhow;
-            }
-        }
-        __syncthreads();  // LDS is restaged for the next tile
-    }
-}
 
 // ------------------------------------------------------------------------------------------
 // Bricks in LDS.  A brick is a block of bc[0] x bc[1] x bc[2] cells; its targets are the
@@ -2106,29 +1294,6 @@ __global__ void __launch_bounds__(kBlock)
     for (int d = 0; d < ND; d++) P.ddgrad[(size_t)s * ND + d] = g[d];
 }
 
-// Active tiles (at least one fluid particle), listed after every rebuild of the cell list.
-template <int ND>
-__global__ void k_tiles(const int *flag, DevParams P, const int *__restrict__ start, int tgx,
-                        int tgy, int ntiles_all, int *__restrict__ tiles, int *__restrict__ ntiles)
-{
-    if (skip(flag)) return;
-    using C = TileCfg<ND>;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles_all) return;
-    const int tx = t % tgx, ty = (t / tgx) % tgy, tz = t / (tgx * tgy);
-    const int X0 = tx * C::TX, X1 = min(X0 + C::TX, P.n[0]) - 1;
-    const int Y0 = ty * C::TY, Y1 = min(Y0 + C::TY, P.n[1]) - 1;
-    const int Z0 = (ND == 3) ? tz * C::TZ : 0;
-    const int Z1 = (ND == 3) ? min(Z0 + C::TZ, P.n[2]) - 1 : 0;
-    int cnt = 0;
-    for (int cz = Z0; cz <= Z1; cz++)
-        for (int cy = Y0; cy <= Y1; cy++) {
-            const int row = P.n[0] * (cy + P.n[1] * cz);
-            cnt += start[row + X1 + 1] - start[row + X0];
-        }
-    if (cnt > 0) tiles[atomicAdd(ntiles, 1)] = t;
-}
-
 // drift!: du = v (semidiscretization.jl:561-571; delta_v = 0 without shifting)
 template <int ND>
 __global__ void k_drift(int nf, const double *__restrict__ v, double *__restrict__ du)
@@ -2352,151 +1517,11 @@ __global__ void k_record_pressure(int n, const double4 *__restrict__ rec, double
     p[s] = rec[2 * (size_t)s + 1].w;
 }
 
-inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
-inline int gate_blocks(int64_t n) { return std::max(1, std::min(nblocks(n), kGateBlocks)); }
 
 }  // namespace
 
-// ==========================================================================================
-// handle
-// ==========================================================================================
-struct tpsph_handle {
-    tpsph_params params{};
-    DevParams P{};
-    int nd = 0;
-    int nf = 0, nw = 0;
-    hipStream_t stream = nullptr;
-    std::string last_error;
-
-    int *flag = nullptr;  // rebuild of the fluid cell list pending
-    // fluid cell list + cell-sorted state
-    int *f_cell = nullptr, *f_rank = nullptr, *f_count = nullptr, *f_start = nullptr,
-        *f_perm = nullptr;
-    int *scan_tiles = nullptr;
-    double *f_mass = nullptr;  // caller order
-    double *xs = nullptr, *ys = nullptr, *zs = nullptr, *snap = nullptr;
-    int *iperm = nullptr;  // sorted index of every caller particle (last rebuild)
-    double4 *rec = nullptr;
-    double2 *vol = nullptr;
-    // walls (static, cell-sorted) + per kick
-    int *w_start = nullptr, *w_perm = nullptr;
-    double *wx = nullptr, *wy = nullptr, *wz = nullptr, *pw = nullptr, *rhow = nullptr;
-    double4 *wpos = nullptr;
-    double2 *wpv = nullptr;
-    unsigned char *wall_near = nullptr;  // per cell: a wall particle lies within reach
-    int search = TPSPH_SEARCH_CELLS;  // resolved pair search
-    // Verlet lists (slot-major, ls slots per row) and their counts
-    int *nbr = nullptr, *cnt = nullptr, *wnbr = nullptr, *wcnt = nullptr;
-    int ls = 0, cap_f = 0, cap_w = 0;
-    // processing order of the list kernels (bricks of cells) and the bricks in LDS
-    int *ord = nullptr, *bcount = nullptr, *bstart = nullptr;
-    int nbricks = 0;
-    bool bricks = false;
-    int *brow_gb = nullptr, *brow_lds = nullptr, *bntot = nullptr, *nel = nullptr, *bwork = nullptr;
-    unsigned short *nbr16 = nullptr;
-    double *dyn = nullptr;       // adaptive sound speed: {c, B, av_c, dd2}
-    double *wvel = nullptr;      // no-slip wall velocities (3 per wall particle)
-    double *ddgrad = nullptr;    // DensityDiffusionAntuono: renormalised density gradients
-    double *vmax_part = nullptr; // per-block maxima of v.v (kGateBlocks)
-    // Verlet lists of the wall particles over the fluid (Adami pass), active wall particles
-    int *anbr = nullptr, *acnt = nullptr, *wactive = nullptr;
-    int lw = 0, cap_a = 0;
-    // counters rebuilt with the lists: [0] non-empty bricks, [1] active wall particles,
-    // [2] fluid particles with wall candidates (wtargets)
-    int *counters = nullptr, *wtargets = nullptr;
-    // tiled fluid kernel
-    bool tiled = false;
-    int tgx = 1, tgy = 1, ntiles_all = 0, n_cu = 256;
-    int *tiles = nullptr, *ntiles = nullptr, *work = nullptr;
-    // host-mode staging and solver scratch
-    double *st_v = nullptr, *st_u = nullptr, *st_dv = nullptr, *st_du = nullptr;
-    double *sol_dv = nullptr, *sol_tv = nullptr, *sol_tu = nullptr;
-    bool have_kick = false;
-
-    bool timing = false;
-    struct Rec {
-        int kind;
-        hipEvent_t a, b;
-    };
-    std::vector<hipEvent_t> event_pool;
-    size_t events_used = 0;
-    std::vector<Rec> records;
-
-    FluidView fluid_view() const { return {f_start, f_perm, xs, ys, zs, rec, vol}; }
-    WallView wall_view() const { return {w_start, wx, wy, wz, wpos, wpv, wvel}; }
-};
-
 namespace {
 
-int fail(tpsph_handle *h, int code, const std::string &msg)
-{
-    if (h)
-        h->last_error = msg;
-    else
-        g_create_error = msg;
-    return code;
-}
-
-#define HIP_TRY(h, expr)                                                                      \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return fail((h), TPSPH_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-#define TRY(expr)                        \
-    do {                                 \
-        int rc_ = (expr);                \
-        if (rc_ != TPSPH_OK) return rc_; \
-    } while (0)
-
-template <typename T>
-int dalloc(tpsph_handle *h, T **p, size_t count)
-{
-    *p = nullptr;
-    if (count == 0) count = 1;
-    HIP_TRY(h, hipMalloc((void **)p, count * sizeof(T)));
-    return TPSPH_OK;
-}
-
-// all bits set: a quiet NaN in every double
-int hip_fill(tpsph_handle *h, double *p, size_t count)
-{
-    HIP_TRY(h, hipMemset(p, 0xff, count * sizeof(double)));
-    return TPSPH_OK;
-}
-
-int launch_check(tpsph_handle *h, const char *what)
-{
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        return fail(h, TPSPH_ERR_HIP, std::string("launch of ") + what + ": " + hipGetErrorString(e));
-    return TPSPH_OK;
-}
-
-struct TimedScope {
-    tpsph_handle *h;
-    hipEvent_t b = nullptr;
-    TimedScope(tpsph_handle *h_, int kind) : h(h_)
-    {
-        if (!h->timing) return;
-        if (h->events_used + 2 > h->event_pool.size()) {
-            for (int k = 0; k < 64; k++) {
-                hipEvent_t e;
-                if (hipEventCreate(&e) != hipSuccess) return;
-                h->event_pool.push_back(e);
-            }
-        }
-        hipEvent_t a = h->event_pool[h->events_used++];
-        b = h->event_pool[h->events_used++];
-        (void)hipEventRecord(a, h->stream);
-        h->records.push_back({kind, a, b});
-    }
-    ~TimedScope()
-    {
-        if (b) (void)hipEventRecord(b, h->stream);
-    }
-};
 
 int scan(tpsph_handle *h, const int *flag, const int *in, int *out, int n)
 {
@@ -2543,12 +1568,6 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
         TimedScope t(h, TPSPH_TIME_CELL_LIST);
         TRY(build_cells<ND>(h, h->flag, nf, u, h->f_cell, h->f_rank, h->f_count, h->f_start,
                             h->f_perm));
-        if (h->search == TPSPH_SEARCH_TILES) {
-            hipLaunchKernelGGL((k_fill_i32), dim3(1), dim3(1), 0, h->stream, h->flag, h->ntiles, 0, 1);
-            hipLaunchKernelGGL((k_tiles<ND>), dim3(nblocks(h->ntiles_all)), dim3(kBlock), 0, h->stream, 
-                h->flag, h->P, h->f_start, h->tgx, h->tgy, h->ntiles_all, h->tiles, h->ntiles);
-            TRY(launch_check(h, "k_tiles"));
-        }
         const bool lists = h->search == TPSPH_SEARCH_LISTS;
         // with lists the gather also counts the bricks (f_cell / f_rank are free again after
         // the cell list build) and zeroes the list-build counters
@@ -2636,12 +1655,6 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
         } else if (h->search == TPSPH_SEARCH_LISTS) {
             hipLaunchKernelGGL((k_fluid_list<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
                 h->flag, h->P, h->fluid_view(), h->wall_view(), h->ord, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
-        } else if (h->search == TPSPH_SEARCH_TILES) {
-            hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->work, 0);
-            const int grid = std::max(1, std::min(h->ntiles_all, 2 * h->n_cu));
-            hipLaunchKernelGGL((k_fluid_tiled<ND>), dim3(grid), dim3(TileCfg<ND>::THREADS), 0, h->stream,
-                h->flag, h->P, h->fluid_view(), h->wall_view(), h->wall_near, h->tiles, h->ntiles, h->work,
-                h->tgx, h->tgy, dv);
         } else {
             hipLaunchKernelGGL((k_fluid<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, h->flag,
                 h->P, h->fluid_view(), h->wall_view(), h->wall_near, dv);
@@ -2652,26 +1665,17 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
     return TPSPH_OK;
 }
 
-// gathered: the cell-sorted records already hold (v, u) (written by k_ck_stage_gather)
-int kick_dispatch(tpsph_handle *h, double *dv, const double *v, const double *u,
-                  bool gathered = false)
+
+
+}  // namespace
+
+int tpsph::kick_dispatch(tpsph_handle *h, double *dv, const double *v, const double *u,
+                         bool gathered)
 {
     return h->nd == 2 ? kick_impl<2>(h, dv, v, u, gathered) : kick_impl<3>(h, dv, v, u, gathered);
 }
 
-bool is_host(const tpsph_handle *h) { return h->params.memory == TPSPH_MEM_HOST; }
-size_t v_len(const tpsph_handle *h) { return (size_t)h->nf * (h->nd + 1); }
-size_t u_len(const tpsph_handle *h) { return (size_t)h->nf * h->nd; }
-
-int ensure_staging(tpsph_handle *h)
-{
-    if (h->st_v) return TPSPH_OK;
-    TRY(dalloc(h, &h->st_v, v_len(h)));
-    TRY(dalloc(h, &h->st_u, u_len(h)));
-    TRY(dalloc(h, &h->st_dv, v_len(h)));
-    TRY(dalloc(h, &h->st_du, u_len(h)));
-    return TPSPH_OK;
-}
+namespace {
 
 int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, const double *wx,
              const double *wm)
@@ -2707,14 +1711,11 @@ int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, co
     if (p->cell_divisions < 0 || p->cell_divisions > 4) return bad("cell_divisions must be in 0..4");
     if (p->x_cell_divisions < 0 || p->x_cell_divisions > 8)
         return bad("x_cell_divisions must be in 0..8");
-    if (p->pair_search < TPSPH_SEARCH_AUTO || p->pair_search > TPSPH_SEARCH_TILES)
+    if (p->pair_search < TPSPH_SEARCH_AUTO || p->pair_search > TPSPH_SEARCH_CELLS)
         return bad("pair_search must be one of enum tpsph_pair_search");
     if (p->use_density_diffusion < TPSPH_DIFFUSION_NONE ||
         p->use_density_diffusion > TPSPH_DIFFUSION_ANTUONO)
         return bad("use_density_diffusion must be one of enum tpsph_density_diffusion");
-    if (p->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO && p->pair_search == TPSPH_SEARCH_TILES)
-        return bad("DensityDiffusionAntuono runs with the Verlet lists or the cell walk, not "
-                   "pair_search TPSPH_SEARCH_TILES");
     if (!(p->skin < 1.0)) return bad("skin must be below 1 (a fraction of the compact support)");
     if (nf > 0 && !fm) return bad("fluid_mass is NULL");
     if (nw > 0 && (!wx || !wm)) return bad("wall_coordinates / wall_mass is NULL");
@@ -2899,17 +1900,9 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         P.rl_union2 = (rl + margin) * (rl + margin);
         P.hs_union = margin;
     }
-    // tiled fluid kernel: needs the default cell divisions (its halo widths)
-    h->tiled = (nd == 3) ? (div == TileCfg<3>::HY && xdiv == TileCfg<3>::HX)
-                         : (div == TileCfg<2>::HY && xdiv == TileCfg<2>::HX);
     h->search = params->pair_search;
     if (h->search == TPSPH_SEARCH_AUTO)
         h->search = P.half_skin > 0.0 ? TPSPH_SEARCH_LISTS : TPSPH_SEARCH_CELLS;
-    if (h->search == TPSPH_SEARCH_TILES && !h->tiled) {
-        delete h;
-        return fail(nullptr, TPSPH_ERR_ARGUMENT,
-                    "pair_search TPSPH_SEARCH_TILES needs the default cell divisions");
-    }
     if (h->search == TPSPH_SEARCH_LISTS) {
         // capacities: 1.5 x the expected count in the list ball (particle volume m / rho0)
         // plus 16; walls fill at most 7/8 (3D) or 3/4 (2D) of the ball around a corner
@@ -2940,15 +1933,6 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         // a wall particle of a flat wall sees fluid in at most half its ball
         h->cap_a = h->nw > 0 ? cap(mf > 0.0 ? 0.6 * ball(rlw) / (mf / P.rho0) : 0.0) : 0;
         h->lw = (h->nw + 63) / 64 * 64;
-    }
-    if (nd == 3) {
-        h->tgx = (P.n[0] + TileCfg<3>::TX - 1) / TileCfg<3>::TX;
-        h->tgy = (P.n[1] + TileCfg<3>::TY - 1) / TileCfg<3>::TY;
-        h->ntiles_all = h->tgx * h->tgy * ((P.n[2] + TileCfg<3>::TZ - 1) / TileCfg<3>::TZ);
-    } else {
-        h->tgx = (P.n[0] + TileCfg<2>::TX - 1) / TileCfg<2>::TX;
-        h->tgy = (P.n[1] + TileCfg<2>::TY - 1) / TileCfg<2>::TY;
-        h->ntiles_all = h->tgx * h->tgy;
     }
     {
         int dev = 0, cus = 0;
@@ -3036,9 +2020,6 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
             if (h->bricks) CTRY(dalloc(h, &h->wtargets, nf));
         }
     }
-    CTRY(dalloc(h, &h->tiles, (size_t)h->ntiles_all));
-    CTRY(dalloc(h, &h->ntiles, 1));
-    CTRY(dalloc(h, &h->work, 1));
     // NaN (all bits set) in the coordinate arrays, including their padding; never zero so a
     // padded entry can not pass the radius test
     for (double *a : {h->xs, h->ys, h->zs})
@@ -3411,7 +2392,7 @@ void tpsph_destroy(tpsph_handle *h)
     void *ptrs[] = {h->flag, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm,
                     h->scan_tiles, h->f_mass, h->xs, h->ys, h->zs, h->snap, h->rec, h->vol,
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
-                    h->tiles, h->ntiles, h->work, h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
+                    h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,
                     h->wtargets, h->iperm, h->dyn, h->vmax_part, h->wvel, h->ddgrad,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,

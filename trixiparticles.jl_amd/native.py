@@ -26,9 +26,7 @@ PAIR_WALL_FLUID = 2
 SEARCH_AUTO = 0
 SEARCH_LISTS = 1
 SEARCH_CELLS = 2
-SEARCH_TILES = 3
-PAIR_SEARCH = {"auto": SEARCH_AUTO, "lists": SEARCH_LISTS, "cells": SEARCH_CELLS,
-               "tiles": SEARCH_TILES}
+PAIR_SEARCH = {"auto": SEARCH_AUTO, "lists": SEARCH_LISTS, "cells": SEARCH_CELLS}
 
 TIME_CELL_LIST = 0
 TIME_GATHER = 1

@@ -176,8 +176,7 @@ class DensityDiffusionAntuono:
     """``DensityDiffusionAntuono(; delta)`` (density_diffusion.jl:89-125): the
     Molteni-Colagrossi term with the renormalised density gradients of both particles
     subtracted, which every kick computes in a pass of its own (``update!``,
-    density_diffusion.jl:176-211).  It runs with the Verlet lists or the cell walk, not
-    with ``pair_search="tiles"``."""
+    density_diffusion.jl:176-211).  It runs with the Verlet lists or the cell walk."""
 
     kind = 2  # enum tpsph_density_diffusion
 

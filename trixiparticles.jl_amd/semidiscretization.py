@@ -64,8 +64,8 @@ class Semidiscretization:
     by ``domain_margin`` times its extent on every side.
 
     ``cell_divisions`` / ``x_cell_divisions`` (cells per compact support along y, z / x),
-    ``skin`` (fraction of the compact support) and ``pair_search`` ("auto", "lists",
-    "cells" or "tiles") tune the device neighbour search; they change speed, never the
+    ``skin`` (fraction of the compact support) and ``pair_search`` ("auto", "lists" or
+    "cells") tune the device neighbour search; they change speed, never the
     neighbour relation."""
 
     def __init__(self, *systems, neighborhood_search=None, parallelization_backend="hip",
