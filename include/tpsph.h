@@ -76,6 +76,14 @@ enum tpsph_density_diffusion {
     TPSPH_DIFFUSION_ANTUONO = 2    /* DensityDiffusionAntuono(; delta)                     */
 };
 
+/* Arithmetic of the pair terms (tpsph_params.precision). */
+enum tpsph_precision {
+    TPSPH_PRECISION_FP64 = 0,  /* FP64 throughout (the reference's Float64)                */
+    TPSPH_PRECISION_MIXED = 1  /* FP32 pair arithmetic on brick-relative coordinates with  */
+                               /* compensated density-rate sums; FP64 positions, state,    */
+                               /* state equation, stage update and reductions              */
+};
+
 /* Kernel identifiers for tpsph_kernel_time. */
 enum tpsph_timed_kernel {
     TPSPH_TIME_CELL_LIST = 0,   /* fluid cell list (counting sort) and Verlet lists        */
@@ -166,7 +174,7 @@ typedef struct tpsph_params {
     double domain_max[3];
     double skin;                   /* < 0 = default (0.04) */
     int32_t adaptive_sound_speed;  /* 0/1: StateEquationAdaptiveCole */
-    int32_t reserved0;
+    int32_t precision;             /* enum tpsph_precision, 0 = FP64 */
     double mach_number_target;
     double min_sound_speed;
     double max_sound_speed;
@@ -208,7 +216,9 @@ int tpsph_drift(tpsph_handle *h, double *du_ode, const double *v_ode, const doub
 
 /* Quantities of the last kick, in the caller's particle order (any pointer may be NULL):
  * system.pressure (n_fluid), boundary_model.pressure (n_wall),
- * boundary_model.cache.density (n_wall). */
+ * boundary_model.cache.density (n_wall).  tpsph_solve_ck2n54 ends with a state no kick has
+ * seen yet, so after it this returns TPSPH_ERR_STATE until the next tpsph_kick (the
+ * callbacks of the reference evaluate kick! on the saved state first, callbacks.jl:28-40). */
 int tpsph_get_pressures(tpsph_handle *h, double *fluid_pressure, double *wall_pressure,
                         double *wall_density);
 
@@ -226,6 +236,19 @@ int tpsph_get_sound_speed(tpsph_handle *h, double *sound_speed);
  * total.  u_ode is the fluid position array as in tpsph_kick. */
 int tpsph_neighbors_csr(tpsph_handle *h, const double *u_ode, int32_t pair, int64_t *offsets,
                         int32_t *idx, int64_t capacity);
+
+/* The pairs the kernels of the last kick evaluated, as CSR in the caller's order (rows and
+ * columns as in tpsph_neighbors_csr, columns ascending per row): the candidates of the lists
+ * the kick read -- the brick lists of LDS indices, the Verlet lists, or the cell walk of an
+ * overflowed list or of the cell search -- that passed the kernels' exact radius test on
+ * the kick's positions.  That is PointNeighbors' foreach_point_neighbor relation
+ * (neighborhood_search.jl:5-13) without the pairs the pair terms skip at distance^2 below
+ * eps(R^2) (wcsph/rhs.jl:27-28, 52) for TPSPH_PAIR_FLUID_FLUID and TPSPH_PAIR_FLUID_WALL,
+ * and with them for the Adami sums of TPSPH_PAIR_WALL_FLUID (dummy_particles.jl:637-672).
+ * Exposes what the kick summed, for the parity tests of the pair search; TPSPH_ERR_STATE
+ * before the first kick.  Capacity semantics as tpsph_neighbors_csr. */
+int tpsph_kick_pairs_csr(tpsph_handle *h, int32_t pair, int64_t *offsets, int32_t *idx,
+                         int64_t capacity);
 
 /* One CarpenterKennedy2N54 stage over y (n values) with register tmp and slope k:
  *   first stage: tmp = dt*k;              y = y + b*tmp

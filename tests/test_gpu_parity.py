@@ -220,10 +220,47 @@ def test_skin_rebuild_keeps_results_exact(tp, orc, search):
 
 
 def test_full_size_3d_kick_matches_oracle(tp, orc):
-    """The 1.02M-particle 3D dam break (dx = 6.25e-3) at full size."""
+    """The 1.02M-particle 3D dam break (dx = 6.25e-3) at full size, with a random velocity
+    field (so the viscosity and the continuity term contribute) and densities perturbed
+    around the hydrostatic ones (so the diffusion term does)."""
     db, semi = setup_case(tp, 3, 6.25e-3)
     v0, u0 = semi.initial_state()
-    v, u = v0.cpu().numpy(), u0.cpu().numpy()
-    dv_ref, *_ = oracle_kick(orc, semi, db, v, u)
+    v, u = v0.cpu().numpy().copy(), u0.cpu().numpy()
+    rng = np.random.default_rng(17)
+    v[:, :3] = rng.normal(0.0, 0.5, size=(v.shape[0], 3))
+    v[:, 3] *= 1.0 + rng.uniform(-2e-3, 2e-3, size=v.shape[0])
+    dv_ref, pf_ref, pw_ref, rw_ref = oracle_kick(orc, semi, db, v, u)
     got = gpu_kick(tp, semi, v, u)
     check_columns(got, dv_ref)
+    pf, pw, rw = semi.update_pressures()
+    assert rel_err(pw, pw_ref) <= RTOL and rel_err(rw, rw_ref) <= RTOL
+
+
+def test_overflowed_lists_between_rebuilds(tp, orc):
+    """With 40x the masses every list overflows and the kicks walk the cells over the SoA
+    coordinates, which the gather keeps current between rebuilds only while some list has
+    overflowed.  Kicks at u, at u moved by less than half the skin (no rebuild: the walk
+    must read the moved positions) and then 5 native CK2N54 steps all match the oracle."""
+    db = tp.dam_break(3, 0.025)
+    db.fluid_system.mass = db.fluid_system.mass * 40.0
+    semi = tp.Semidiscretization(db.fluid_system, db.boundary_system,
+                                 parallelization_backend="hip", pair_search="lists")
+    v0, u0 = semi.initial_state()
+    v, u = v0.cpu().numpy().copy(), u0.cpu().numpy().copy()
+    rng = np.random.default_rng(19)
+    v[:, :3] = rng.normal(0.0, 0.2, size=(v.shape[0], 3))
+    R = 2 * db.fluid_system.smoothing_length
+    dv_ref, *_ = oracle_kick(orc, semi, db, v, u)
+    check_columns(gpu_kick(tp, semi, v, u), dv_ref)
+    u2 = u + rng.uniform(-0.005 * R, 0.005 * R, size=u.shape)
+    dv_ref, *_ = oracle_kick(orc, semi, db, v, u2)
+    check_columns(gpu_kick(tp, semi, v, u2), dv_ref)
+    import torch
+    ode = tp.DynamicalODEProblem(tp.kick, tp.drift, torch.from_numpy(v).cuda(),
+                                 torch.from_numpy(u2).cuda(), (0.0, 5e-5), {"semi": semi})
+    vs, us = tp.solve(ode, dt=1e-5, nsteps=5)
+    P = orc.params_from_semi(semi)
+    vr, ur = orc.solve_ck2n54(P, v, u2, db.fluid_system.mass, db.boundary_system.coordinates.T,
+                              db.boundary_system.boundary_model.hydrodynamic_mass, 1e-5, 5)
+    check_columns(vs.cpu().numpy(), vr, rtol=1e-11, what="v")
+    check_columns(us.cpu().numpy() - u2, ur - u2, rtol=1e-11, what="u displacement")

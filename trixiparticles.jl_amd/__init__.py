@@ -13,7 +13,7 @@ from .schemes import (AdamiPressureExtrapolation, ArtificialViscosityMonaghan,
                       WallBoundarySystem, WeaklyCompressibleSPHSystem, WendlandC2Kernel)
 from .semidiscretization import (CK2N54_A, CK2N54_B, CK2N54_C, CarpenterKennedy2N54,
                                  DynamicalODEProblem, Semidiscretization, calculate_dt, drift,
-                                 kick, semidiscretize, solve, solve_tstops)
+                                 kick, kick_pairs, semidiscretize, solve, solve_tstops)
 from .postprocess import compute_pressure, interpolate_line, interpolate_points, max_x_coord
 from .setups import InitialCondition, RectangularShape, RectangularTank, splitmix64_uniform
 from . import native
@@ -24,7 +24,7 @@ __all__ = [
     "SchoenbergCubicSplineKernel",
     "StateEquationAdaptiveCole", "StateEquationCole", "ViscosityAdami", "WallBoundarySystem", "WeaklyCompressibleSPHSystem", "WendlandC2Kernel",
     "CK2N54_A", "CK2N54_B", "CK2N54_C", "CarpenterKennedy2N54", "DynamicalODEProblem",
-    "Semidiscretization", "calculate_dt", "drift", "kick", "semidiscretize", "solve",
+    "Semidiscretization", "calculate_dt", "drift", "kick", "kick_pairs", "semidiscretize", "solve",
     "solve_tstops", "InitialCondition",
     "RectangularShape", "RectangularTank", "splitmix64_uniform", "DamBreak", "dam_break",
     "marrone_dam_break_2d", "compute_pressure", "interpolate_line", "interpolate_points",

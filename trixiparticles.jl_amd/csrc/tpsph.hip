@@ -1482,6 +1482,125 @@ __global__ void k_nbr(DevParams P, int nrows, const double *__restrict__ rx,
     }
 }
 
+// Pairs that the last kick's kernels evaluated (parity tests of the pair search): for every
+// row particle, the candidates its kernel read -- the brick list (LDS indices mapped back to
+// sorted indices through the brick's row tables, as the staging pass maps them), the int
+// Verlet list, or the cell walk of an overflowed list or of the cell search -- that pass the
+// kernel's exact test on the kick's positions: az2 <= d^2 <= R^2 for the fluid-fluid and
+// fluid-wall sums (the pair terms), d^2 <= R^2 for the Adami sums of the walls.
+struct EvalPairsArgs {
+    int pair;           // TPSPH_PAIR_*
+    int lists, bricks;  // the kick's pair search
+    const int *ord, *bstart, *bntot, *brow_gb, *brow_lds;
+    int nbricks;
+    const unsigned short *nbr16;
+    const int *nbr, *cnt, *wnbr, *wcnt;
+    int ls;
+    const int *anbr, *acnt;
+    int lw;
+    const int *w_perm;
+};
+
+template <int ND, bool FILL>
+__global__ void k_eval_pairs(DevParams P, FluidView F, WallView Wv, EvalPairsArgs a,
+                             int *__restrict__ counts, const int64_t *__restrict__ offsets,
+                             int *__restrict__ idx)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool wall_rows = a.pair == TPSPH_PAIR_WALL_FLUID;
+    const int nrows = wall_rows ? P.nw : P.nf;
+    if (r >= nrows) return;
+    // the row particle: sorted index s (fluid rows in the kernels' target order)
+    const int s = (!wall_rows && a.lists) ? a.ord[r] : r;
+    double xa[3];
+    if (wall_rows) {
+        const double4 w = Wv.pos[s];
+        xa[0] = w.x, xa[1] = w.y, xa[2] = w.z;
+    } else {
+        const double4 f = F.rec[2 * (size_t)s];
+        xa[0] = f.x, xa[1] = f.y, xa[2] = f.z;
+    }
+    const int row = wall_rows ? a.w_perm[s] : F.perm[s];
+    const bool wall_cols = a.pair == TPSPH_PAIR_FLUID_WALL;
+    const double lower = wall_rows ? -1.0 : P.az2;
+    int64_t o = FILL ? offsets[row] : 0;
+    const int64_t o0 = o;
+    int n_hit = 0;
+    auto consider = [&](int j) {
+        double xb[3];
+        if (wall_cols) {
+            const double4 w = Wv.pos[j];
+            xb[0] = w.x, xb[1] = w.y, xb[2] = w.z;
+        } else {
+            const double4 f = F.rec[2 * (size_t)j];
+            xb[0] = f.x, xb[1] = f.y, xb[2] = f.z;
+        }
+        const double d2 = dist2<ND>(xa[0] - xb[0], xa[1] - xb[1], (ND == 3) ? xa[2] - xb[2] : 0.0);
+        if (d2 <= P.R2 && d2 >= lower) {
+            if (FILL) idx[o++] = wall_cols ? a.w_perm[j] : F.perm[j];
+            n_hit++;
+        }
+    };
+    auto walk = [&](const int *start, double hs) {
+        Pencils<ND> it;
+        it.init(P, xa[0], xa[1], xa[2], hs);
+        int jb, je;
+        while (it.next(P, start, jb, je))
+            for (int j = jb; j < je; j++) consider(j);
+    };
+    if (!a.lists) {
+        if (wall_cols)
+            walk(Wv.start, P.hs_wall);
+        else
+            walk(F.start, P.hs_fluid);
+    } else if (wall_rows) {
+        const int n = a.acnt[s];
+        if (n >= 0)
+            for (int k = 0; k < n; k++) consider(a.anbr[(size_t)k * a.lw + s]);
+        else
+            walk(F.start, P.hs_fluid);
+    } else if (wall_cols) {
+        const int n = a.wcnt[r];
+        if (n >= 0)
+            for (int k = 0; k < n; k++) consider(a.wnbr[(size_t)k * a.ls + r]);
+        else
+            walk(Wv.start, P.hs_wall);
+    } else {
+        const int n = a.cnt[r];
+        int b = -1;
+        if (a.bricks) b = bsearch_le(a.bstart, a.nbricks, r);
+        if (b >= 0 && a.bntot[b] <= kBrickCap - 1) {
+            // the brick list: four LDS indices per group, the far record pads the last group
+            const int *gb = a.brow_gb + (size_t)b * P.maxr;
+            const int *lds = a.brow_lds + (size_t)b * (P.maxr + 1);
+            const int ng = (max(n, 0) + 3) >> 2;
+            for (int g = 0; g < 4 * ng; g++) {
+                const int e = a.nbr16[((size_t)(g >> 2) * a.ls + r) * 4 + (g & 3)];
+                if (e == kBrickCap - 1) continue;
+                const int row_e = bsearch_le(lds, P.maxr + 1, e);
+                consider(gb[row_e] + (e - lds[row_e]));
+            }
+        } else if (n >= 0) {
+            for (int k = 0; k < n; k++) consider(a.nbr[(size_t)k * a.ls + r]);
+        } else {
+            walk(F.start, P.hs_fluid);
+        }
+    }
+    if (!FILL) {
+        counts[row] = n_hit;
+        return;
+    }
+    for (int64_t k = o0 + 1; k < o; k++) {  // ascending columns
+        const int key = idx[k];
+        int64_t j = k - 1;
+        while (j >= o0 && idx[j] > key) {
+            idx[j + 1] = idx[j];
+            j--;
+        }
+        idx[j + 1] = key;
+    }
+}
+
 // cell-sorted SoA coordinates (and optionally {x, y, z, mass} records) of n points
 template <int ND>
 __global__ void k_positions(int n, const double *__restrict__ x, const int *__restrict__ perm,
@@ -1677,6 +1796,127 @@ int tpsph::kick_dispatch(tpsph_handle *h, double *dv, const double *v, const dou
 
 namespace {
 
+// The device parameters of params for nf fluid and nw wall particles on the grid of the box
+// [lo, hi] (one padding cell per side): kernel and state-equation constants, cells, skin
+// radii and bricks.  Failures go to the create error (tpsph_last_error(NULL)).
+int derive_params(const tpsph_params *params, int nf, int nw, const double *lo,
+                  const double *hi, DevParams &P, int &nbricks)
+{
+    const int nd = params->ndims;
+    // --- derived parameters
+    const double hl = params->smoothing_length;
+    const double R = 2.0 * hl;  // compact_support(WendlandC2, h)
+    P.nd = nd;
+    P.nf = nf;
+    P.nw = nw;
+    P.use_visc = params->use_viscosity ? 1 : 0;
+    P.use_dd = params->use_density_diffusion ? 1 : 0;
+    P.clip_fluid = params->clip_fluid_eos ? 1 : 0;
+    P.clip_wall = params->clip_wall_pressure ? 1 : 0;
+    P.h = hl;
+    P.h_inv = 1.0 / hl;
+    P.R = R;
+    P.R2 = R * R;
+    P.two_h = 2.0 * hl;
+    P.az2 = std::nextafter(R * R, INFINITY) - R * R;  // sqrt(eps(R^2))^2
+    const double sigma = nd == 2 ? (7.0 / (4.0 * M_PI)) * (P.h_inv * P.h_inv)
+                                 : (21.0 / (16.0 * M_PI)) * (P.h_inv * P.h_inv * P.h_inv);
+    P.kval_c = sigma;
+    P.kgrad_c = -5.0 * sigma * (P.h_inv * P.h_inv);
+    P.rho0 = params->reference_density;
+    P.c = params->sound_speed;
+    P.gamma = params->exponent;
+    P.inv_gamma = 1.0 / params->exponent;
+    P.p_bg = params->background_pressure;
+    P.B = P.rho0 * (P.c * P.c) / P.gamma;
+    P.alpha = params->alpha;
+    P.beta = params->beta;
+    P.eps_h2 = params->epsilon * (hl * hl);
+    P.av_c = params->alpha * P.c * hl;
+    P.dd2 = params->use_density_diffusion ? 2.0 * params->delta * hl * P.c : 0.0;
+    P.ddgrad = nullptr;  // set below for DensityDiffusionAntuono
+    P.eps_hh = std::nextafter(hl * hl, INFINITY) - hl * hl;
+    for (int d = 0; d < 3; d++) P.g[d] = d < nd ? params->acceleration[d] : 0.0;
+    P.p_offset = params->pressure_offset;
+    P.dyn = nullptr;  // set below for an adaptive state equation
+    P.visc_w = params->wall_viscosity;
+    P.w_alpha = params->wall_alpha;
+    P.w_ac = params->wall_alpha * P.c;
+    P.w_beta = params->wall_beta;
+    P.w_eps_h2 = params->wall_epsilon * hl * hl;
+    P.w_nu = params->wall_nu;
+    P.nu_kin = params->alpha * hl * P.c / (2 * nd + 4);
+    P.mach = params->mach_number_target;
+    P.c_min = params->min_sound_speed;
+    P.c_max = params->max_sound_speed;
+
+    const int div = params->cell_divisions > 0 ? params->cell_divisions : 2;
+    const int xdiv = params->x_cell_divisions > 0 ? params->x_cell_divisions : 2 * div;
+    const double skin = params->skin >= 0.0 ? params->skin : 0.04;
+    double extent = R;
+    for (int d = 0; d < nd; d++) extent = std::fmax(extent, std::fmax(std::fabs(lo[d]), std::fabs(hi[d])) + 4.0 * R);
+    // cell-interval rounding margin: far above the error of floor((x - lo) / cs)
+    const double margin = 1e-9 * R + 64.0 * DBL_EPSILON * extent;
+    P.half_skin = 0.5 * skin * R;
+    P.half_skin2 = P.half_skin * P.half_skin;
+    P.hs_fluid = P.half_skin + margin;
+    P.hs_wall = margin;
+    {
+        const double rlf = R + 2.0 * P.half_skin + margin, rlw = R + P.half_skin + margin;
+        P.rl2_fluid = rlf * rlf;
+        P.rl2_wall = rlw * rlw;
+        P.hs_list_fluid = 2.0 * P.half_skin + 2.0 * margin;
+        P.hs_list_wall = P.half_skin + 2.0 * margin;
+    }
+    // k cells span the reach of a point: R (as bounded by the FP32 square roots of the pencil
+    // widths, which run at most 2e-6 R above it), its own drift and a neighbour's drift
+    const double span = (R * 1.00001 + 2.0 * P.hs_fluid) * (1.0 + 1e-9);
+    int64_t ncells = 1;
+    for (int d = 0; d < 3; d++) {
+        const double cs = span / (d == 0 ? xdiv : div);
+        P.cs[d] = cs;
+        P.inv_cs[d] = 1.0 / cs;
+        if (d < nd) {
+            P.lo[d] = lo[d] - cs;
+            int64_t n = (int64_t)std::ceil((hi[d] + cs - P.lo[d]) * P.inv_cs[d]) + 1;
+            if (n < 1) n = 1;
+            if (n > (1 << 20))
+                return fail(nullptr, TPSPH_ERR_ARGUMENT, "domain box too large for the cell grid");
+            P.n[d] = (int)n;
+        } else {
+            P.lo[d] = 0.0;
+            P.n[d] = 1;
+        }
+        ncells *= P.n[d];
+    }
+    if (ncells > (int64_t)INT32_MAX / 4)
+        return fail(nullptr, TPSPH_ERR_ARGUMENT, "cell grid too large (domain / smoothing_length)");
+    P.ncells = (int)ncells;
+    {
+        // bricks of 3 (3D) or 8 (2D) cell spans per side: about 3.1 R (8.3 R) wide, at
+        // h = 1.2 dx about 420 (400) targets and a union of about 1700 (600) particles
+        const int per = nd == 3 ? 3 : 8;
+        int64_t nb = 1;
+        for (int d = 0; d < 3; d++) {
+            P.bc[d] = d < nd ? per * (d == 0 ? xdiv : div) : 1;
+            P.nb[d] = (P.n[d] + P.bc[d] - 1) / P.bc[d];
+            nb *= P.nb[d];
+        }
+        nbricks = (int)nb;
+        // union rows: cells within the list radius of the brick box (grown by the margin)
+        const double rl = R + 2.0 * P.half_skin + margin;
+        int rows = 1;
+        for (int d = 0; d < 3; d++) {
+            P.ur[d] = (d >= 1 && d < nd) ? (int)std::ceil(rl / P.cs[d]) + 1 : 0;
+            if (d >= 1 && d < nd) rows *= P.bc[d] + 2 * P.ur[d];
+        }
+        P.maxr = rows;
+        P.rl_union2 = (rl + margin) * (rl + margin);
+        P.hs_union = margin;
+    }
+    return TPSPH_OK;
+}
+
 int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, const double *wx,
              const double *wm)
 {
@@ -1722,6 +1962,59 @@ int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, co
     return TPSPH_OK;
 }
 
+// CSR of nrows rows in the caller's order from a count launch and a fill launch of the same
+// kernel: launch(fill, counts, offsets, idx).  offsets / idx may be host or device memory.
+template <typename Launch>
+int emit_csr(tpsph_handle *h, int nrows, Launch launch, int64_t *offsets, int32_t *idx,
+             int64_t capacity)
+{
+    int *counts = nullptr;
+    int64_t *doff = nullptr;
+    int *didx = nullptr;
+    TRY(dalloc(h, &counts, (size_t)nrows));
+    std::vector<int> hc((size_t)nrows);
+    std::vector<int64_t> ho((size_t)nrows + 1, 0);
+    int rc = TPSPH_OK;
+    if (nrows > 0) {
+        launch(false, counts, nullptr, nullptr);
+        rc = launch_check(h, "pair count");
+        if (rc == TPSPH_OK) {
+            hipError_t e = hipMemcpyAsync(hc.data(), counts, (size_t)nrows * sizeof(int), hipMemcpyDeviceToHost, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) rc = fail(h, TPSPH_ERR_HIP, std::string("neighbour counts: ") + hipGetErrorString(e));
+        }
+    }
+    for (int r = 0; r < nrows; r++) ho[(size_t)r + 1] = ho[(size_t)r] + hc[(size_t)r];
+    const int64_t total = ho[(size_t)nrows];
+    if (rc == TPSPH_OK) {
+        hipError_t e = hipMemcpy(offsets, ho.data(), ho.size() * sizeof(int64_t), hipMemcpyDefault);
+        if (e != hipSuccess) rc = fail(h, TPSPH_ERR_HIP, std::string("offsets: ") + hipGetErrorString(e));
+    }
+    if (rc == TPSPH_OK && idx != nullptr && capacity < total)
+        rc = fail(h, TPSPH_ERR_CAPACITY, "idx capacity is smaller than the number of pairs");
+    if (rc == TPSPH_OK && idx != nullptr && total > 0) {
+        rc = dalloc(h, &doff, ho.size());
+        if (rc == TPSPH_OK) rc = dalloc(h, &didx, (size_t)total);
+        if (rc == TPSPH_OK) {
+            hipError_t e = hipMemcpy(doff, ho.data(), ho.size() * sizeof(int64_t), hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = fail(h, TPSPH_ERR_HIP, "offsets upload failed");
+        }
+        if (rc == TPSPH_OK) {
+            launch(true, nullptr, doff, didx);
+            rc = launch_check(h, "pair fill");
+        }
+        if (rc == TPSPH_OK) {
+            hipError_t e = hipMemcpyAsync(idx, didx, (size_t)total * sizeof(int), hipMemcpyDefault, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+            if (e != hipSuccess) rc = fail(h, TPSPH_ERR_HIP, std::string("neighbour lists: ") + hipGetErrorString(e));
+        }
+    }
+    (void)hipFree(counts);
+    (void)hipFree(doff);
+    (void)hipFree(didx);
+    return rc;
+}
+
 }  // namespace
 
 // ==========================================================================================
@@ -1765,54 +2058,6 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         }
     }
 
-    // --- derived parameters
-    const double hl = params->smoothing_length;
-    const double R = 2.0 * hl;  // compact_support(WendlandC2, h)
-    DevParams &P = h->P;
-    P.nd = nd;
-    P.nf = h->nf;
-    P.nw = h->nw;
-    P.use_visc = params->use_viscosity ? 1 : 0;
-    P.use_dd = params->use_density_diffusion ? 1 : 0;
-    P.clip_fluid = params->clip_fluid_eos ? 1 : 0;
-    P.clip_wall = params->clip_wall_pressure ? 1 : 0;
-    P.h = hl;
-    P.h_inv = 1.0 / hl;
-    P.R = R;
-    P.R2 = R * R;
-    P.two_h = 2.0 * hl;
-    P.az2 = std::nextafter(R * R, INFINITY) - R * R;  // sqrt(eps(R^2))^2
-    const double sigma = nd == 2 ? (7.0 / (4.0 * M_PI)) * (P.h_inv * P.h_inv)
-                                 : (21.0 / (16.0 * M_PI)) * (P.h_inv * P.h_inv * P.h_inv);
-    P.kval_c = sigma;
-    P.kgrad_c = -5.0 * sigma * (P.h_inv * P.h_inv);
-    P.rho0 = params->reference_density;
-    P.c = params->sound_speed;
-    P.gamma = params->exponent;
-    P.inv_gamma = 1.0 / params->exponent;
-    P.p_bg = params->background_pressure;
-    P.B = P.rho0 * (P.c * P.c) / P.gamma;
-    P.alpha = params->alpha;
-    P.beta = params->beta;
-    P.eps_h2 = params->epsilon * (hl * hl);
-    P.av_c = params->alpha * P.c * hl;
-    P.dd2 = params->use_density_diffusion ? 2.0 * params->delta * hl * P.c : 0.0;
-    P.ddgrad = nullptr;  // set below for DensityDiffusionAntuono
-    P.eps_hh = std::nextafter(hl * hl, INFINITY) - hl * hl;
-    for (int d = 0; d < 3; d++) P.g[d] = d < nd ? params->acceleration[d] : 0.0;
-    P.p_offset = params->pressure_offset;
-    P.dyn = nullptr;  // set below for an adaptive state equation
-    P.visc_w = params->wall_viscosity;
-    P.w_alpha = params->wall_alpha;
-    P.w_ac = params->wall_alpha * P.c;
-    P.w_beta = params->wall_beta;
-    P.w_eps_h2 = params->wall_epsilon * hl * hl;
-    P.w_nu = params->wall_nu;
-    P.nu_kin = params->alpha * hl * P.c / (2 * nd + 4);
-    P.mach = params->mach_number_target;
-    P.c_min = params->min_sound_speed;
-    P.c_max = params->max_sound_speed;
-
     // --- grid: the domain box (or the wall box when unset), one padding cell per side
     double lo[3], hi[3];
     bool have_box = true;
@@ -1832,74 +2077,15 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
             lo[d] = 0.0;
             hi[d] = 0.0;
         }
-    const int div = params->cell_divisions > 0 ? params->cell_divisions : 2;
-    const int xdiv = params->x_cell_divisions > 0 ? params->x_cell_divisions : 2 * div;
-    const double skin = params->skin >= 0.0 ? params->skin : 0.04;
-    double extent = R;
-    for (int d = 0; d < nd; d++) extent = std::fmax(extent, std::fmax(std::fabs(lo[d]), std::fabs(hi[d])) + 4.0 * R);
-    // cell-interval rounding margin: far above the error of floor((x - lo) / cs)
-    const double margin = 1e-9 * R + 64.0 * DBL_EPSILON * extent;
-    P.half_skin = 0.5 * skin * R;
-    P.half_skin2 = P.half_skin * P.half_skin;
-    P.hs_fluid = P.half_skin + margin;
-    P.hs_wall = margin;
     {
-        const double rlf = R + 2.0 * P.half_skin + margin, rlw = R + P.half_skin + margin;
-        P.rl2_fluid = rlf * rlf;
-        P.rl2_wall = rlw * rlw;
-        P.hs_list_fluid = 2.0 * P.half_skin + 2.0 * margin;
-        P.hs_list_wall = P.half_skin + 2.0 * margin;
-    }
-    // k cells span the reach of a point: R (as bounded by the FP32 square roots of the pencil
-    // widths, which run at most 2e-6 R above it), its own drift and a neighbour's drift
-    const double span = (R * 1.00001 + 2.0 * P.hs_fluid) * (1.0 + 1e-9);
-    int64_t ncells = 1;
-    for (int d = 0; d < 3; d++) {
-        const double cs = span / (d == 0 ? xdiv : div);
-        P.cs[d] = cs;
-        P.inv_cs[d] = 1.0 / cs;
-        if (d < nd) {
-            P.lo[d] = lo[d] - cs;
-            int64_t n = (int64_t)std::ceil((hi[d] + cs - P.lo[d]) * P.inv_cs[d]) + 1;
-            if (n < 1) n = 1;
-            if (n > (1 << 20)) {
-                delete h;
-                return fail(nullptr, TPSPH_ERR_ARGUMENT, "domain box too large for the cell grid");
-            }
-            P.n[d] = (int)n;
-        } else {
-            P.lo[d] = 0.0;
-            P.n[d] = 1;
+        const int rc = derive_params(params, h->nf, h->nw, lo, hi, h->P, h->nbricks);
+        if (rc != TPSPH_OK) {
+            delete h;
+            return rc;
         }
-        ncells *= P.n[d];
     }
-    if (ncells > (int64_t)INT32_MAX / 4) {
-        delete h;
-        return fail(nullptr, TPSPH_ERR_ARGUMENT, "cell grid too large (domain / smoothing_length)");
-    }
-    P.ncells = (int)ncells;
-    {
-        // bricks of 3 (3D) or 8 (2D) cell spans per side: about 3.1 R (8.3 R) wide, at
-        // h = 1.2 dx about 420 (400) targets and a union of about 1700 (600) particles
-        const int per = nd == 3 ? 3 : 8;
-        int64_t nb = 1;
-        for (int d = 0; d < 3; d++) {
-            P.bc[d] = d < nd ? per * (d == 0 ? xdiv : div) : 1;
-            P.nb[d] = (P.n[d] + P.bc[d] - 1) / P.bc[d];
-            nb *= P.nb[d];
-        }
-        h->nbricks = (int)nb;
-        // union rows: cells within the list radius of the brick box (grown by the margin)
-        const double rl = R + 2.0 * P.half_skin + margin;
-        int rows = 1;
-        for (int d = 0; d < 3; d++) {
-            P.ur[d] = (d >= 1 && d < nd) ? (int)std::ceil(rl / P.cs[d]) + 1 : 0;
-            if (d >= 1 && d < nd) rows *= P.bc[d] + 2 * P.ur[d];
-        }
-        P.maxr = rows;
-        P.rl_union2 = (rl + margin) * (rl + margin);
-        P.hs_union = margin;
-    }
+    DevParams &P = h->P;
+    const int64_t ncells = P.ncells;
     h->search = params->pair_search;
     if (h->search == TPSPH_SEARCH_AUTO)
         h->search = P.half_skin > 0.0 ? TPSPH_SEARCH_LISTS : TPSPH_SEARCH_CELLS;
@@ -2217,57 +2403,52 @@ int tpsph_neighbors_csr(tpsph_handle *h, const double *u_ode, int32_t pair, int6
         col_start = h->f_start; cx = h->xs; cy = h->ys; cz = h->zs; col_perm = h->f_perm;
         hs = h->P.hs_fluid;
     }
-    int *counts = nullptr;
-    int64_t *doff = nullptr;
-    int *didx = nullptr;
-    TRY(dalloc(h, &counts, (size_t)nrows));
-    std::vector<int> hc((size_t)nrows);
-    std::vector<int64_t> ho((size_t)nrows + 1, 0);
-    int rc = TPSPH_OK;
-    if (nrows > 0) {
-        if (nd == 2)
-            hipLaunchKernelGGL((k_nbr<2, false>), dim3(nblocks(nrows)), dim3(kBlock), 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, counts, nullptr, nullptr);
+    auto launch = [&](bool fill, int *counts, const int64_t *doff, int *didx) {
+        const dim3 g(nblocks(nrows)), b(kBlock);
+        if (nd == 2 && !fill)
+            hipLaunchKernelGGL((k_nbr<2, false>), g, b, 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, counts, nullptr, nullptr);
+        else if (nd == 2)
+            hipLaunchKernelGGL((k_nbr<2, true>), g, b, 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, nullptr, doff, didx);
+        else if (!fill)
+            hipLaunchKernelGGL((k_nbr<3, false>), g, b, 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, counts, nullptr, nullptr);
         else
-            hipLaunchKernelGGL((k_nbr<3, false>), dim3(nblocks(nrows)), dim3(kBlock), 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, counts, nullptr, nullptr);
-        rc = launch_check(h, "k_nbr<count>");
-        if (rc == TPSPH_OK) {
-            hipError_t e = hipMemcpyAsync(hc.data(), counts, (size_t)nrows * sizeof(int), hipMemcpyDeviceToHost, h->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-            if (e != hipSuccess) rc = fail(h, TPSPH_ERR_HIP, std::string("neighbour counts: ") + hipGetErrorString(e));
-        }
+            hipLaunchKernelGGL((k_nbr<3, true>), g, b, 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, nullptr, doff, didx);
+    };
+    return emit_csr(h, nrows, launch, offsets, idx, capacity);
+}
+
+int tpsph_kick_pairs_csr(tpsph_handle *h, int32_t pair, int64_t *offsets, int32_t *idx,
+                         int64_t capacity)
+{
+    if (!h) return fail(nullptr, TPSPH_ERR_ARGUMENT, "handle is NULL");
+    if (pair < 0 || pair > 2) return fail(h, TPSPH_ERR_ARGUMENT, "unknown pair kind");
+    if (!offsets) return fail(h, TPSPH_ERR_ARGUMENT, "offsets must not be NULL");
+    if (!h->have_kick) return fail(h, TPSPH_ERR_STATE, "no kick has been evaluated yet");
+    const bool lists = h->search == TPSPH_SEARCH_LISTS;
+    const EvalPairsArgs a{pair, lists ? 1 : 0, h->bricks ? 1 : 0, h->ord, h->bstart, h->bntot,
+                          h->brow_gb, h->brow_lds, h->nbricks, h->nbr16, h->nbr, h->cnt,
+                          h->wnbr, h->wcnt, h->ls, h->anbr, h->acnt, h->lw, h->w_perm};
+    const int nrows = pair == TPSPH_PAIR_WALL_FLUID ? h->nw : h->nf;
+    if ((pair != TPSPH_PAIR_FLUID_FLUID && h->nw == 0) || h->nf == 0) {
+        // no pairs: all offsets zero
+        std::vector<int64_t> zero((size_t)nrows + 1, 0);
+        HIP_TRY(h, hipMemcpy(offsets, zero.data(), zero.size() * sizeof(int64_t), hipMemcpyDefault));
+        return TPSPH_OK;
     }
-    for (int r = 0; r < nrows; r++) ho[(size_t)r + 1] = ho[(size_t)r] + hc[(size_t)r];
-    const int64_t total = ho[(size_t)nrows];
-    if (rc == TPSPH_OK) {
-        hipError_t e = hipMemcpy(offsets, ho.data(), ho.size() * sizeof(int64_t), hipMemcpyDefault);
-        if (e != hipSuccess) rc = fail(h, TPSPH_ERR_HIP, std::string("offsets: ") + hipGetErrorString(e));
-    }
-    if (rc == TPSPH_OK && idx != nullptr && capacity < total)
-        rc = fail(h, TPSPH_ERR_CAPACITY, "idx capacity is smaller than the number of pairs");
-    if (rc == TPSPH_OK && idx != nullptr && total > 0) {
-        rc = dalloc(h, &doff, ho.size());
-        if (rc == TPSPH_OK) rc = dalloc(h, &didx, (size_t)total);
-        if (rc == TPSPH_OK) {
-            hipError_t e = hipMemcpy(doff, ho.data(), ho.size() * sizeof(int64_t), hipMemcpyHostToDevice);
-            if (e != hipSuccess) rc = fail(h, TPSPH_ERR_HIP, "offsets upload failed");
-        }
-        if (rc == TPSPH_OK) {
-            if (nd == 2)
-                hipLaunchKernelGGL((k_nbr<2, true>), dim3(nblocks(nrows)), dim3(kBlock), 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, nullptr, doff, didx);
-            else
-                hipLaunchKernelGGL((k_nbr<3, true>), dim3(nblocks(nrows)), dim3(kBlock), 0, h->stream, h->P, nrows, rx, ry, rz, row_perm, col_start, cx, cy, cz, col_perm, hs, nullptr, doff, didx);
-            rc = launch_check(h, "k_nbr<fill>");
-        }
-        if (rc == TPSPH_OK) {
-            hipError_t e = hipMemcpyAsync(idx, didx, (size_t)total * sizeof(int), hipMemcpyDefault, h->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-            if (e != hipSuccess) rc = fail(h, TPSPH_ERR_HIP, std::string("neighbour lists: ") + hipGetErrorString(e));
-        }
-    }
-    (void)hipFree(counts);
-    (void)hipFree(doff);
-    (void)hipFree(didx);
-    return rc;
+    auto launch = [&](bool fill, int *counts, const int64_t *doff, int *didx) {
+        const dim3 g(nblocks(nrows)), b(kBlock);
+        const FluidView F = h->fluid_view();
+        const WallView W = h->wall_view();
+        if (h->nd == 2 && !fill)
+            hipLaunchKernelGGL((k_eval_pairs<2, false>), g, b, 0, h->stream, h->P, F, W, a, counts, nullptr, nullptr);
+        else if (h->nd == 2)
+            hipLaunchKernelGGL((k_eval_pairs<2, true>), g, b, 0, h->stream, h->P, F, W, a, nullptr, doff, didx);
+        else if (!fill)
+            hipLaunchKernelGGL((k_eval_pairs<3, false>), g, b, 0, h->stream, h->P, F, W, a, counts, nullptr, nullptr);
+        else
+            hipLaunchKernelGGL((k_eval_pairs<3, true>), g, b, 0, h->stream, h->P, F, W, a, nullptr, doff, didx);
+    };
+    return emit_csr(h, nrows, launch, offsets, idx, capacity);
 }
 
 int tpsph_rk_stage(tpsph_handle *h, double *y, double *tmp, const double *k, double a,
@@ -2341,6 +2522,8 @@ int tpsph_solve_ck2n54(tpsph_handle *h, double *v_ode, double *u_ode, double t0,
             gathered = true;
         }
     }
+    // the state has advanced past the last kick: its pressures belong to no current state
+    h->have_kick = false;
     if (is_host(h)) {
         HIP_TRY(h, hipMemcpyAsync(v_ode, v, v_len(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIP_TRY(h, hipMemcpyAsync(u_ode, u, u_len(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));

@@ -28,6 +28,9 @@ SEARCH_LISTS = 1
 SEARCH_CELLS = 2
 PAIR_SEARCH = {"auto": SEARCH_AUTO, "lists": SEARCH_LISTS, "cells": SEARCH_CELLS}
 
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
 TIME_CELL_LIST = 0
 TIME_GATHER = 1
 TIME_WALL = 2
@@ -38,7 +41,7 @@ TIMED_KERNELS = {"cell_list": 0, "gather": 1, "wall": 2, "fluid": 3, "stage": 4}
 EXPORTED_SYMBOLS = (
     "tpsph_abi_version", "tpsph_create", "tpsph_kick", "tpsph_drift", "tpsph_get_pressures",
     "tpsph_get_sound_speed",
-    "tpsph_neighbors_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54", "tpsph_synchronize",
+    "tpsph_neighbors_csr", "tpsph_kick_pairs_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54", "tpsph_synchronize",
     "tpsph_enable_timing", "tpsph_kernel_time", "tpsph_last_error", "tpsph_destroy",
 )
 
@@ -72,7 +75,7 @@ class TpsphParams(C.Structure):
         ("domain_max", C.c_double * 3),
         ("skin", C.c_double),
         ("adaptive_sound_speed", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("precision", C.c_int32),
         ("mach_number_target", C.c_double),
         ("min_sound_speed", C.c_double),
         ("max_sound_speed", C.c_double),
@@ -118,6 +121,7 @@ def load(path: str | None = None):
     lib.tpsph_get_pressures.argtypes = [P, P, P, P]
     lib.tpsph_get_sound_speed.argtypes = [P, dp]
     lib.tpsph_neighbors_csr.argtypes = [P, P, C.c_int32, P, P, C.c_int64]
+    lib.tpsph_kick_pairs_csr.argtypes = [P, C.c_int32, P, P, C.c_int64]
     lib.tpsph_rk_stage.argtypes = [P, P, P, P, C.c_double, C.c_double, C.c_double, C.c_int64,
                                    C.c_int32]
     lib.tpsph_solve_ck2n54.argtypes = [P, P, P, C.c_double, C.c_double, C.c_int64]
@@ -129,7 +133,7 @@ def load(path: str | None = None):
     lib.tpsph_destroy.argtypes = [P]
     lib.tpsph_destroy.restype = None
     for name in ("tpsph_create", "tpsph_kick", "tpsph_drift", "tpsph_get_pressures",
-                 "tpsph_neighbors_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54",
+                 "tpsph_neighbors_csr", "tpsph_kick_pairs_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54",
                  "tpsph_synchronize", "tpsph_enable_timing", "tpsph_kernel_time"):
         getattr(lib, name).restype = C.c_int
     _lib = lib

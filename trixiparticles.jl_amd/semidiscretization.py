@@ -28,6 +28,7 @@ from .schemes import (StateEquationAdaptiveCole, ViscosityAdami, WallBoundarySys
                       WeaklyCompressibleSPHSystem)
 
 __all__ = ["Semidiscretization", "semidiscretize", "DynamicalODEProblem", "kick", "drift",
+           "kick_pairs",
            "calculate_dt", "solve_tstops",
            "CarpenterKennedy2N54", "solve", "CK2N54_A", "CK2N54_B", "CK2N54_C"]
 
@@ -301,6 +302,36 @@ class Semidiscretization:
                                              offsets.ctypes.data, idx.ctypes.data, idx.size),
                      self.handle)
         return offsets, idx[: int(offsets[-1])]
+
+
+def _csr_from_library(semi, call, nrows):
+    """Run a CSR export of the library twice (offsets, then indices) into host arrays."""
+    if semi.backend == "hip":
+        import torch
+        dev = f"cuda:{semi.device}"
+        off_d = torch.zeros(nrows + 1, dtype=torch.int64, device=dev)
+        native.check(call(off_d.data_ptr(), None, 0), semi.handle)
+        offsets = off_d.cpu().numpy()
+        idx_d = torch.zeros(max(int(offsets[-1]), 1), dtype=torch.int32, device=dev)
+        native.check(call(off_d.data_ptr(), idx_d.data_ptr(), idx_d.numel()), semi.handle)
+        return offsets, idx_d.cpu().numpy()[: int(offsets[-1])]
+    offsets = np.zeros(nrows + 1, dtype=np.int64)
+    native.check(call(offsets.ctypes.data, None, 0), semi.handle)
+    idx = np.zeros(max(int(offsets[-1]), 1), dtype=np.int32)
+    native.check(call(offsets.ctypes.data, idx.ctypes.data, idx.size), semi.handle)
+    return offsets, idx[: int(offsets[-1])]
+
+
+def kick_pairs(semi, pair=native.PAIR_FLUID_FLUID):
+    """The pairs the kernels of the last ``kick`` summed, as host CSR ``(offsets, indices)``
+    (``tpsph_kick_pairs_csr``): for the parity tests of the pair search."""
+    lib = native.load()
+    nf = semi.fluid.n_particles
+    nw = semi.wall.n_particles if semi.wall is not None else 0
+    nrows = nw if pair == native.PAIR_WALL_FLUID else nf
+    return _csr_from_library(
+        semi, lambda off, idx, cap: lib.tpsph_kick_pairs_csr(semi.handle, pair, off, idx, cap),
+        nrows)
 
 
 def kick(dv_ode, v_ode, u_ode, p, t):
