@@ -263,6 +263,27 @@ int tpsph_rk_stage(tpsph_handle *h, double *y, double *tmp, const double *k, dou
 int tpsph_solve_ck2n54(tpsph_handle *h, double *v_ode, double *u_ode, double t0, double dt,
                        int64_t nsteps);
 
+/* kick! without any device allocation, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops).  No counterpart in the reference: it is tpsph_create +
+ * tpsph_kick + tpsph_destroy in one call, with every intermediate array (cell lists,
+ * cell-sorted records, wall pressures) in a fixed 64 MiB device array of the library, one
+ * per GPU, held until the process ends.  Results equal those of tpsph_kick on a handle
+ * created with the same params and pair_search TPSPH_SEARCH_CELLS.
+ *   - device pointers only (params.memory == TPSPH_MEM_DEVICE), asynchronous on `stream`;
+ *   - params.domain_min < domain_max on every axis (the grid box; it is not derived from the
+ *     wall coordinates, which would need a copy to the host);
+ *   - StateEquationCole, Monaghan viscosity, Molteni-Colagrossi or no diffusion, walls
+ *     without viscosity, FP64;
+ *   - sizes whose arrays exceed the 64 MiB (about 120 bytes per fluid and 100 per wall
+ *     particle plus 17 per cell) are rejected with TPSPH_ERR_ARGUMENT;
+ *   - calls on one GPU share its array: calls that overlap on one GPU (different streams or
+ *     threads) race and must be serialised by the caller.
+ * Errors go to tpsph_last_error(NULL). */
+int tpsph_kick_noalloc(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
+                       const double *fluid_mass, const double *wall_coordinates,
+                       const double *wall_mass, double *dv_ode, const double *v_ode,
+                       const double *u_ode, double t, void *stream);
+
 /* Wait for all work queued on the handle's stream. */
 int tpsph_synchronize(tpsph_handle *h);
 

@@ -119,3 +119,28 @@ def test_system_constructors_reject_out_of_scope_options(tp):
     with pytest.raises(NotImplementedError):
         tp.BoundaryModelDummyParticles([1000.0], [1.0], tp.AdamiPressureExtrapolation(), k,
                                        0.12, state_equation=se, viscosity=object())
+
+
+def test_noalloc_kick_rejects_sizes_beyond_its_scratch(tp):
+    """tpsph_kick_noalloc measures its arrays before touching the device: 2^25 fluid particles
+    need far more than its fixed 64 MiB and are rejected with the limit in the message."""
+    lib = tp.native.load()
+    p = tp.native.TpsphParams()
+    p.ndims = 3
+    p.memory = tp.native.TPSPH_MEM_DEVICE
+    p.smoothing_length = 0.012
+    p.reference_density = 1000.0
+    p.sound_speed = 48.5
+    p.exponent = 7.0
+    for d in range(3):
+        p.domain_min[d], p.domain_max[d] = 0.0, 1.0
+    dummy = C.c_void_p(8)  # never dereferenced: the size check comes first
+    rc = lib.tpsph_kick_noalloc(C.byref(p), 1 << 25, 0, dummy, None, None, dummy, dummy, dummy,
+                                0.0, None)
+    assert rc == tp.native.TPSPH_ERR_ARGUMENT
+    msg = lib.tpsph_last_error(None).decode()
+    assert "limit is 67108864 bytes" in msg, msg
+    p.domain_max[0] = 0.0  # no grid box
+    rc = lib.tpsph_kick_noalloc(C.byref(p), 10, 0, dummy, None, None, dummy, dummy, dummy, 0.0,
+                                None)
+    assert rc == tp.native.TPSPH_ERR_ARGUMENT and "grid box" in lib.tpsph_last_error(None).decode()

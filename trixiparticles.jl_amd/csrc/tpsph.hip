@@ -34,6 +34,8 @@
 // 256-byte rows per wavefront) and applies the same exact test to every entry.
 #include "tpsph_internal.h"
 
+#include <mutex>
+
 namespace tpsph {
 thread_local std::string g_create_error;
 }  // namespace tpsph
@@ -2015,6 +2017,134 @@ int emit_csr(tpsph_handle *h, int nrows, Launch launch, int64_t *offsets, int32_
     return rc;
 }
 
+// ------------------------------------------------------------------------------------------
+// Kick without allocation (tpsph_kick_noalloc): every intermediate array of one kick lives in
+// a file-scope device array, `scratch`, laid out anew on each call.  Host code reaches the
+// array through the address hipGetSymbolAddress returns on each device (the symbol itself is
+// a host-side stand-in and never a device address).  Counters and histograms are cleared
+// by kernels, so the call records no memset (it can be captured in a HIP graph).
+// ------------------------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(64) << 20;  // 64 MiB per device
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+__global__ void k_fill_i32(int *p, int value, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+__global__ void k_fill_u8(unsigned char *p, unsigned char value, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+__global__ void k_fill_f64(double *p, double value, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// device address of `scratch` on the current device, looked up once per device
+int scratch_base(unsigned char **base)
+{
+    static std::mutex mu;
+    static unsigned char *addr[kMaxDevices] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return fail(nullptr, TPSPH_ERR_HIP, "tpsph_kick_noalloc: no current HIP device");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(nullptr, TPSPH_ERR_HIP,
+                        std::string("tpsph_kick_noalloc: hipGetSymbolAddress(scratch): ") +
+                            hipGetErrorString(e));
+        addr[dev] = static_cast<unsigned char *>(p);
+    }
+    *base = addr[dev];
+    return TPSPH_OK;
+}
+
+// Bump allocator over the scratch array (256-byte aligned regions).
+struct ScratchLayout {
+    unsigned char *base;
+    size_t used = 0;
+    template <typename T>
+    T *take(size_t count)
+    {
+        T *p = reinterpret_cast<T *>(base ? base + used : nullptr);
+        used += (std::max<size_t>(count, 1) * sizeof(T) + 255) / 256 * 256;
+        return p;
+    }
+};
+
+// The kick of kick_impl with the cell walk (TPSPH_SEARCH_CELLS), fluid and wall cell lists
+// built from scratch, on a handle whose arrays all point into the scratch array.  With
+// base == nullptr only the layout is measured (h->... stay null, nothing is launched).
+template <int ND>
+int noalloc_kick(tpsph_handle *h, unsigned char *base, size_t *bytes, const double *fluid_mass,
+                 const double *wall_x, const double *wall_mass, double *dv, const double *v,
+                 const double *u)
+{
+    const int nf = h->nf, nw = h->nw;
+    const int nc = h->P.ncells;
+    const int tiles = (nc + 1 + kScanTile - 1) / kScanTile;
+    ScratchLayout L{base};
+    h->flag = L.take<int>(1);
+    h->scan_tiles = L.take<int>((size_t)tiles + 1);
+    h->f_cell = L.take<int>(nf);
+    h->f_rank = L.take<int>(nf);
+    h->f_perm = L.take<int>(nf);
+    h->iperm = L.take<int>(nf);
+    h->f_count = L.take<int>((size_t)nc + 1);
+    h->f_start = L.take<int>((size_t)nc + 1);
+    h->xs = L.take<double>((size_t)nf + kPad);
+    h->ys = L.take<double>((size_t)nf + kPad);
+    h->zs = L.take<double>((size_t)nf + kPad);
+    h->snap = L.take<double>((size_t)ND * nf);
+    h->rec = L.take<double4>(2 * (size_t)nf);
+    h->vol = L.take<double2>(nf);
+    int *w_cell = L.take<int>(nw), *w_rank = L.take<int>(nw), *w_count = L.take<int>((size_t)nc + 1);
+    h->w_start = L.take<int>((size_t)nc + 1);
+    h->w_perm = L.take<int>(nw);
+    h->wx = L.take<double>((size_t)nw + kPad);
+    h->wy = L.take<double>((size_t)nw + kPad);
+    h->wz = L.take<double>((size_t)nw + kPad);
+    h->wpos = L.take<double4>(nw);
+    h->wpv = L.take<double2>(nw);
+    h->pw = L.take<double>(nw);
+    h->rhow = L.take<double>(nw);
+    h->wall_near = L.take<unsigned char>(nc);
+    *bytes = L.used;
+    if (!base) return TPSPH_OK;
+    hipStream_t st = h->stream;
+    const double nan = __builtin_nan("");
+    // what must start at zero (or NaN): histograms, the wall-near flags, the SoA padding
+    hipLaunchKernelGGL((k_fill_i32), dim3(gate_blocks(nc + 1)), dim3(kBlock), 0, st, h->f_count, 0, nc + 1);
+    hipLaunchKernelGGL((k_fill_i32), dim3(gate_blocks(nc + 1)), dim3(kBlock), 0, st, w_count, 0, nc + 1);
+    hipLaunchKernelGGL((k_fill_u8), dim3(gate_blocks(nc)), dim3(kBlock), 0, st, h->wall_near, (unsigned char)0, nc);
+    for (double *a : {h->xs, h->ys, h->zs})
+        hipLaunchKernelGGL((k_fill_f64), dim3(gate_blocks(nf + kPad)), dim3(kBlock), 0, st, a, nan, nf + kPad);
+    for (double *a : {h->wx, h->wy, h->wz})
+        hipLaunchKernelGGL((k_fill_f64), dim3(gate_blocks(nw + kPad)), dim3(kBlock), 0, st, a, nan, nw + kPad);
+    hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, st, h->flag, 1);
+    TRY(launch_check(h, "tpsph_kick_noalloc clearing"));
+    // static walls: cell list, sorted positions and masses, cells near walls
+    if (nw > 0) {
+        TRY(build_cells<ND>(h, nullptr, nw, wall_x, w_cell, w_rank, w_count, h->w_start, h->w_perm));
+        hipLaunchKernelGGL((k_positions<ND>), dim3(nblocks(nw)), dim3(kBlock), 0, st, nw, wall_x,
+                           h->w_perm, h->wx, h->wy, ND == 3 ? h->wz : nullptr, h->wpos, wall_mass);
+        hipLaunchKernelGGL((k_wall_near<ND>), dim3(nblocks(nw)), dim3(kBlock), 0, st, h->P, nw, h->wpos, h->wall_near);
+        TRY(launch_check(h, "tpsph_kick_noalloc walls"));
+    }
+    h->f_mass = const_cast<double *>(fluid_mass);
+    return kick_impl<ND>(h, dv, v, u, false);
+}
+
 }  // namespace
 
 // ==========================================================================================
@@ -2529,6 +2659,53 @@ int tpsph_solve_ck2n54(tpsph_handle *h, double *v_ode, double *u_ode, double t0,
         HIP_TRY(h, hipMemcpyAsync(u_ode, u, u_len(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIP_TRY(h, hipStreamSynchronize(h->stream));
     }
+    return TPSPH_OK;
+}
+
+int tpsph_kick_noalloc(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
+                       const double *fluid_mass, const double *wall_coordinates,
+                       const double *wall_mass, double *dv_ode, const double *v_ode,
+                       const double *u_ode, double t, void *stream)
+{
+    (void)t;
+    TRY(validate(params, n_fluid, n_wall, fluid_mass, wall_coordinates, wall_mass));
+    auto bad = [](const std::string &m) { return fail(nullptr, TPSPH_ERR_ARGUMENT, "tpsph_kick_noalloc: " + m); };
+    if (params->memory != TPSPH_MEM_DEVICE) return bad("needs device pointers (TPSPH_MEM_DEVICE)");
+    if (params->adaptive_sound_speed || params->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO ||
+        params->wall_viscosity || params->precision != TPSPH_PRECISION_FP64)
+        return bad("supports StateEquationCole, Molteni-Colagrossi diffusion, walls without "
+                   "viscosity and FP64");
+    if (n_fluid > 0 && (!dv_ode || !v_ode || !u_ode))
+        return bad("dv_ode, v_ode and u_ode must not be NULL");
+    double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < params->ndims; d++) {
+        if (!(params->domain_min[d] < params->domain_max[d]))
+            return bad("needs the grid box (domain_min < domain_max on every axis)");
+        lo[d] = params->domain_min[d];
+        hi[d] = params->domain_max[d];
+    }
+    // a handle on the stack whose arrays point into the scratch array; it owns nothing
+    tpsph_handle h;
+    h.params = *params;
+    h.nd = params->ndims;
+    h.nf = (int)n_fluid;
+    h.nw = (int)n_wall;
+    h.stream = (hipStream_t)stream;
+    h.search = TPSPH_SEARCH_CELLS;
+    TRY(derive_params(params, h.nf, h.nw, lo, hi, h.P, h.nbricks));
+    size_t bytes = 0;
+    TRY(h.nd == 2 ? noalloc_kick<2>(&h, nullptr, &bytes, fluid_mass, wall_coordinates, wall_mass, dv_ode, v_ode, u_ode)
+                  : noalloc_kick<3>(&h, nullptr, &bytes, fluid_mass, wall_coordinates, wall_mass, dv_ode, v_ode, u_ode));
+    if (bytes > kScratchBytes)
+        return bad("needs " + std::to_string(bytes) + " bytes of scratch for " +
+                   std::to_string(n_fluid) + " fluid, " + std::to_string(n_wall) +
+                   " wall particles and " + std::to_string(h.P.ncells) +
+                   " cells; the limit is " + std::to_string(kScratchBytes) + " bytes");
+    unsigned char *base = nullptr;
+    TRY(scratch_base(&base));
+    const int rc = h.nd == 2 ? noalloc_kick<2>(&h, base, &bytes, fluid_mass, wall_coordinates, wall_mass, dv_ode, v_ode, u_ode)
+                             : noalloc_kick<3>(&h, base, &bytes, fluid_mass, wall_coordinates, wall_mass, dv_ode, v_ode, u_ode);
+    if (rc != TPSPH_OK) return fail(nullptr, rc, h.last_error);
     return TPSPH_OK;
 }
 
