@@ -284,6 +284,17 @@ int tpsph_kick_noalloc(const tpsph_params *params, int64_t n_fluid, int64_t n_wa
                        const double *wall_mass, double *dv_ode, const double *v_ode,
                        const double *u_ode, double t, void *stream);
 
+/* Integrals of the fluid state (v_ode, u_ode) in FP64, summed in a fixed order on the
+ * device (deterministic), the global reductions a run reports beside its state:
+ *   out[0]    total mass sum m_a
+ *   out[1..3] momentum sum m_a v_a (out[3] = 0 in 2D)
+ *   out[4]    kinetic energy sum m_a |v_a|^2 / 2
+ *   out[5]    potential energy -sum m_a g.x_a (g = params.acceleration)
+ *   out[6]    fluid volume sum m_a / rho_a
+ *   out[7]    sum rho_a
+ * out holds 8 doubles (host or device memory).  Synchronises the handle's stream. */
+int tpsph_integrals(tpsph_handle *h, const double *v_ode, const double *u_ode, double *out);
+
 /* Wait for all work queued on the handle's stream. */
 int tpsph_synchronize(tpsph_handle *h);
 

@@ -65,115 +65,6 @@ __global__ void k_cells(const int *flag, DevParams P, int n, const double *__res
     }
 }
 
-__device__ __forceinline__ int block_exclusive_scan(int value, int *s, int &total)
-{
-    s[threadIdx.x] = value;
-    __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-        int t = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
-    }
-    total = s[kBlock - 1];
-    int excl = s[threadIdx.x] - value;
-    __syncthreads();
-    return excl;
-}
-
-__global__ void __launch_bounds__(kBlock)
-    k_scan_tiles(const int *flag, const int *in, int *out, int *tile_sums, int n)
-{
-    if (skip(flag)) return;
-    __shared__ int s[kBlock];
-    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    int v[kScanItems];
-    int sum = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
-        v[k] = (base + k < n) ? in[base + k] : 0;
-        sum += v[k];
-    }
-    int total;
-    int run = block_exclusive_scan(sum, s, total);
-#pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
-        if (base + k < n) out[base + k] = run;
-        run += v[k];
-    }
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
-}
-
-__global__ void __launch_bounds__(kBlock) k_scan_single(const int *flag, int *data, int n)
-{
-    if (skip(flag)) return;
-    __shared__ int s[kBlock];
-    int carry = 0;
-    for (int chunk = 0; chunk < n; chunk += kScanTile) {
-        int base = chunk + threadIdx.x * kScanItems;
-        int v[kScanItems];
-        int sum = 0;
-#pragma unroll
-        for (int k = 0; k < kScanItems; k++) {
-            v[k] = (base + k < n) ? data[base + k] : 0;
-            sum += v[k];
-        }
-        int total;
-        int run = carry + block_exclusive_scan(sum, s, total);
-#pragma unroll
-        for (int k = 0; k < kScanItems; k++) {
-            if (base + k < n) data[base + k] = run;
-            run += v[k];
-        }
-        carry += total;
-    }
-}
-
-__global__ void k_scan_add(const int *flag, int *out, const int *tile_sums, int n)
-{
-    if (skip(flag)) return;
-    int i = blockIdx.x * kScanTile + threadIdx.x;
-    int add = tile_sums[blockIdx.x];
-    for (int k = 0; k < kScanItems; k++, i += kBlock)
-        if (i < n) out[i] += add;
-}
-
-// Also clears the histogram again: the counts are zero between builds, so the next build
-// needs no separate clearing pass.
-__global__ void k_scatter(const int *flag, int n, const int *__restrict__ cell,
-                          const int *__restrict__ rank, const int *__restrict__ start,
-                          int *__restrict__ perm, int *__restrict__ count)
-{
-    if (skip(flag)) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int c = cell[i];
-        perm[start[c] + rank[i]] = i;
-        count[c] = 0;
-    }
-}
-
-// Order every cell's members by particle index: the result is the stable counting sort,
-// independent of the order in which the histogram atomics were served.
-__global__ void k_sort_cells(const int *flag, int ncells, const int *__restrict__ start,
-                             int *__restrict__ perm)
-{
-    if (skip(flag)) return;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
-        int b = start[c], e = start[c + 1];
-        for (int k = b + 1; k < e; k++) {
-            int key = perm[k];
-            int j = k - 1;
-            while (j >= b && perm[j] > key) {
-                perm[j + 1] = perm[j];
-                j--;
-            }
-            perm[j + 1] = key;
-        }
-    }
-}
-
-__global__ void k_set_i32(int *p, int value) { *p = value; }
-
 // Processing order of the list kernels: the cell-sorted particles grouped by bricks of cells
 // (counted by k_gather<1>), so the targets of a block are a compact cluster and the union of
 // their neighbourhoods is small.  The order inside a brick follows the histogram atomics; it
@@ -341,73 +232,6 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-
-// Two-phase sweep over the neighbours of one target (see the file header).  lst points at
-// this lane's column of the block's LDS lists (stride kBlock).  Phase 1 tests four
-// candidates per iteration without branches: the arrays cx/cy/cz carry kPad NaN entries
-// past their end, so reads beyond a range stay in bounds and never pass the test, and every
-// candidate is written to the list while the count only advances on a hit.
-template <int ND, bool WALL, int CAP = kCap, int STRIDE = kBlock>
-__device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *__restrict__ start,
-                                                const double *__restrict__ cx,
-                                                const double *__restrict__ cy,
-                                                const double *__restrict__ cz, double hs,
-                                                bool active, const FluidView &F,
-                                                const WallView &Wv, int *lst, const double *xa,
-                                                const double *va, double rho_a, double p_a,
-                                                double inv_rho_a, double *acc, double &drho,
-                                                double m_a = 0.0, const double *ga = nullptr)
-{
-    Pencils<ND> it;
-    it.init(P, xa[0], xa[1], (ND == 3) ? xa[2] : 0.0, hs);
-    bool done = !active;
-    int j = 0, je = 0;
-    while (true) {
-        int cnt = 0;
-        // phase 1: exact radius test
-        while (!done && cnt <= CAP - 4) {
-            if (j >= je) {
-                if (!it.next(P, start, j, je)) done = true;
-                continue;
-            }
-            const uint32_t o = (uint32_t)j * 8u;
-            double d2[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const double p0 = xa[0] - ldo(cx, o + 8u * k);
-                const double p1 = xa[1] - ldo(cy, o + 8u * k);
-                const double p2 = (ND == 3) ? xa[2] - ldo(cz, o + 8u * k) : 0.0;
-                d2[k] = dist2<ND>(p0, p1, p2);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const bool hit = (j + k < je) & (d2[k] <= P.R2) & (d2[k] >= P.az2);
-                lst[cnt * STRIDE] = j + k;
-                cnt += hit ? 1 : 0;
-            }
-            j += 4;
-        }
-        // phase 2: pair terms on the dense list, loads one pair ahead
-        if (cnt > 0) {
-            if constexpr (WALL) {
-                WallNbr cur = load_wall(Wv, lst[0]);
-                for (int k = 0; k < cnt; k++) {
-                    const WallNbr nxt = load_wall(Wv, lst[min(k + 1, cnt - 1) * STRIDE]);
-                    pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, m_a);
-                    cur = nxt;
-                }
-            } else {
-                FluidNbr cur = load_fluid(F, lst[0]);
-                for (int k = 0; k < cnt; k++) {
-                    const FluidNbr nxt = load_fluid(F, lst[min(k + 1, cnt - 1) * STRIDE]);
-                    pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, ga);
-                    cur = nxt;
-                }
-            }
-        }
-        if (!__any(!done)) break;
-    }
-}
 
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
@@ -973,7 +797,9 @@ __device__ __forceinline__ FluidNbr lds_fluid(const BrickSmem<ND> &S, int e)
 // Fluid-fluid kernel over the bricks (one block of kBrickThreads per non-empty brick, taken
 // from a work counter that k_gather<0> resets).  Targets with wall candidates are completed
 // by k_fluid_walls.
-template <int ND>
+// GLOBAL_ONLY: only the bricks that do not stage, taken by blocks in turn (no work counter);
+// the mixed-precision pass (tpsph_mixed.hip) leaves them to this kernel.
+template <int ND, bool GLOBAL_ONLY = false>
 __global__ void __launch_bounds__(kBrickThreads)
     k_fluid_brick(int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
                   const int *__restrict__ bstart, const int *__restrict__ nel,
@@ -994,15 +820,19 @@ __global__ void __launch_bounds__(kBrickThreads)
     if (blockIdx.x == 0 && tid == 0) *flag = 0;  // the rebuild (if any) is done
     int *lst = reinterpret_cast<int *>(&S) + tid;  // fallback lists, stride kBrickThreads
     const int ne = *nne;
-    while (true) {
-        if (tid == 0) S.item = atomicAdd(work, 1);
-        __syncthreads();
-        const int item = S.item;
+    for (int turn = blockIdx.x;; turn += gridDim.x) {
+        int item = turn;
+        if (!GLOBAL_ONLY) {
+            if (tid == 0) S.item = atomicAdd(work, 1);
+            __syncthreads();
+            item = S.item;
+        }
         if (item >= ne) break;
         const int b = nel[item];
         const int t0 = bstart[b], t1 = bstart[b + 1];
         const int ntot = bntot[b];
         const bool staged = ntot <= kBrickCap - 1;
+        if (GLOBAL_ONLY && staged) continue;  // uniform per block; no LDS is touched
         if (staged) {
             for (int r = tid; r <= P.maxr; r += kBrickThreads) {
                 if (r < P.maxr) S.row_gb[r] = brow_gb[(size_t)b * P.maxr + r];
@@ -1427,6 +1257,60 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// FP64 integrals of the fluid state (tpsph_integrals): per-block partial sums over a fixed
+// assignment of particles to blocks and a fixed tree inside each block, then one block sums
+// the partials in block order -- the result does not depend on scheduling.
+// ------------------------------------------------------------------------------------------
+constexpr int kIntegrals = 8;
+constexpr int kIntegralBlocks = 1024;
+
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_integrals(int nf, const double *__restrict__ v, const double *__restrict__ u,
+                const double *__restrict__ m, DevParams P, double *__restrict__ part)
+{
+    __shared__ double s[kIntegrals][kBlock];
+    double acc[kIntegrals] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < nf; i += gridDim.x * kBlock) {
+        const double mi = m[i];
+        const double rho = v[(size_t)i * (ND + 1) + ND];
+        double v2 = 0.0, gx = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+            const double vd = v[(size_t)i * (ND + 1) + d];
+            acc[1 + d] += mi * vd;
+            v2 += vd * vd;
+            gx += P.g[d] * u[(size_t)i * ND + d];
+        }
+        acc[0] += mi;
+        acc[4] += 0.5 * mi * v2;
+        acc[5] -= mi * gx;
+        acc[6] += mi / rho;
+        acc[7] += rho;
+    }
+#pragma unroll
+    for (int k = 0; k < kIntegrals; k++) s[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+#pragma unroll
+            for (int k = 0; k < kIntegrals; k++) s[k][threadIdx.x] += s[k][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < kIntegrals) part[(size_t)blockIdx.x * kIntegrals + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_integrals_final(int nparts, const double *__restrict__ part,
+                                                            double *__restrict__ out)
+{
+    if ((int)threadIdx.x >= kIntegrals) return;
+    double acc = 0.0;
+    for (int b = 0; b < nparts; b++) acc += part[(size_t)b * kIntegrals + threadIdx.x];
+    out[threadIdx.x] = acc;
+}
+
 __global__ void k_rk_stage(int64_t n, double *__restrict__ y, double *__restrict__ tmp,
                            const double *__restrict__ k, double a, double b, double dt,
                            int first)
@@ -1644,14 +1528,6 @@ __global__ void k_record_pressure(int n, const double4 *__restrict__ rec, double
 namespace {
 
 
-int scan(tpsph_handle *h, const int *flag, const int *in, int *out, int n)
-{
-    int tiles = (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL((k_scan_tiles), dim3(tiles), dim3(kBlock), 0, h->stream, flag, in, out, h->scan_tiles, n);
-    hipLaunchKernelGGL((k_scan_single), dim3(1), dim3(kBlock), 0, h->stream, flag, h->scan_tiles, tiles);
-    hipLaunchKernelGGL((k_scan_add), dim3(tiles), dim3(kBlock), 0, h->stream, flag, out, h->scan_tiles, n);
-    return launch_check(h, "scan");
-}
 
 // Cell list of n points x (ND doubles per point, caller order) into start / perm.
 template <int ND>
@@ -1764,7 +1640,17 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
                                lists ? h->nbr : nullptr, lists ? h->cnt : nullptr, h->ls);
             TRY(launch_check(h, "k_antuono_gradient"));
         }
-        if (h->search == TPSPH_SEARCH_LISTS && h->bricks) {
+        if (h->search == TPSPH_SEARCH_LISTS && h->bricks && h->mixed) {
+            TRY(launch_fluid_brick_mixed(h, dv));
+            hipLaunchKernelGGL((k_fluid_brick<ND, true>), dim3(std::max(1, std::min(h->nbricks, h->n_cu))),
+                dim3(kBrickThreads), 0, h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
+                h->ord, h->bstart, h->nel, h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
+                h->nbr16, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
+            if (nw > 0)
+                hipLaunchKernelGGL((k_fluid_walls<ND>), dim3(gate_blocks(4 * (int64_t)nf)), dim3(kBlock), 0, h->stream,
+                    h->P, h->fluid_view(), h->wall_view(), h->ord, h->wtargets, h->counters + 2,
+                    h->wnbr, h->wcnt, h->ls, dv);
+        } else if (h->search == TPSPH_SEARCH_LISTS && h->bricks) {
             hipLaunchKernelGGL((k_fluid_brick<ND>), dim3(std::max(1, std::min(h->nbricks, h->n_cu))),
                 dim3(kBrickThreads), 0, h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
                 h->ord, h->bstart, h->nel, h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
@@ -1790,18 +1676,10 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
 
 }  // namespace
 
-int tpsph::kick_dispatch(tpsph_handle *h, double *dv, const double *v, const double *u,
-                         bool gathered)
-{
-    return h->nd == 2 ? kick_impl<2>(h, dv, v, u, gathered) : kick_impl<3>(h, dv, v, u, gathered);
-}
-
-namespace {
-
 // The device parameters of params for nf fluid and nw wall particles on the grid of the box
 // [lo, hi] (one padding cell per side): kernel and state-equation constants, cells, skin
 // radii and bricks.  Failures go to the create error (tpsph_last_error(NULL)).
-int derive_params(const tpsph_params *params, int nf, int nw, const double *lo,
+int tpsph::derive_params(const tpsph_params *params, int nf, int nw, const double *lo,
                   const double *hi, DevParams &P, int &nbricks)
 {
     const int nd = params->ndims;
@@ -1919,7 +1797,7 @@ int derive_params(const tpsph_params *params, int nf, int nw, const double *lo,
     return TPSPH_OK;
 }
 
-int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, const double *wx,
+int tpsph::validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, const double *wx,
              const double *wm)
 {
     auto bad = [](const std::string &m) { return fail(nullptr, TPSPH_ERR_ARGUMENT, m); };
@@ -1959,10 +1837,21 @@ int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, co
         p->use_density_diffusion > TPSPH_DIFFUSION_ANTUONO)
         return bad("use_density_diffusion must be one of enum tpsph_density_diffusion");
     if (!(p->skin < 1.0)) return bad("skin must be below 1 (a fraction of the compact support)");
+    if (p->precision != TPSPH_PRECISION_FP64 && p->precision != TPSPH_PRECISION_MIXED)
+        return bad("precision must be TPSPH_PRECISION_FP64 or TPSPH_PRECISION_MIXED");
     if (nf > 0 && !fm) return bad("fluid_mass is NULL");
     if (nw > 0 && (!wx || !wm)) return bad("wall_coordinates / wall_mass is NULL");
     return TPSPH_OK;
 }
+
+int tpsph::kick_dispatch(tpsph_handle *h, double *dv, const double *v, const double *u,
+                         bool gathered)
+{
+    return h->nd == 2 ? kick_impl<2>(h, dv, v, u, gathered) : kick_impl<3>(h, dv, v, u, gathered);
+}
+
+namespace {
+
 
 // CSR of nrows rows in the caller's order from a count launch and a fill launch of the same
 // kernel: launch(fill, counts, offsets, idx).  offsets / idx may be host or device memory.
@@ -2267,6 +2156,15 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     if ((rc = (expr)) != TPSPH_OK) return cleanup_fail(rc)
     const int nf = h->nf, nw = h->nw;
     const int tiles = (int)((ncells + 1 + kScanTile - 1) / kScanTile);
+    h->mixed = params->precision == TPSPH_PRECISION_MIXED;
+    if (h->mixed && !(h->search == TPSPH_SEARCH_LISTS && P.maxr <= kBrickMaxRows &&
+                      params->use_density_diffusion != TPSPH_DIFFUSION_ANTUONO)) {
+        delete h;
+        return fail(nullptr, TPSPH_ERR_ARGUMENT,
+                    "precision TPSPH_PRECISION_MIXED runs on the brick kernel: it needs the "
+                    "Verlet lists (skin > 0), the default brick sizes and no "
+                    "DensityDiffusionAntuono");
+    }
     CTRY(dalloc(h, &h->flag, 1));
     if (params->wall_viscosity && nw > 0) CTRY(dalloc(h, &h->wvel, 3 * (size_t)nw));
     if (params->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO && nf > 0) {
@@ -2709,6 +2607,33 @@ int tpsph_kick_noalloc(const tpsph_params *params, int64_t n_fluid, int64_t n_wa
     return TPSPH_OK;
 }
 
+int tpsph_integrals(tpsph_handle *h, const double *v_ode, const double *u_ode, double *out)
+{
+    if (!h) return fail(nullptr, TPSPH_ERR_ARGUMENT, "handle is NULL");
+    if (!out) return fail(h, TPSPH_ERR_ARGUMENT, "out must not be NULL");
+    if (h->nf > 0 && (!v_ode || !u_ode)) return fail(h, TPSPH_ERR_ARGUMENT, "v_ode and u_ode must not be NULL");
+    const double *v = v_ode, *u = u_ode;
+    if (is_host(h) && h->nf > 0) {
+        TRY(ensure_staging(h));
+        HIP_TRY(h, hipMemcpyAsync(h->st_v, v_ode, v_len(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(h, hipMemcpyAsync(h->st_u, u_ode, u_len(h) * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        v = h->st_v;
+        u = h->st_u;
+    }
+    if (!h->integ) TRY(dalloc(h, &h->integ, (size_t)(kIntegralBlocks + 1) * kIntegrals));
+    const int nb = std::max(1, std::min(nblocks(h->nf), kIntegralBlocks));
+    if (h->nd == 2)
+        hipLaunchKernelGGL((k_integrals<2>), dim3(nb), dim3(kBlock), 0, h->stream, h->nf, v, u, h->f_mass, h->P, h->integ);
+    else
+        hipLaunchKernelGGL((k_integrals<3>), dim3(nb), dim3(kBlock), 0, h->stream, h->nf, v, u, h->f_mass, h->P, h->integ);
+    double *res = h->integ + (size_t)kIntegralBlocks * kIntegrals;
+    hipLaunchKernelGGL((k_integrals_final), dim3(1), dim3(kBlock), 0, h->stream, nb, h->integ, res);
+    TRY(launch_check(h, "k_integrals"));
+    HIP_TRY(h, hipMemcpyAsync(out, res, kIntegrals * sizeof(double), hipMemcpyDefault, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return TPSPH_OK;
+}
+
 int tpsph_synchronize(tpsph_handle *h)
 {
     if (!h) return fail(nullptr, TPSPH_ERR_ARGUMENT, "handle is NULL");
@@ -2756,7 +2681,7 @@ void tpsph_destroy(tpsph_handle *h)
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,
                     h->wtargets, h->iperm, h->dyn, h->vmax_part, h->wvel, h->ddgrad,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
-                    h->sol_tu};
+                    h->sol_tu, h->integ};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);

@@ -488,6 +488,76 @@ __device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const Flui
     drho = fma(n.v.x * kdr, fma(P.dd2, rho_a - rho_b, rho_a * vr), drho);
 }
 
+// Two-phase sweep over the neighbours of one target: phase 1 runs the exact radius test over
+// the candidates of the pencils and appends hits to a per-lane list in LDS, phase 2 evaluates
+// the pair terms on the dense list; the phases alternate until every lane has exhausted its
+// pencils.  lst points at
+// this lane's column of the block's LDS lists (stride kBlock).  Phase 1 tests four
+// candidates per iteration without branches: the arrays cx/cy/cz carry kPad NaN entries
+// past their end, so reads beyond a range stay in bounds and never pass the test, and every
+// candidate is written to the list while the count only advances on a hit.
+template <int ND, bool WALL, int CAP = kCap, int STRIDE = kBlock>
+__device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *__restrict__ start,
+                                                const double *__restrict__ cx,
+                                                const double *__restrict__ cy,
+                                                const double *__restrict__ cz, double hs,
+                                                bool active, const FluidView &F,
+                                                const WallView &Wv, int *lst, const double *xa,
+                                                const double *va, double rho_a, double p_a,
+                                                double inv_rho_a, double *acc, double &drho,
+                                                double m_a = 0.0, const double *ga = nullptr)
+{
+    Pencils<ND> it;
+    it.init(P, xa[0], xa[1], (ND == 3) ? xa[2] : 0.0, hs);
+    bool done = !active;
+    int j = 0, je = 0;
+    while (true) {
+        int cnt = 0;
+        // phase 1: exact radius test
+        while (!done && cnt <= CAP - 4) {
+            if (j >= je) {
+                if (!it.next(P, start, j, je)) done = true;
+                continue;
+            }
+            const uint32_t o = (uint32_t)j * 8u;
+            double d2[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double p0 = xa[0] - ldo(cx, o + 8u * k);
+                const double p1 = xa[1] - ldo(cy, o + 8u * k);
+                const double p2 = (ND == 3) ? xa[2] - ldo(cz, o + 8u * k) : 0.0;
+                d2[k] = dist2<ND>(p0, p1, p2);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool hit = (j + k < je) & (d2[k] <= P.R2) & (d2[k] >= P.az2);
+                lst[cnt * STRIDE] = j + k;
+                cnt += hit ? 1 : 0;
+            }
+            j += 4;
+        }
+        // phase 2: pair terms on the dense list, loads one pair ahead
+        if (cnt > 0) {
+            if constexpr (WALL) {
+                WallNbr cur = load_wall(Wv, lst[0]);
+                for (int k = 0; k < cnt; k++) {
+                    const WallNbr nxt = load_wall(Wv, lst[min(k + 1, cnt - 1) * STRIDE]);
+                    pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, m_a);
+                    cur = nxt;
+                }
+            } else {
+                FluidNbr cur = load_fluid(F, lst[0]);
+                for (int k = 0; k < cnt; k++) {
+                    const FluidNbr nxt = load_fluid(F, lst[min(k + 1, cnt - 1) * STRIDE]);
+                    pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, ga);
+                    cur = nxt;
+                }
+            }
+        }
+        if (!__any(!done)) break;
+    }
+}
+
 // largest r in [0, n) with pre[r] <= i (pre ascending, pre[0] = 0)
 __device__ __forceinline__ int bsearch_le(const int *pre, int n, int i)
 {
@@ -515,6 +585,127 @@ __device__ __forceinline__ double cell_hi(const DevParams &P, int d, int c, doub
 
 inline int nblocks(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 inline int gate_blocks(int64_t n) { return std::max(1, std::min(nblocks(n), kGateBlocks)); }
+
+}  // namespace tpsph
+
+namespace tpsph {
+// ------------------------------------------------------------------------------------------
+// Counting-sort building blocks shared by the cell lists of the library's translation units
+// (each unit holds its own copy: they live in an unnamed namespace).  Every kernel is gated
+// by a rebuild flag (nullptr = always runs).
+// ------------------------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ int block_exclusive_scan(int value, int *s, int &total)
+{
+    s[threadIdx.x] = value;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        int t = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    total = s[kBlock - 1];
+    int excl = s[threadIdx.x] - value;
+    __syncthreads();
+    return excl;
+}
+
+[[maybe_unused]] __global__ void __launch_bounds__(kBlock)
+    k_scan_tiles(const int *flag, const int *in, int *out, int *tile_sums, int n)
+{
+    if (skip(flag)) return;
+    __shared__ int s[kBlock];
+    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        sum += v[k];
+    }
+    int total;
+    int run = block_exclusive_scan(sum, s, total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+[[maybe_unused]] __global__ void __launch_bounds__(kBlock) k_scan_single(const int *flag, int *data, int n)
+{
+    if (skip(flag)) return;
+    __shared__ int s[kBlock];
+    int carry = 0;
+    for (int chunk = 0; chunk < n; chunk += kScanTile) {
+        int base = chunk + threadIdx.x * kScanItems;
+        int v[kScanItems];
+        int sum = 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            v[k] = (base + k < n) ? data[base + k] : 0;
+            sum += v[k];
+        }
+        int total;
+        int run = carry + block_exclusive_scan(sum, s, total);
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) {
+            if (base + k < n) data[base + k] = run;
+            run += v[k];
+        }
+        carry += total;
+    }
+}
+
+[[maybe_unused]] __global__ void k_scan_add(const int *flag, int *out, const int *tile_sums, int n)
+{
+    if (skip(flag)) return;
+    int i = blockIdx.x * kScanTile + threadIdx.x;
+    int add = tile_sums[blockIdx.x];
+    for (int k = 0; k < kScanItems; k++, i += kBlock)
+        if (i < n) out[i] += add;
+}
+
+// Also clears the histogram again: the counts are zero between builds, so the next build
+// needs no separate clearing pass.
+[[maybe_unused]] __global__ void k_scatter(const int *flag, int n, const int *__restrict__ cell,
+                          const int *__restrict__ rank, const int *__restrict__ start,
+                          int *__restrict__ perm, int *__restrict__ count)
+{
+    if (skip(flag)) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c = cell[i];
+        perm[start[c] + rank[i]] = i;
+        count[c] = 0;
+    }
+}
+
+// Order every cell's members by particle index: the result is the stable counting sort,
+// independent of the order in which the histogram atomics were served.
+[[maybe_unused]] __global__ void k_sort_cells(const int *flag, int ncells, const int *__restrict__ start,
+                             int *__restrict__ perm)
+{
+    if (skip(flag)) return;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
+        int b = start[c], e = start[c + 1];
+        for (int k = b + 1; k < e; k++) {
+            int key = perm[k];
+            int j = k - 1;
+            while (j >= b && perm[j] > key) {
+                perm[j + 1] = perm[j];
+                j--;
+            }
+            perm[j + 1] = key;
+        }
+    }
+}
+
+[[maybe_unused]] __global__ void k_set_i32(int *p, int value) { *p = value; }
+
+}  // namespace
 
 }  // namespace tpsph
 
@@ -570,6 +761,8 @@ struct tpsph_handle {
     double *st_v = nullptr, *st_u = nullptr, *st_dv = nullptr, *st_du = nullptr;
     double *sol_dv = nullptr, *sol_tv = nullptr, *sol_tu = nullptr;
     bool have_kick = false;
+    bool mixed = false;  // TPSPH_PRECISION_MIXED: FP32 pair terms in the brick kernel
+    double *integ = nullptr;  // tpsph_integrals: block partials and the result
 
     bool timing = false;
     struct Rec {
@@ -656,6 +849,29 @@ struct TimedScope {
         if (b) (void)hipEventRecord(b, h->stream);
     }
 };
+
+// Argument checks of tpsph_create (the reference's ArgumentError checks); failures go to the
+// create error (tpsph_last_error(NULL)).
+int validate(const tpsph_params *p, int64_t nf, int64_t nw, const double *fm, const double *wx,
+             const double *wm);
+
+// The device parameters of params for nf fluid and nw wall particles on the grid of the box
+// [lo, hi]: kernel and state-equation constants, cells, skin radii and bricks.
+int derive_params(const tpsph_params *params, int nf, int nw, const double *lo,
+                  const double *hi, DevParams &P, int &nbricks);
+
+// the fluid-fluid pass of TPSPH_PRECISION_MIXED (tpsph_mixed.hip), in place of k_fluid_brick
+int launch_fluid_brick_mixed(tpsph_handle *h, double *dv);
+
+// Exclusive scan of n ints from in to out on the handle's stream (tile sums in h->scan_tiles).
+inline int scan(tpsph_handle *h, const int *flag, const int *in, int *out, int n)
+{
+    int tiles = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL((k_scan_tiles), dim3(tiles), dim3(kBlock), 0, h->stream, flag, in, out, h->scan_tiles, n);
+    hipLaunchKernelGGL((k_scan_single), dim3(1), dim3(kBlock), 0, h->stream, flag, h->scan_tiles, tiles);
+    hipLaunchKernelGGL((k_scan_add), dim3(tiles), dim3(kBlock), 0, h->stream, flag, out, h->scan_tiles, n);
+    return launch_check(h, "scan");
+}
 
 // gathered: the cell-sorted records already hold (v, u) (written by k_ck_stage_gather)
 int kick_dispatch(tpsph_handle *h, double *dv, const double *v, const double *u,

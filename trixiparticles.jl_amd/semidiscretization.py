@@ -64,6 +64,10 @@ class Semidiscretization:
     ``FullGridCellList`` box).  By default it is the bounding box of both systems, grown
     by ``domain_margin`` times its extent on every side.
 
+    ``precision``: "fp64" (the reference's Float64 throughout) or "mixed" (FP32 pair
+    arithmetic on brick-relative coordinates with compensated density-rate sums; FP64
+    state, pressures, wall sums, stage update and integrals).
+
     ``cell_divisions`` / ``x_cell_divisions`` (cells per compact support along y, z / x),
     ``skin`` (fraction of the compact support) and ``pair_search`` ("auto", "lists" or
     "cells") tune the device neighbour search; they change speed, never the
@@ -71,7 +75,7 @@ class Semidiscretization:
 
     def __init__(self, *systems, neighborhood_search=None, parallelization_backend="hip",
                  domain=None, domain_margin=0.05, device=0, cell_divisions=0,
-                 x_cell_divisions=0, skin=-1.0, pair_search="auto"):
+                 x_cell_divisions=0, skin=-1.0, pair_search="auto", precision="fp64"):
         fluids = [s for s in systems if isinstance(s, WeaklyCompressibleSPHSystem)]
         walls = [s for s in systems if isinstance(s, WallBoundarySystem)]
         if len(fluids) != 1 or len(walls) > 1 or len(fluids) + len(walls) != len(systems):
@@ -112,6 +116,9 @@ class Semidiscretization:
         if pair_search not in native.PAIR_SEARCH:
             raise ValueError(f"pair_search must be one of {sorted(native.PAIR_SEARCH)}")
         self.pair_search = pair_search
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
         self._handle = None
 
     # -- parameters of the C ABI
@@ -126,6 +133,8 @@ class Semidiscretization:
         p.x_cell_divisions = self.x_cell_divisions
         p.skin = self.skin
         p.pair_search = native.PAIR_SEARCH[self.pair_search]
+        p.precision = native.PRECISION_MIXED if self.precision == "mixed" else \
+            native.PRECISION_FP64
         p.clip_fluid_eos = int(se.clip_negative_pressure)
         p.smoothing_length = f.smoothing_length
         p.reference_density = se.reference_density
@@ -240,6 +249,22 @@ class Semidiscretization:
         v0 = np.concatenate([ic.velocity, ic.density.reshape(1, -1)], axis=0).T
         u0 = ic.coordinates.T
         return self._array(v0), self._array(u0)
+
+    def integrals(self, v_ode, u_ode):
+        """FP64 integrals of a fluid state (``tpsph_integrals``): total mass, momentum (3),
+        kinetic energy, potential energy, fluid volume sum m/rho and sum rho, as a dict."""
+        out = np.zeros(8)
+        if self.backend == "hip":
+            import torch
+            out_d = torch.zeros(8, dtype=torch.float64, device=f"cuda:{self.device}")
+            native.check(native.load().tpsph_integrals(self.handle, _ptr(v_ode), _ptr(u_ode),
+                                                       out_d.data_ptr()), self.handle)
+            out = out_d.cpu().numpy()
+        else:
+            native.check(native.load().tpsph_integrals(self.handle, _ptr(v_ode), _ptr(u_ode),
+                                                       out.ctypes.data), self.handle)
+        return {"mass": out[0], "momentum": out[1:4].copy(), "kinetic_energy": out[4],
+                "potential_energy": out[5], "volume": out[6], "density_sum": out[7]}
 
     def synchronize(self):
         native.check(native.load().tpsph_synchronize(self.handle), self.handle)
