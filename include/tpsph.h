@@ -295,6 +295,11 @@ int tpsph_kick_noalloc(const tpsph_params *params, int64_t n_fluid, int64_t n_wa
  * out holds 8 doubles (host or device memory).  Synchronises the handle's stream. */
 int tpsph_integrals(tpsph_handle *h, const double *v_ode, const double *u_ode, double *out);
 
+/* Number of rebuilds of the fluid cell-sorted order (and of the Verlet lists) since
+ * tpsph_create: a kick rebuilds when a particle has moved by more than half the skin since
+ * the last rebuild.  Synchronises the handle's stream. */
+int tpsph_rebuild_count(tpsph_handle *h, int64_t *count);
+
 /* Wait for all work queued on the handle's stream. */
 int tpsph_synchronize(tpsph_handle *h);
 

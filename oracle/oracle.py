@@ -99,6 +99,9 @@ def lib():
         L.oracle_drift.argtypes = [C.c_int, i64, P, P]
         L.oracle_drift.restype = None
         L.oracle_solve_ck2n54.argtypes = [P, i64, P, P, P, i64, P, P, d, i64]
+        L.oracle_dual_kick.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, P, P]
+        L.oracle_dual_solve_ck2n54.argtypes = [P, i64, P, P, P, i64, P, P, d, i64, P, P, P, P,
+                                               P, P, P]
         L.oracle_ck2n54_coefficients.argtypes = [P, P, P]
         L.oracle_ck2n54_coefficients.restype = None
         L.oracle_num_threads.restype = C.c_int
@@ -309,6 +312,57 @@ def solve_ck2n54(P, v, u, m, xw, mw, dt, nsteps):
                                    _p(mw), dt, nsteps)
     assert rc == 0
     return v, u
+
+
+DUAL_TANGENTS = 4
+
+
+def _dual_seeds(nf, nd, seed_params, seed_v, seed_u):
+    T = DUAL_TANGENTS
+    sp = None if seed_params is None else np.ascontiguousarray(seed_params, dtype=np.float64).reshape(3, T)
+    sv = None if seed_v is None else np.ascontiguousarray(seed_v, dtype=np.float64).reshape(T, nf, nd + 1)
+    su = None if seed_u is None else np.ascontiguousarray(seed_u, dtype=np.float64).reshape(T, nf, nd)
+    return sp, sv, su
+
+
+def dual_kick(P, v, u, m, xw, mw, seed_params=None, seed_v=None, seed_u=None):
+    """kick! on dual numbers (sph_dual.cpp): returns (dv, tangents of dv), the tangents of
+    shape (4, nf, nd+1) along the seeds: seed_params (3 x 4: alpha, delta, g), seed_v
+    (4 x nf x (nd+1)), seed_u (4 x nf x nd)."""
+    v, u, m = _f64(v), _f64(u), _f64(m)
+    nf, nd = v.shape[0], v.shape[1] - 1
+    nw = 0 if xw is None else xw.shape[0]
+    xw = _f64(xw) if nw else None
+    mw = _f64(mw) if nw else None
+    sp, sv, su = _dual_seeds(nf, nd, seed_params, seed_v, seed_u)
+    dv = np.zeros_like(v)
+    tan = np.zeros((DUAL_TANGENTS,) + v.shape)
+    rc = lib().oracle_dual_kick(C.byref(P), nf, _p(v), _p(u), _p(m), nw, _p(xw), _p(mw),
+                                _p(sp), _p(sv), _p(su), _p(dv), _p(tan))
+    assert rc == 0
+    return dv, tan
+
+
+def dual_solve_ck2n54(P, v, u, m, xw, mw, dt, nsteps, seed_params=None, seed_v=None,
+                      seed_u=None, u_target=None):
+    """CarpenterKennedy2N54 on dual numbers: returns (v, u, v tangents, u tangents, loss),
+    loss = (L, dL along the 4 seeds) for L = sum |u - u_target|^2 / nf, or None."""
+    v, u = _f64(v).copy(), _f64(u).copy()
+    m = _f64(m)
+    nf, nd = v.shape[0], v.shape[1] - 1
+    nw = 0 if xw is None else xw.shape[0]
+    xw = _f64(xw) if nw else None
+    mw = _f64(mw) if nw else None
+    sp, sv, su = _dual_seeds(nf, nd, seed_params, seed_v, seed_u)
+    tv = np.zeros((DUAL_TANGENTS,) + v.shape)
+    tu = np.zeros((DUAL_TANGENTS,) + u.shape)
+    ut = None if u_target is None else _f64(u_target)
+    loss = np.zeros(1 + DUAL_TANGENTS)
+    rc = lib().oracle_dual_solve_ck2n54(C.byref(P), nf, _p(v), _p(u), _p(m), nw, _p(xw),
+                                        _p(mw), dt, nsteps, _p(sp), _p(sv), _p(su), _p(tv),
+                                        _p(tu), _p(ut), _p(loss) if ut is not None else None)
+    assert rc == 0
+    return v, u, tv, tu, (loss if ut is not None else None)
 
 
 def ck2n54_coefficients():

@@ -101,7 +101,7 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
                          double *__restrict__ snap, int *__restrict__ iperm,
                          int *__restrict__ bid, int *__restrict__ rank,
                          int *__restrict__ bcount, int *__restrict__ zero, int nzero,
-                         const int *__restrict__ noverflow)
+                         const int *__restrict__ noverflow, int *__restrict__ nrebuild)
 {
     P = with_sound(P);
     const int f = *flag;
@@ -183,6 +183,7 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
         }
     }
     if (MODE == 1 && zero && blockIdx.x == 0 && (int)threadIdx.x < nzero) zero[threadIdx.x] = 0;
+    if (MODE == 1 && nrebuild && blockIdx.x == 0 && threadIdx.x == 0) *nrebuild += 1;
 }
 
 
@@ -1558,7 +1559,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
         hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, h->bwork, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
             h->snap, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
-            h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr);
+            h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr, nullptr);
         TRY(launch_check(h, "k_gather<0>"));
     }
     {
@@ -1571,7 +1572,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
         hipLaunchKernelGGL((k_gather<ND, 1>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, nullptr, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
             h->snap, h->iperm, lists ? h->f_cell : nullptr, lists ? h->f_rank : nullptr,
-            lists ? h->bcount : nullptr, lists ? h->counters : nullptr, 4, nullptr);
+            lists ? h->bcount : nullptr, lists ? h->counters : nullptr, 4, nullptr, h->nrebuild);
         if (lists) {
             TRY(scan(h, h->flag, h->bcount, h->bstart, h->nbricks + 1));
             hipLaunchKernelGGL((k_brick_scatter), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
@@ -2166,6 +2167,11 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
                     "DensityDiffusionAntuono");
     }
     CTRY(dalloc(h, &h->flag, 1));
+    CTRY(dalloc(h, &h->nrebuild, 1));
+    if (hipMemset(h->nrebuild, 0, sizeof(int)) != hipSuccess) {
+        h->last_error = "clearing the rebuild count failed";
+        return cleanup_fail(TPSPH_ERR_HIP);
+    }
     if (params->wall_viscosity && nw > 0) CTRY(dalloc(h, &h->wvel, 3 * (size_t)nw));
     if (params->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO && nf > 0) {
         CTRY(dalloc(h, &h->ddgrad, (size_t)nd * nf));
@@ -2634,6 +2640,18 @@ int tpsph_integrals(tpsph_handle *h, const double *v_ode, const double *u_ode, d
     return TPSPH_OK;
 }
 
+int tpsph_rebuild_count(tpsph_handle *h, int64_t *count)
+{
+    if (!h || !count) return fail(h, TPSPH_ERR_ARGUMENT, "handle or count is NULL");
+    int n = 0;
+    if (h->nrebuild) {
+        HIP_TRY(h, hipMemcpyAsync(&n, h->nrebuild, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+    }
+    *count = n;
+    return TPSPH_OK;
+}
+
 int tpsph_synchronize(tpsph_handle *h)
 {
     if (!h) return fail(nullptr, TPSPH_ERR_ARGUMENT, "handle is NULL");
@@ -2681,7 +2699,7 @@ void tpsph_destroy(tpsph_handle *h)
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,
                     h->wtargets, h->iperm, h->dyn, h->vmax_part, h->wvel, h->ddgrad,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
-                    h->sol_tu, h->integ};
+                    h->sol_tu, h->integ, h->nrebuild};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);

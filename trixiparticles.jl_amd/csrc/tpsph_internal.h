@@ -763,6 +763,7 @@ struct tpsph_handle {
     bool have_kick = false;
     bool mixed = false;  // TPSPH_PRECISION_MIXED: FP32 pair terms in the brick kernel
     double *integ = nullptr;  // tpsph_integrals: block partials and the result
+    int *nrebuild = nullptr;  // cell-list rebuilds since create (tpsph_rebuild_count)
 
     bool timing = false;
     struct Rec {

@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "tpsph_get_sound_speed",
     "tpsph_neighbors_csr", "tpsph_kick_pairs_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54", "tpsph_synchronize",
     "tpsph_enable_timing", "tpsph_kernel_time", "tpsph_last_error", "tpsph_destroy",
-    "tpsph_kick_noalloc", "tpsph_integrals",
+    "tpsph_kick_noalloc", "tpsph_integrals", "tpsph_rebuild_count",
 )
 
 
@@ -129,6 +129,7 @@ def load(path: str | None = None):
     lib.tpsph_kick_noalloc.argtypes = [C.POINTER(TpsphParams), C.c_int64, C.c_int64, P, P, P,
                                        P, P, P, C.c_double, P]
     lib.tpsph_integrals.argtypes = [P, P, P, P]
+    lib.tpsph_rebuild_count.argtypes = [P, C.POINTER(C.c_int64)]
     lib.tpsph_synchronize.argtypes = [P]
     lib.tpsph_enable_timing.argtypes = [P, C.c_int32]
     lib.tpsph_kernel_time.argtypes = [P, C.c_int32, C.POINTER(C.c_int64), dp]
@@ -139,7 +140,7 @@ def load(path: str | None = None):
     for name in ("tpsph_create", "tpsph_kick", "tpsph_drift", "tpsph_get_pressures",
                  "tpsph_neighbors_csr", "tpsph_kick_pairs_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54",
                  "tpsph_synchronize", "tpsph_enable_timing", "tpsph_kernel_time",
-                 "tpsph_kick_noalloc", "tpsph_integrals"):
+                 "tpsph_kick_noalloc", "tpsph_integrals", "tpsph_rebuild_count"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
