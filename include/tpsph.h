@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define TPSPH_ABI_VERSION 3
+#define TPSPH_ABI_VERSION 4
 
 enum tpsph_status {
     TPSPH_OK = 0,
@@ -91,7 +91,8 @@ enum tpsph_timed_kernel {
     TPSPH_TIME_WALL = 2,        /* Adami pressure extrapolation + inverse state equation  */
     TPSPH_TIME_FLUID = 3,       /* fused fluid-fluid + fluid-wall interaction + gravity   */
     TPSPH_TIME_STAGE = 4,       /* CK2N54 stage update                                    */
-    TPSPH_TIME_COUNT = 5
+    TPSPH_TIME_ADJOINT = 5,     /* reverse-mode passes (tpsph_kick_vjp, tpsph_adjoint_*)  */
+    TPSPH_TIME_COUNT = 6
 };
 
 /*
@@ -258,6 +259,14 @@ int tpsph_kick_pairs_csr(tpsph_handle *h, int32_t pair, int64_t *offsets, int32_
 int tpsph_rk_stage(tpsph_handle *h, double *y, double *tmp, const double *k, double a,
                    double b, double dt, int64_t n, int32_t first_stage);
 
+/* y = a*y + b*x over n values: the single broadcasts OrdinaryDiffEq's low-storage step
+ * issues one at a time (LowStorageRK2N perform_step! with williamson_condition = false:
+ * tmp = dt*k, u = u + B*tmp, tmp = A*tmp, tmp = tmp + dt*k), for a device-array type that
+ * runs them on the device.  b == 0 reads only y (y = a*y), a == 0 reads only x (y = b*x),
+ * a == b == 0 writes zeros (fill!); otherwise y = fma(b, x, a*y), muladd's rounding, so the
+ * sequence reproduces tpsph_rk_stage and tpsph_solve_ck2n54 bit for bit. */
+int tpsph_axpby(tpsph_handle *h, int64_t n, double *y, double a, const double *x, double b);
+
 /* nsteps fixed steps of CarpenterKennedy2N54(williamson_condition=false) from t0, fully on
  * the device: per stage kick! + drift! + the fused stage update of (v_ode, u_ode). */
 int tpsph_solve_ck2n54(tpsph_handle *h, double *v_ode, double *u_ode, double t0, double dt,
@@ -283,6 +292,42 @@ int tpsph_kick_noalloc(const tpsph_params *params, int64_t n_fluid, int64_t n_wa
                        const double *fluid_mass, const double *wall_coordinates,
                        const double *wall_mass, double *dv_ode, const double *v_ode,
                        const double *u_ode, double t, void *stream);
+
+/*
+ * Reverse mode.  The reference has no automatic differentiation through its right-hand side;
+ * these entry points are what a ChainRulesCore.rrule of kick! and of a trajectory would call
+ * (the Julia side is in INTEGRATION.md).  Differentiated inputs: the fluid state and the
+ * scalars alpha (ArtificialViscosityMonaghan), delta (DensityDiffusionMolteniColagrossi) and
+ * g (gravity = g * acceleration / |acceleration|); param_bar = {dL/dalpha, dL/ddelta, dL/dg}
+ * (3 doubles, host or device memory; the call synchronises when it is given).  Supported:
+ * device pointers, StateEquationCole with a fixed sound speed, Monaghan viscosity,
+ * Molteni-Colagrossi or no density diffusion, walls without viscosity, FP64; anything else
+ * returns TPSPH_ERR_ARGUMENT.  Non-smooth points take the branch the forward pass takes (the
+ * viscosity switch, the wall-pressure clipping).  Deterministic (gathers only).
+ */
+
+/* The pullback of kick! at (v_ode, u_ode) (semidiscretization.jl:589-612, the terms of
+ * wcsph/rhs.jl:5-127 and dummy_particles.jl:489-672): for the adjoint dv_bar of dv_ode,
+ * v_bar = (d dv / d v)^T dv_bar, u_bar = (d dv / d u)^T dv_bar (shapes of v_ode, u_ode,
+ * overwritten) and param_bar as above (may be NULL). */
+int tpsph_kick_vjp(tpsph_handle *h, const double *dv_bar, const double *v_ode,
+                   const double *u_ode, double t, double *v_bar, double *u_bar,
+                   double *param_bar);
+
+/* The forward half of a trajectory's rrule: nsteps of CarpenterKennedy2N54 from
+ * (v0_ode, u0_ode) into (v_ode, u_ode) (which may alias the inputs), as tpsph_solve_ck2n54,
+ * keeping the state every checkpoint_every steps in the handle for tpsph_adjoint_reverse. */
+int tpsph_adjoint_forward(tpsph_handle *h, const double *v0_ode, const double *u0_ode,
+                          double t0, double dt, int64_t nsteps, int64_t checkpoint_every,
+                          double *v_ode, double *u_ode);
+
+/* The pullback of the last tpsph_adjoint_forward: given the adjoints (v_bar, u_bar) of the
+ * final state, writes those of the initial state (v0_bar, u0_bar) and param_bar.  Each
+ * segment between checkpoints is recomputed from its checkpoint with every stage state kept
+ * (5 checkpoint_every states of device memory), then swept backwards.  TPSPH_ERR_STATE
+ * without a stored trajectory. */
+int tpsph_adjoint_reverse(tpsph_handle *h, const double *v_bar, const double *u_bar,
+                          double *v0_bar, double *u0_bar, double *param_bar);
 
 /* Integrals of the fluid state (v_ode, u_ode) in FP64, summed in a fixed order on the
  * device (deterministic), the global reductions a run reports beside its state:

@@ -9,8 +9,8 @@ import pytest
 from conftest import REPO
 
 
-def declared_symbols():
-    text = open(os.path.join(REPO, "include", "tpsph.h")).read()
+def declared_symbols(header="tpsph.h"):
+    text = open(os.path.join(REPO, "include", header)).read()
     return sorted(set(re.findall(r"\b(tpsph_[a-z0-9_]+)\s*\(", text)))
 
 
@@ -18,10 +18,33 @@ def test_library_exports_every_declared_symbol(tp):
     lib = tp.native.load()
     syms = declared_symbols()
     assert "tpsph_kick" in syms and "tpsph_create" in syms
+    assert {"tpsph_kick_vjp", "tpsph_adjoint_forward", "tpsph_adjoint_reverse"} <= set(syms)
     for s in syms:
         assert hasattr(lib, s), f"{s} declared in include/tpsph.h but not exported"
     assert set(tp.native.EXPORTED_SYMBOLS) == set(syms)
-    assert lib.tpsph_abi_version() == 3
+    assert lib.tpsph_abi_version() == 4
+
+
+def test_library_exports_every_symbol_of_every_header(tp):
+    lib = tp.native.load()
+    headers = sorted(f for f in os.listdir(os.path.join(REPO, "include")) if f.endswith(".h"))
+    assert "tpsph_batch.h" in headers
+    for header in headers:
+        for s in declared_symbols(header):
+            assert hasattr(lib, s), f"{s} declared in include/{header} but not exported"
+
+
+def test_reverse_mode_entry_points_reject_a_null_handle(tp):
+    """The adjoint entry points validate before touching the device."""
+    lib = tp.native.load()
+    pb = (C.c_double * 3)()
+    assert lib.tpsph_kick_vjp(None, None, None, None, 0.0, None, None, pb) == \
+        tp.native.TPSPH_ERR_ARGUMENT
+    assert lib.tpsph_adjoint_forward(None, None, None, 0.0, 1e-5, 1, 1, None, None) == \
+        tp.native.TPSPH_ERR_ARGUMENT
+    assert lib.tpsph_adjoint_reverse(None, None, None, None, None, pb) == \
+        tp.native.TPSPH_ERR_ARGUMENT
+    assert "NULL" in lib.tpsph_last_error(None).decode()
 
 
 def test_params_struct_layout_matches_header(tp):

@@ -1,0 +1,259 @@
+"""Reverse mode (tpsph_kick_vjp, tpsph_adjoint_forward / _reverse) against the oracle's
+dual-number restatement (oracle/sph_dual.cpp, itself pinned by the plain oracle, central
+differences and closed forms in test_oracle_dual.py) and against central finite differences
+of the HIP forward run.
+
+- kick: <dv_bar, J s> from the dual kick equals <J^T dv_bar, s> from the pullback for the
+  parameter tangents (alpha, delta, g) and for random directions in velocity, density and
+  position, each on its own, so that a wrong term shows up in its own check;
+- trajectory (the differentiation workload's 2D copy: 7,200 fluid particles at dx = 0.01,
+  v0 ~ N(0, 0.05^2), 100 CK2N54 steps at dt = 2e-5, checkpoints every 10 steps, targets
+  from alpha, delta, g = 0.03, 0.12, 9.6): the gradient along alpha, delta, g and a random
+  direction in (v0, u0) against the dual trajectory within 1e-6 relative, and against
+  central differences with eps = 1e-6;
+- the checkpoint interval does not change the gradient; the autograd functions give the
+  same gradients as the direct calls.
+Each comparison prints its relative error."""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs an MI355X")]
+
+DT, STEPS, EVERY = 2e-5, 100, 10
+TARGET = (0.03, 0.12, 9.6)
+
+
+@pytest.fixture(scope="module")
+def ad(tp):
+    from trixiparticles_jl_amd import adjoint
+    return adjoint
+
+
+def oracle_args(db):
+    return (db.fluid_system.mass, db.boundary_system.coordinates.T.copy(),
+            db.boundary_system.boundary_model.hydrodynamic_mass)
+
+
+def perturbed_state(semi, nd, seed, vscale=0.3):
+    v0, u0 = semi.initial_state()
+    v, u = v0.cpu().numpy().copy(), u0.cpu().numpy().copy()
+    rng = np.random.default_rng(seed)
+    v[:, :nd] = rng.normal(0.0, vscale, size=(v.shape[0], nd))
+    v[:, nd] *= 1.0 + rng.uniform(-2e-3, 2e-3, size=v.shape[0])
+    return v, u
+
+
+def rel(a, b, scale):
+    return abs(a - b) / scale
+
+
+@pytest.mark.parametrize("nd,dx", [(2, 0.02), (3, 0.04)])
+def test_kick_pullback_matches_dual_tangents(tp, orc, ad, nd, dx):
+    import torch
+    db = tp.dam_break(nd, dx)
+    semi = tp.Semidiscretization(db.fluid_system, db.boundary_system)
+    P = orc.params_from_semi(semi)
+    v, u = perturbed_state(semi, nd, 11)
+    nf = v.shape[0]
+    rng = np.random.default_rng(5)
+    dv_bar = rng.normal(size=v.shape)
+    dv_bar[:, nd] *= 1e-2  # drho is about 100x dv: comparable weights
+    vd, ud = torch.from_numpy(v).cuda(), torch.from_numpy(u).cuda()
+    vb, ub, pb = ad.kick_vjp(semi, torch.from_numpy(dv_bar).cuda(), vd, ud)
+    vb, ub = vb.cpu().numpy(), ub.cpu().numpy()
+    # tangents: alpha, delta, g, then velocity, density, position and all together
+    sp = np.zeros((3, 4))
+    sp[0, 0] = sp[1, 1] = sp[2, 2] = 1.0
+    sv1 = np.zeros((4, nf, nd + 1))
+    su1 = np.zeros((4, nf, nd))
+    sv2 = np.zeros((4, nf, nd + 1))
+    su2 = np.zeros((4, nf, nd))
+    sv2[0, :, :nd] = rng.normal(size=(nf, nd))
+    sv2[1, :, nd] = rng.normal(size=nf)
+    su2[2] = rng.normal(size=(nf, nd))
+    sv2[3] = rng.normal(size=(nf, nd + 1))
+    su2[3] = rng.normal(size=(nf, nd))
+    m, xw, mw = oracle_args(db)
+    dv1, tan1 = orc.dual_kick(P, v, u, m, xw, mw, sp, sv1, su1)
+    _, tan2 = orc.dual_kick(P, v, u, m, xw, mw, None, sv2, su2)
+    errs = {}
+    for k, name in enumerate(("alpha", "delta", "g")):
+        terms = dv_bar * tan1[k]
+        errs[name] = rel(terms.sum(), pb[k], np.abs(terms).sum())
+    for k, name in enumerate(("velocity", "density", "position", "all")):
+        terms = dv_bar * tan2[k]
+        got = (vb * sv2[k]).sum() + (ub * su2[k]).sum()
+        errs[name] = rel(terms.sum(), got, np.abs(terms).sum())
+    print(f"kick pullback {nd}D ({nf} fluid): " +
+          ", ".join(f"{k} {e:.2e}" for k, e in errs.items()), flush=True)
+    for name, e in errs.items():
+        assert e <= 1e-10, (name, e)
+    semi.close()
+
+
+def workload_2d(tp, ad):
+    """The 2D copy of the differentiation workload and its targets."""
+    import torch
+    db, semi, v0, u0 = ad.differentiation_case(2, 0.01)
+    tsemi = ad.with_parameters(semi, *TARGET)
+    vt, ut = v0.clone(), u0.clone()
+    from trixiparticles_jl_amd import native
+    native.check(native.load().tpsph_solve_ck2n54(tsemi.handle, vt.data_ptr(), ut.data_ptr(),
+                                                  0.0, DT, STEPS), tsemi.handle)
+    torch.cuda.synchronize()
+    tsemi.close()
+    return db, semi, v0, u0, ut
+
+
+def gpu_loss(tp, ad, semi, v0, u0, target, **params):
+    import torch
+    s = ad.with_parameters(semi, **params) if params else semi
+    from trixiparticles_jl_amd import native
+    v, u = v0.clone(), u0.clone()
+    native.check(native.load().tpsph_solve_ck2n54(s.handle, v.data_ptr(), u.data_ptr(), 0.0,
+                                                  DT, STEPS), s.handle)
+    torch.cuda.synchronize()
+    loss = float(((u - target) ** 2).sum() / u.shape[0])
+    if params:
+        s.close()
+    return loss
+
+
+def test_trajectory_gradient_matches_dual_and_central_differences(tp, orc, ad):
+    import torch
+    db, semi, v0, u0, target = workload_2d(tp, ad)
+    nf = db.n_fluid
+    assert nf == 7200
+    loss, v0b, u0b, pb = ad.gradient(semi, v0, u0, target, DT, STEPS, EVERY)
+    v0b, u0b = v0b.cpu().numpy(), u0b.cpu().numpy()
+    rng = np.random.default_rng(17)
+    dv_dir = np.zeros((nf, 3))
+    dv_dir[:, :2] = rng.normal(size=(nf, 2))
+    du_dir = 1e-3 * rng.normal(size=(nf, 2))
+    grad = {"alpha": pb[0], "delta": pb[1], "g": pb[2],
+            "direction": float((v0b * dv_dir).sum() + (u0b * du_dir).sum())}
+    # dual numbers through the oracle's trajectory
+    P = orc.params_from_semi(semi)
+    sp = np.zeros((3, 4))
+    sp[0, 0] = sp[1, 1] = sp[2, 2] = 1.0
+    sv = np.zeros((4, nf, 3))
+    su = np.zeros((4, nf, 2))
+    sv[3], su[3] = dv_dir, du_dir
+    m, xw, mw = oracle_args(db)
+    *_, dual = orc.dual_solve_ck2n54(P, v0.cpu().numpy(), u0.cpu().numpy(), m, xw, mw, DT,
+                                     STEPS, sp, sv, su, target.cpu().numpy())
+    assert abs(dual[0] - loss) <= 1e-9 * loss
+    errs = {k: rel(grad[k], dual[1 + i], abs(dual[1 + i]))
+            for i, k in enumerate(("alpha", "delta", "g", "direction"))}
+    print("trajectory gradient vs dual: " + ", ".join(f"{k} {e:.2e}" for k, e in errs.items()),
+          f"(loss {loss:.6e}, dL/dalpha {pb[0]:.6e}, dL/ddelta {pb[1]:.6e}, dL/dg {pb[2]:.6e})",
+          flush=True)
+    for k, e in errs.items():
+        assert e <= 1e-6, (k, e)
+    # central differences of the HIP forward run, eps = 1e-6
+    eps = 1e-6
+    p0 = ad.parameters(semi)
+    fd = {}
+    for i, k in enumerate(("alpha", "delta", "g")):
+        hi = dict(zip(("alpha", "delta", "g"), p0))
+        lo = dict(hi)
+        hi[k] += eps
+        lo[k] -= eps
+        fd[k] = (gpu_loss(tp, ad, semi, v0, u0, target, **hi) -
+                 gpu_loss(tp, ad, semi, v0, u0, target, **lo)) / (2 * eps)
+    dvt = torch.from_numpy(dv_dir).cuda()
+    dut = torch.from_numpy(du_dir).cuda()
+    fd["direction"] = (gpu_loss(tp, ad, semi, v0 + eps * dvt, u0 + eps * dut, target) -
+                       gpu_loss(tp, ad, semi, v0 - eps * dvt, u0 - eps * dut, target)) / (2 * eps)
+    errs = {k: rel(grad[k], fd[k], abs(fd[k])) for k in fd}
+    print("trajectory gradient vs central differences (eps 1e-6): " +
+          ", ".join(f"{k} {e:.2e}" for k, e in errs.items()), flush=True)
+    for k, e in errs.items():
+        assert e <= 1e-6, (k, e)
+    semi.close()
+
+
+def test_checkpoint_interval_does_not_change_the_gradient(tp, ad):
+    db, semi, v0, u0, target = workload_2d(tp, ad)
+    steps = 30
+    ref = None
+    for every in (30, 7, 1):
+        loss, v0b, u0b, pb = ad.gradient(semi, v0, u0, target, DT, steps, every)
+        got = (loss, v0b.cpu().numpy(), u0b.cpu().numpy(), pb)
+        if ref is None:
+            ref = got
+            continue
+        assert got[0] == ref[0]
+        for a, b in zip(got[1:], ref[1:]):
+            scale = np.abs(b).max()
+            assert np.abs(a - b).max() <= 1e-12 * scale, every
+    semi.close()
+
+
+def test_autograd_functions_give_the_direct_gradients(tp, ad):
+    import torch
+    db, semi, v0, u0, target = workload_2d(tp, ad)
+    steps = 20
+    loss, v0b, u0b, pb = ad.gradient(semi, v0, u0, target, DT, steps, 5)
+    params = torch.tensor(ad.parameters(semi), device=v0.device, requires_grad=True)
+    v0r, u0r = v0.clone().requires_grad_(True), u0.clone().requires_grad_(True)
+    v, u = ad.Trajectory.apply(v0r, u0r, params, semi, DT, steps, 5)
+    L = ad.mse_loss(u, target)
+    L.backward()
+    assert float(L) == loss
+    assert torch.equal(v0r.grad, v0b) and torch.equal(u0r.grad, u0b)
+    assert np.array_equal(params.grad.cpu().numpy(), pb)
+    # Kick: the pullback of one kick through autograd
+    params.grad = None
+    v0r.grad = u0r.grad = None
+    dv = ad.Kick.apply(v0r, u0r, params, semi)
+    w = torch.randn_like(dv)
+    (dv * w).sum().backward()
+    vb, ub, pk = ad.kick_vjp(semi, w, v0, u0)
+    assert torch.equal(v0r.grad, vb) and torch.equal(u0r.grad, ub)
+    assert np.array_equal(params.grad.cpu().numpy(), pk)
+    with pytest.raises(ValueError):
+        ad.Trajectory.apply(v0r, u0r, params.detach() * 2, semi, DT, steps, 5)
+    semi.close()
+
+
+def test_full_size_3d_gradient_is_consistent(tp, ad):
+    """The 3D differentiation workload at full size (250,000 fluid particles, 100 steps):
+    the gradient along alpha and along a random initial-velocity direction against central
+    differences of the forward run (eps = 1e-6)."""
+    import torch
+    db, semi, v0, u0 = ad.differentiation_case(3, 0.01)
+    assert db.n_fluid == 250000
+    tsemi = ad.with_parameters(semi, *TARGET)
+    from trixiparticles_jl_amd import native
+    vt, ut = v0.clone(), u0.clone()
+    native.check(native.load().tpsph_solve_ck2n54(tsemi.handle, vt.data_ptr(), ut.data_ptr(),
+                                                  0.0, DT, STEPS), tsemi.handle)
+    torch.cuda.synchronize()
+    tsemi.close()
+    target = ut
+    loss, v0b, u0b, pb = ad.gradient(semi, v0, u0, target, DT, STEPS, EVERY)
+    assert np.isfinite(pb).all() and bool(torch.isfinite(v0b).all())
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    dvt = torch.zeros_like(v0)
+    dvt[:, :3] = torch.randn(v0.shape[0], 3, generator=gen, device="cuda", dtype=v0.dtype)
+    eps = 1e-6
+    grad_dir = float((v0b * dvt).sum())
+    fd_dir = (gpu_loss(tp, ad, semi, v0 + eps * dvt, u0, target) -
+              gpu_loss(tp, ad, semi, v0 - eps * dvt, u0, target)) / (2 * eps)
+    p0 = dict(zip(("alpha", "delta", "g"), ad.parameters(semi)))
+    hi, lo = dict(p0), dict(p0)
+    hi["alpha"] += eps
+    lo["alpha"] -= eps
+    fd_alpha = (gpu_loss(tp, ad, semi, v0, u0, target, **hi) -
+                gpu_loss(tp, ad, semi, v0, u0, target, **lo)) / (2 * eps)
+    e_dir = rel(grad_dir, fd_dir, abs(fd_dir))
+    e_alpha = rel(pb[0], fd_alpha, abs(fd_alpha))
+    print(f"3D 250k gradient vs central differences: direction {e_dir:.2e}, "
+          f"alpha {e_alpha:.2e} (loss {loss:.6e}, dL/dalpha {pb[0]:.6e}, "
+          f"dL/ddelta {pb[1]:.6e}, dL/dg {pb[2]:.6e})", flush=True)
+    assert e_dir <= 1e-5 and e_alpha <= 1e-5
+    semi.close()

@@ -170,12 +170,17 @@ def test_trajectory_matches_oracle(tp, orc):
     check_columns(du_got, du_ref, rtol=1e-10, what="u displacement")
 
 
-def test_callback_driver_matches_native_solver(tp):
+@pytest.mark.parametrize("stage_update", ["fused", "broadcasts"])
+def test_callback_driver_matches_native_solver(tp, stage_update):
+    """kick!/drift! driven from the host with the stage update of tpsph_rk_stage, or with
+    LowStorageRK2N's single broadcasts through tpsph_axpby, equal the device solver bit for
+    bit."""
     db, semi = setup_case(tp, 2, 0.02)
     ode = tp.semidiscretize(semi, (0.0, 5e-5))
     v1, u1 = tp.solve(ode, dt=1e-5, nsteps=5)
     seen = []
-    v2, u2 = tp.solve(ode, dt=1e-5, nsteps=5, callback=lambda v, u, t, s: seen.append(s))
+    v2, u2 = tp.solve(ode, dt=1e-5, nsteps=5, callback=lambda v, u, t, s: seen.append(s),
+                      stage_update=stage_update)
     assert seen == [1, 2, 3, 4, 5]
     assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
     assert np.array_equal(u1.cpu().numpy(), u2.cpu().numpy())

@@ -11,7 +11,7 @@ import sys
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libtpsph.so")
-SOURCES = [os.path.join(PKG_DIR, "csrc", f) for f in ("tpsph.hip", "tpsph_mixed.hip", "tpsph_batch.hip")]
+SOURCES = [os.path.join(PKG_DIR, "csrc", f) for f in ("tpsph.hip", "tpsph_mixed.hip", "tpsph_batch.hip", "tpsph_adjoint.hip")]
 HEADERS = [os.path.join(PKG_DIR, "csrc", "tpsph_internal.h")]
 
 HIPCC_FLAGS = [

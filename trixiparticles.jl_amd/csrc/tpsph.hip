@@ -1323,6 +1323,21 @@ __global__ void k_rk_stage(int64_t n, double *__restrict__ y, double *__restrict
     y[i] = fma(b, t, y[i]);
 }
 
+// y = a y + b x, the broadcasts of a low-storage Runge-Kutta step one at a time: b == 0 reads
+// only y (y = a y), a == 0 only x (y = b x), both zero write zeros; otherwise fma(b, x, a y)
+// (the rounding of muladd(b, x, a y)).
+__global__ void k_axpby(int64_t n, double *__restrict__ y, double a, const double *x, double b)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (b == 0.0)
+        y[i] = a == 0.0 ? 0.0 : a * y[i];
+    else if (a == 0.0)
+        y[i] = b * x[i];
+    else
+        y[i] = fma(b, x[i], a * y[i]);
+}
+
 // ------------------------------------------------------------------------------------------
 // neighbour CSR export (parity tests): rows = cell-sorted points (rx, ry, rz; row_perm to the
 // caller's order), columns = cell-sorted points with cell offsets col_start
@@ -1604,8 +1619,11 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
     return TPSPH_OK;
 }
 
+// Everything a kick does before the fluid pass: the sound speed (adaptive), the cell-sorted
+// records of (v, u) with the fluid pressures, the rebuild of the cell list and the lists (if
+// a particle left its skin), and the Adami pass of the walls.
 template <int ND>
-int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, bool gathered)
+int prepare_impl(tpsph_handle *h, const double *v, const double *u, bool gathered)
 {
     const int nf = h->nf, nw = h->nw;
     if (h->dyn && nf > 0) {
@@ -1632,6 +1650,14 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
                 nullptr, h->wvel);
         TRY(launch_check(h, "k_wall"));
     }
+    return TPSPH_OK;
+}
+
+template <int ND>
+int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, bool gathered)
+{
+    const int nf = h->nf, nw = h->nw;
+    TRY(prepare_impl<ND>(h, v, u, gathered));
     if (nf > 0) {
         TimedScope t(h, TPSPH_TIME_FLUID);
         if (h->P.ddgrad) {
@@ -1849,6 +1875,62 @@ int tpsph::kick_dispatch(tpsph_handle *h, double *dv, const double *v, const dou
                          bool gathered)
 {
     return h->nd == 2 ? kick_impl<2>(h, dv, v, u, gathered) : kick_impl<3>(h, dv, v, u, gathered);
+}
+
+int tpsph::prepare_dispatch(tpsph_handle *h, const double *v, const double *u)
+{
+    return h->nd == 2 ? prepare_impl<2>(h, v, u, false) : prepare_impl<3>(h, v, u, false);
+}
+
+const double tpsph::kCkA[5] = {0.0, -567301805773.0 / 1357537059087.0,
+                               -2404267990393.0 / 2016746695238.0,
+                               -3550918686646.0 / 2091501179385.0,
+                               -1275806237668.0 / 842570457699.0};
+const double tpsph::kCkB[5] = {1432997174477.0 / 9575080441755.0,
+                               5161836677717.0 / 13612068292357.0,
+                               1720146321549.0 / 2090206949498.0,
+                               3134564353537.0 / 4481467310338.0,
+                               2277821191437.0 / 14882151754819.0};
+
+int tpsph::run_ck2n54(tpsph_handle *h, double *v, double *u, double dt, int64_t nsteps,
+                      double *store_v, double *store_u)
+{
+    if (!h->sol_dv) {
+        TRY(dalloc(h, &h->sol_dv, v_len(h)));
+        TRY(dalloc(h, &h->sol_tv, v_len(h)));
+        TRY(dalloc(h, &h->sol_tu, u_len(h)));
+    }
+    // every stage kernel also writes the cell-sorted records of the new state, so the kicks
+    // after the first skip the gather
+    bool gathered = false;
+    for (int64_t s = 0; s < nsteps; s++) {
+        for (int i = 0; i < 5; i++) {
+            if (store_v) {  // the state this stage's kick sees
+                const size_t k = (size_t)(5 * s + i);
+                HIP_TRY(h, hipMemcpyAsync(store_v + k * v_len(h), v, v_len(h) * sizeof(double),
+                                          hipMemcpyDeviceToDevice, h->stream));
+                HIP_TRY(h, hipMemcpyAsync(store_u + k * u_len(h), u, u_len(h) * sizeof(double),
+                                          hipMemcpyDeviceToDevice, h->stream));
+            }
+            TRY(kick_dispatch(h, h->sol_dv, v, u, gathered));
+            TimedScope ts(h, TPSPH_TIME_STAGE);
+            if (h->nd == 2)
+                hipLaunchKernelGGL((k_ck_stage_gather<2>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
+                    h->flag, h->bwork, h->P, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, kCkA[i], kCkB[i], dt,
+                    i == 0, h->iperm, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap,
+                    h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr);
+            else
+                hipLaunchKernelGGL((k_ck_stage_gather<3>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
+                    h->flag, h->bwork, h->P, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, kCkA[i], kCkB[i], dt,
+                    i == 0, h->iperm, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap,
+                    h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr);
+            TRY(launch_check(h, "k_ck_stage_gather"));
+            gathered = true;
+        }
+    }
+    // the state has advanced past the last kick: its pressures belong to no current state
+    h->have_kick = false;
+    return TPSPH_OK;
 }
 
 namespace {
@@ -2485,6 +2567,28 @@ int tpsph_kick_pairs_csr(tpsph_handle *h, int32_t pair, int64_t *offsets, int32_
     return emit_csr(h, nrows, launch, offsets, idx, capacity);
 }
 
+int tpsph_axpby(tpsph_handle *h, int64_t n, double *y, double a, const double *x, double b)
+{
+    if (!h) return fail(nullptr, TPSPH_ERR_ARGUMENT, "handle is NULL");
+    if (n < 0) return fail(h, TPSPH_ERR_ARGUMENT, "n must be non-negative");
+    if (n == 0) return TPSPH_OK;
+    if (!y || (b != 0.0 && !x)) return fail(h, TPSPH_ERR_ARGUMENT, "y (and x when b != 0) must not be NULL");
+    if (is_host(h)) {
+        for (int64_t i = 0; i < n; i++) {
+            if (b == 0.0)
+                y[i] = a == 0.0 ? 0.0 : a * y[i];
+            else if (a == 0.0)
+                y[i] = b * x[i];
+            else
+                y[i] = std::fma(b, x[i], a * y[i]);
+        }
+        return TPSPH_OK;
+    }
+    TimedScope ts(h, TPSPH_TIME_STAGE);
+    hipLaunchKernelGGL((k_axpby), dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, h->stream, n, y, a, x, b);
+    return launch_check(h, "k_axpby");
+}
+
 int tpsph_rk_stage(tpsph_handle *h, double *y, double *tmp, const double *k, double a,
                    double b, double dt, int64_t n, int32_t first_stage)
 {
@@ -2508,15 +2612,6 @@ int tpsph_rk_stage(tpsph_handle *h, double *y, double *tmp, const double *k, dou
 int tpsph_solve_ck2n54(tpsph_handle *h, double *v_ode, double *u_ode, double t0, double dt,
                        int64_t nsteps)
 {
-    static const double A[5] = {0.0, -567301805773.0 / 1357537059087.0,
-                                -2404267990393.0 / 2016746695238.0,
-                                -3550918686646.0 / 2091501179385.0,
-                                -1275806237668.0 / 842570457699.0};
-    static const double B[5] = {1432997174477.0 / 9575080441755.0,
-                                5161836677717.0 / 13612068292357.0,
-                                1720146321549.0 / 2090206949498.0,
-                                3134564353537.0 / 4481467310338.0,
-                                2277821191437.0 / 14882151754819.0};
     if (!h) return fail(nullptr, TPSPH_ERR_ARGUMENT, "handle is NULL");
     if (nsteps < 0) return fail(h, TPSPH_ERR_ARGUMENT, "nsteps must be non-negative");
     if (h->nf == 0 || nsteps == 0) return TPSPH_OK;
@@ -2530,34 +2625,7 @@ int tpsph_solve_ck2n54(tpsph_handle *h, double *v_ode, double *u_ode, double t0,
         v = h->st_v;
         u = h->st_u;
     }
-    if (!h->sol_dv) {
-        TRY(dalloc(h, &h->sol_dv, v_len(h)));
-        TRY(dalloc(h, &h->sol_tv, v_len(h)));
-        TRY(dalloc(h, &h->sol_tu, u_len(h)));
-    }
-    // every stage kernel also writes the cell-sorted records of the new state, so the kicks
-    // after the first skip the gather
-    bool gathered = false;
-    for (int64_t s = 0; s < nsteps; s++) {
-        for (int i = 0; i < 5; i++) {
-            TRY(kick_dispatch(h, h->sol_dv, v, u, gathered));
-            TimedScope ts(h, TPSPH_TIME_STAGE);
-            if (h->nd == 2)
-                hipLaunchKernelGGL((k_ck_stage_gather<2>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
-                    h->flag, h->bwork, h->P, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt,
-                    i == 0, h->iperm, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap,
-                    h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr);
-            else
-                hipLaunchKernelGGL((k_ck_stage_gather<3>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
-                    h->flag, h->bwork, h->P, h->nf, v, u, h->sol_tv, h->sol_tu, h->sol_dv, A[i], B[i], dt,
-                    i == 0, h->iperm, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol, h->snap,
-                    h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr);
-            TRY(launch_check(h, "k_ck_stage_gather"));
-            gathered = true;
-        }
-    }
-    // the state has advanced past the last kick: its pressures belong to no current state
-    h->have_kick = false;
+    TRY(run_ck2n54(h, v, u, dt, nsteps, nullptr, nullptr));
     if (is_host(h)) {
         HIP_TRY(h, hipMemcpyAsync(v_ode, v, v_len(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
         HIP_TRY(h, hipMemcpyAsync(u_ode, u, u_len(h) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -2699,7 +2767,9 @@ void tpsph_destroy(tpsph_handle *h)
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,
                     h->wtargets, h->iperm, h->dyn, h->vmax_part, h->wvel, h->ddgrad,
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
-                    h->sol_tu, h->integ, h->nrebuild};
+                    h->sol_tu, h->integ, h->nrebuild, h->adj_kb, h->adj_coef, h->adj_th,
+                    h->adj_gw, h->adj_red, h->adj_vb, h->adj_ub, h->adj_tvb, h->adj_tub,
+                    h->adj_ck, h->adj_st};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);

@@ -764,6 +764,19 @@ struct tpsph_handle {
     bool mixed = false;  // TPSPH_PRECISION_MIXED: FP32 pair terms in the brick kernel
     double *integ = nullptr;  // tpsph_integrals: block partials and the result
     int *nrebuild = nullptr;  // cell-list rebuilds since create (tpsph_rebuild_count)
+    // reverse mode (tpsph_adjoint.hip): cell-sorted adjoints of the rates {dv, drho}, the
+    // Adami coefficients of the walls, per-particle parameter partials (alpha, delta, g in the
+    // caller's order; g per wall particle), reduction partials, the reverse sweep's adjoints
+    // of (v, u) and of the stage registers, the checkpoints and one segment's stage states
+    double4 *adj_kb = nullptr;
+    double2 *adj_coef = nullptr;
+    double *adj_th = nullptr, *adj_gw = nullptr, *adj_red = nullptr;
+    double *adj_vb = nullptr, *adj_ub = nullptr, *adj_tvb = nullptr, *adj_tub = nullptr;
+    double *adj_ck = nullptr, *adj_st = nullptr;
+    size_t adj_ck_cap = 0, adj_st_cap = 0;  // doubles allocated
+    int64_t adj_nsteps = 0, adj_every = 0;
+    double adj_dt = 0.0;
+    bool adj_ready = false;  // tpsph_adjoint_forward has stored a trajectory
 
     bool timing = false;
     struct Rec {
@@ -877,6 +890,18 @@ inline int scan(tpsph_handle *h, const int *flag, const int *in, int *out, int n
 // gathered: the cell-sorted records already hold (v, u) (written by k_ck_stage_gather)
 int kick_dispatch(tpsph_handle *h, double *dv, const double *v, const double *u,
                   bool gathered = false);
+
+// A kick without its fluid pass: the records of (v, u) with the fluid pressures, the rebuild
+// of the cell order and the lists (when a particle left its skin), the Adami pass.
+int prepare_dispatch(tpsph_handle *h, const double *v, const double *u);
+
+// CarpenterKennedy2N54 (williamson_condition = false) coefficients A (A[0] unused) and B
+extern const double kCkA[5], kCkB[5];
+
+// nsteps CK2N54 steps of the device state (v, u) in place.  With store_v / store_u the state
+// every stage's kick sees is copied there first, stage k = 5 step + i at offset k v_len / u_len.
+int run_ck2n54(tpsph_handle *h, double *v, double *u, double dt, int64_t nsteps,
+               double *store_v, double *store_u);
 
 inline bool is_host(const tpsph_handle *h) { return h->params.memory == TPSPH_MEM_HOST; }
 inline size_t v_len(const tpsph_handle *h) { return (size_t)h->nf * (h->nd + 1); }

@@ -36,7 +36,8 @@ TIME_GATHER = 1
 TIME_WALL = 2
 TIME_FLUID = 3
 TIME_STAGE = 4
-TIMED_KERNELS = {"cell_list": 0, "gather": 1, "wall": 2, "fluid": 3, "stage": 4}
+TIME_ADJOINT = 5
+TIMED_KERNELS = {"cell_list": 0, "gather": 1, "wall": 2, "fluid": 3, "stage": 4, "adjoint": 5}
 
 EXPORTED_SYMBOLS = (
     "tpsph_abi_version", "tpsph_create", "tpsph_kick", "tpsph_drift", "tpsph_get_pressures",
@@ -44,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "tpsph_neighbors_csr", "tpsph_kick_pairs_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54", "tpsph_synchronize",
     "tpsph_enable_timing", "tpsph_kernel_time", "tpsph_last_error", "tpsph_destroy",
     "tpsph_kick_noalloc", "tpsph_integrals", "tpsph_rebuild_count",
+    "tpsph_kick_vjp", "tpsph_adjoint_forward", "tpsph_adjoint_reverse", "tpsph_axpby",
 )
 
 
@@ -126,9 +128,14 @@ def load(path: str | None = None):
     lib.tpsph_rk_stage.argtypes = [P, P, P, P, C.c_double, C.c_double, C.c_double, C.c_int64,
                                    C.c_int32]
     lib.tpsph_solve_ck2n54.argtypes = [P, P, P, C.c_double, C.c_double, C.c_int64]
+    lib.tpsph_axpby.argtypes = [P, C.c_int64, P, C.c_double, P, C.c_double]
     lib.tpsph_kick_noalloc.argtypes = [C.POINTER(TpsphParams), C.c_int64, C.c_int64, P, P, P,
                                        P, P, P, C.c_double, P]
     lib.tpsph_integrals.argtypes = [P, P, P, P]
+    lib.tpsph_kick_vjp.argtypes = [P, P, P, P, C.c_double, P, P, P]
+    lib.tpsph_adjoint_forward.argtypes = [P, P, P, C.c_double, C.c_double, C.c_int64,
+                                          C.c_int64, P, P]
+    lib.tpsph_adjoint_reverse.argtypes = [P, P, P, P, P, P]
     lib.tpsph_rebuild_count.argtypes = [P, C.POINTER(C.c_int64)]
     lib.tpsph_synchronize.argtypes = [P]
     lib.tpsph_enable_timing.argtypes = [P, C.c_int32]
@@ -140,7 +147,9 @@ def load(path: str | None = None):
     for name in ("tpsph_create", "tpsph_kick", "tpsph_drift", "tpsph_get_pressures",
                  "tpsph_neighbors_csr", "tpsph_kick_pairs_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54",
                  "tpsph_synchronize", "tpsph_enable_timing", "tpsph_kernel_time",
-                 "tpsph_kick_noalloc", "tpsph_integrals", "tpsph_rebuild_count"):
+                 "tpsph_kick_noalloc", "tpsph_integrals", "tpsph_rebuild_count",
+                 "tpsph_kick_vjp", "tpsph_adjoint_forward", "tpsph_adjoint_reverse",
+                 "tpsph_axpby"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

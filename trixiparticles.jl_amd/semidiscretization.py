@@ -403,13 +403,20 @@ class CarpenterKennedy2N54:
     williamson_condition: bool = False
 
 
-def solve(ode: DynamicalODEProblem, alg=None, *, dt, nsteps=None, callback=None):
+def solve(ode: DynamicalODEProblem, alg=None, *, dt, nsteps=None, callback=None,
+          stage_update="fused"):
     """Fixed-step ``solve(ode, CarpenterKennedy2N54(); dt, adaptive=false)``.
 
     Without ``callback`` the steps run entirely on the device in one library call
     (``tpsph_solve_ck2n54``).  With ``callback(v, u, t, step)`` every step is driven from
-    here through ``kick``/``drift`` and ``tpsph_rk_stage``, the same sequence OrdinaryDiffEq
-    runs.  Returns the final ``(v, u)`` (copies of the initial state, advanced)."""
+    here through ``kick``/``drift`` and the stage update, the sequence OrdinaryDiffEq runs:
+    ``stage_update="fused"`` updates each register and state in one ``tpsph_rk_stage``;
+    ``"broadcasts"`` issues the single broadcasts of LowStorageRK2N's ``perform_step!``
+    (``tmp = dt k``, ``u = u + B tmp``, ``tmp = A tmp``, ``tmp = tmp + dt k``) as
+    ``tpsph_axpby`` calls, as a device-array type does under OrdinaryDiffEq.  Returns the
+    final ``(v, u)`` (copies of the initial state, advanced)."""
+    if stage_update not in ("fused", "broadcasts"):
+        raise ValueError("stage_update must be 'fused' or 'broadcasts'")
     semi = ode.p["semi"]
     t0, t1 = ode.tspan
     if nsteps is None:
@@ -424,16 +431,28 @@ def solve(ode: DynamicalODEProblem, alg=None, *, dt, nsteps=None, callback=None)
     dv, du = semi._zeros(*v.shape), semi._zeros(*u.shape)
     tv, tu = semi._zeros(*v.shape), semi._zeros(*u.shape)
     t = t0
+
+    def axpby(y, a, x, b):
+        native.check(lib.tpsph_axpby(semi.handle, int(np.prod(y.shape)), _ptr(y), float(a),
+                                     _ptr(x), float(b)), semi.handle)
+
     for step in range(int(nsteps)):
         for i in range(5):
             ts = t + CK2N54_C[i] * dt
+            if stage_update == "broadcasts" and i > 0:
+                axpby(tv, CK2N54_A[i], tv, 0.0)  # tmp = A tmp
+                axpby(tu, CK2N54_A[i], tu, 0.0)
             ode.f1(dv, v, u, ode.p, ts)
             ode.f2(du, v, u, ode.p, ts)
             for y, tmp, k in ((v, tv, dv), (u, tu, du)):
-                native.check(lib.tpsph_rk_stage(semi.handle, _ptr(y), _ptr(tmp), _ptr(k),
-                                                CK2N54_A[i], CK2N54_B[i], float(dt),
-                                                int(np.prod(y.shape)), int(i == 0)),
-                             semi.handle)
+                if stage_update == "broadcasts":
+                    axpby(tmp, 0.0 if i == 0 else 1.0, k, dt)  # tmp = dt k / tmp + dt k
+                    axpby(y, 1.0, tmp, CK2N54_B[i])             # u = u + B tmp
+                else:
+                    native.check(lib.tpsph_rk_stage(semi.handle, _ptr(y), _ptr(tmp), _ptr(k),
+                                                    CK2N54_A[i], CK2N54_B[i], float(dt),
+                                                    int(np.prod(y.shape)), int(i == 0)),
+                                 semi.handle)
         t = t0 + (step + 1) * dt
         callback(v, u, t, step + 1)
     return v, u
