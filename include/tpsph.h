@@ -13,7 +13,8 @@
  *
  * State layout is the reference's (column-major, one column per particle, FP64):
  *   v_ode : (ndims+1) x n_fluid  = [v_1 .. v_ndims, rho] per particle
- *           (wcsph/system.jl:240-242, 258-279; the wall block of v_ode is empty)
+ *           (wcsph/system.jl:240-242, 258-279; the wall block of v_ode is empty);
+ *           ndims x n_fluid = [v_1 .. v_ndims] with SummationDensity (wcsph/system.jl:236)
  *   u_ode :  ndims    x n_fluid  = [x_1 .. x_ndims]       per particle
  *   dv_ode, du_ode: same shapes as v_ode, u_ode.
  * Results are always in the caller's particle order; the library's cell-sorted internal
@@ -41,7 +42,7 @@
 extern "C" {
 #endif
 
-#define TPSPH_ABI_VERSION 4
+#define TPSPH_ABI_VERSION 5
 
 enum tpsph_status {
     TPSPH_OK = 0,
@@ -82,6 +83,14 @@ enum tpsph_precision {
     TPSPH_PRECISION_MIXED = 1  /* FP32 pair arithmetic on brick-relative coordinates with  */
                                /* compensated density-rate sums; FP64 positions, state,    */
                                /* state equation, stage update and reductions              */
+};
+
+/* Density calculator of the fluid (tpsph_params.density_calculator). */
+enum tpsph_density_calculator {
+    TPSPH_DENSITY_CONTINUITY = 0, /* ContinuityDensity(): rho is integrated, v_ode row ndims+1 */
+    TPSPH_DENSITY_SUMMATION = 1   /* SummationDensity(): rho = sum_b m_b W_ab over the fluid */
+                                  /* and wall neighbours each kick; v_ode holds the velocity */
+                                  /* only (ndims rows) and dv_ode has no density rate        */
 };
 
 /* Kernel identifiers for tpsph_kernel_time. */
@@ -148,6 +157,19 @@ enum tpsph_timed_kernel {
  *                                      fluid-wall pairs with the no-slip wall velocity the
  *                                      Adami pass extrapolates (dummy_particles.jl:716-756);
  *                                      needs the fluid's ArtificialViscosityMonaghan
+ *   density_calculator                 WeaklyCompressibleSPHSystem(; density_calculator):
+ *                                      ContinuityDensity (density_calculators.jl:26) or
+ *                                      SummationDensity (density_calculators.jl:11, 26-48;
+ *                                      compute_density!, fluid.jl:133-139).  SummationDensity
+ *                                      changes the state layout (v_ode and dv_ode hold
+ *                                      ndims rows per particle, wcsph/system.jl:236-238,
+ *                                      252-256), adds the summation pass before the state
+ *                                      equation, takes the pressure acceleration
+ *                                      -m_b (p_a/rho_a^2 + p_b/rho_b^2) grad W
+ *                                      (pressure_acceleration.jl:8-15, 139-145) and has no
+ *                                      continuity or density-diffusion term (fluid.jl:
+ *                                      133-139); it runs the cell-walk pair search
+ *                                      (TPSPH_SEARCH_CELLS; AUTO selects it) in FP64
  */
 typedef struct tpsph_params {
     int32_t ndims;                 /* 2 or 3 */
@@ -180,7 +202,7 @@ typedef struct tpsph_params {
     double min_sound_speed;
     double max_sound_speed;
     int32_t wall_viscosity;        /* 0 nothing, 1 ArtificialViscosityMonaghan, 2 ViscosityAdami */
-    int32_t reserved1;
+    int32_t density_calculator;    /* enum tpsph_density_calculator, 0 = ContinuityDensity */
     double wall_alpha;             /* ArtificialViscosityMonaghan(; alpha, beta) of the walls */
     double wall_beta;
     double wall_epsilon;           /* epsilon of the wall model */
@@ -222,6 +244,11 @@ int tpsph_drift(tpsph_handle *h, double *du_ode, const double *v_ode, const doub
  * callbacks of the reference evaluate kick! on the saved state first, callbacks.jl:28-40). */
 int tpsph_get_pressures(tpsph_handle *h, double *fluid_pressure, double *wall_pressure,
                         double *wall_density);
+
+/* current_density(v, system) of the last kick (n_fluid values, caller order): the summed
+ * density (system.cache.density, wcsph/system.jl:269-273) under SummationDensity, the density
+ * row of v_ode under ContinuityDensity.  TPSPH_ERR_STATE as tpsph_get_pressures. */
+int tpsph_get_density(tpsph_handle *h, double *fluid_density);
 
 /* system_sound_speed(system) (wcsph/system.jl:285) as of the last kick: the sound speed the
  * state equation, viscosity and density diffusion used (with adaptive_sound_speed the value

@@ -4,7 +4,8 @@ TEST INFRASTRUCTURE ONLY: imported by ``tests/``, by ``__graft_entry__.smoke()``
 bench's ``cpu_baseline`` leg, as the checker and the CPU baseline; never by the product.
 
 Array conventions match the C oracle: coordinates ``(n, ndims)`` C-contiguous (= the
-reference's column-major ``ndims x n``), fluid state ``(n, ndims+1)``."""
+reference's column-major ``ndims x n``), fluid state ``(n, ndims+1)`` (``(n, ndims)`` with
+SummationDensity)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -37,7 +38,7 @@ class OracleParams(C.Structure):
         ("gravity", C.c_double * 3),
         ("pressure_offset", C.c_double),
         ("adaptive_sound_speed", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("density_calculator", C.c_int32),
         ("mach_number_target", C.c_double),
         ("min_sound_speed", C.c_double),
         ("max_sound_speed", C.c_double),
@@ -94,10 +95,14 @@ def lib():
         L.oracle_av_adami.argtypes = [C.c_int, d, d, d, d, P, d, P, d, d, d, d, P, P, P]
         L.oracle_av_adami.restype = None
         L.oracle_kick.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, P]
+        L.oracle_kick_rho.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, P, P]
+        L.oracle_summation_density.argtypes = [P, i64, P, P, i64, P, P, P]
         L.oracle_adaptive_sound_speed.argtypes = [P, i64, P]
         L.oracle_adaptive_sound_speed.restype = d
         L.oracle_drift.argtypes = [C.c_int, i64, P, P]
         L.oracle_drift.restype = None
+        L.oracle_drift_rows.argtypes = [C.c_int, C.c_int, i64, P, P]
+        L.oracle_drift_rows.restype = None
         L.oracle_solve_ck2n54.argtypes = [P, i64, P, P, P, i64, P, P, d, i64]
         L.oracle_dual_kick.argtypes = [P, i64, P, P, P, i64, P, P, P, P, P, P, P]
         L.oracle_dual_solve_ck2n54.argtypes = [P, i64, P, P, P, i64, P, P, d, i64, P, P, P, P,
@@ -140,6 +145,8 @@ def params_from_semi(semi) -> OracleParams:
             P.wall_viscosity, P.wall_nu = 2, wv.nu
         else:
             P.wall_viscosity, P.wall_alpha, P.wall_beta = 1, wv.alpha, wv.beta
+    if type(f.density_calculator).__name__ == "SummationDensity":
+        P.density_calculator = 1
     if hasattr(se, "mach_number_target"):  # StateEquationAdaptiveCole
         P.adaptive_sound_speed = 1
         P.mach_number_target = se.mach_number_target
@@ -273,20 +280,39 @@ def av_adami(nd, nu_a, nu_b, epsilon, h, pos_diff, grad_kernel, m_a, m_b, rho_a,
     return dv
 
 
-def kick(P, v, u, m, xw=None, mw=None):
-    """kick!: returns (dv, fluid pressure, wall pressure, wall density)."""
+def kick(P, v, u, m, xw=None, mw=None, return_rho=False):
+    """kick!: returns (dv, fluid pressure, wall pressure, wall density[, fluid density]).
+    v and dv are (nf, nd+1) with ContinuityDensity and (nf, nd) with SummationDensity
+    (``P.density_calculator == 1``), whose summed density ``return_rho`` adds."""
     v, u, m = _f64(v), _f64(u), _f64(m)
     nf = v.shape[0]
+    assert v.shape[1] == P.ndims + (0 if P.density_calculator == 1 else 1), v.shape
     nw = 0 if xw is None else xw.shape[0]
     xw = _f64(xw) if nw else None
     mw = _f64(mw) if nw else None
     dv = np.zeros_like(v)
     pf = np.zeros(nf)
+    rho = np.zeros(nf)
     pw, rw = np.zeros(max(nw, 1)), np.zeros(max(nw, 1))
-    rc = lib().oracle_kick(C.byref(P), nf, _p(v), _p(u), _p(m), nw, _p(xw), _p(mw), _p(dv),
-                           _p(pf), _p(pw), _p(rw))
+    rc = lib().oracle_kick_rho(C.byref(P), nf, _p(v), _p(u), _p(m), nw, _p(xw), _p(mw), _p(dv),
+                               _p(pf), _p(pw), _p(rw), _p(rho))
     assert rc == 0
+    if return_rho:
+        return dv, pf, pw[:nw], rw[:nw], rho
     return dv, pf, pw[:nw], rw[:nw]
+
+
+def summation_density(P, u, m, xw=None, mw=None):
+    """summation_density! of the fluid at positions u (fluid, then wall neighbours)."""
+    u, m = _f64(u), _f64(m)
+    nw = 0 if xw is None else xw.shape[0]
+    xw = _f64(xw) if nw else None
+    mw = _f64(mw) if nw else None
+    rho = np.zeros(u.shape[0])
+    rc = lib().oracle_summation_density(C.byref(P), u.shape[0], _p(u), _p(m), nw, _p(xw),
+                                        _p(mw), _p(rho))
+    assert rc == 0
+    return rho
 
 
 def adaptive_sound_speed(P, v):
@@ -296,9 +322,11 @@ def adaptive_sound_speed(P, v):
 
 
 def drift(nd, v):
+    """drift!: the velocity rows of v (nd + 1 rows with ContinuityDensity, nd with
+    SummationDensity)."""
     v = _f64(v)
     du = np.zeros((v.shape[0], nd))
-    lib().oracle_drift(nd, v.shape[0], _p(v), _p(du))
+    lib().oracle_drift_rows(nd, v.shape[1], v.shape[0], _p(v), _p(du))
     return du
 
 

@@ -52,7 +52,7 @@ typedef struct {
     double gravity[3];
     double pressure_offset;
     int32_t adaptive_sound_speed;
-    int32_t reserved0;
+    int32_t density_calculator;
     double mach_number_target;
     double min_sound_speed;
     double max_sound_speed;

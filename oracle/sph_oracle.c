@@ -60,7 +60,7 @@ typedef struct {
     double gravity[3];
     double pressure_offset;       /* AdamiPressureExtrapolation(pressure_offset=...) */
     int32_t adaptive_sound_speed; /* StateEquationAdaptiveCole: sound_speed set per kick */
-    int32_t reserved0;
+    int32_t density_calculator;   /* 0 = ContinuityDensity, 1 = SummationDensity */
     double mach_number_target;
     double min_sound_speed;
     double max_sound_speed;
@@ -663,6 +663,20 @@ int oracle_antuono_gradient(const oracle_params *P, int64_t nf, const double *x,
     return 0;
 }
 
+/* The scalar of the pressure acceleration (dv_pressure = factor * grad W) of the density
+ * calculator's default formulation (choose_pressure_acceleration_formulation,
+ * pressure_acceleration.jl:139-154):
+ *   ContinuityDensity  pressure_acceleration_continuity_density, -m_b (p_a + p_b)/(rho_a rho_b)
+ *                      (pressure_acceleration.jl:34-41)
+ *   SummationDensity   pressure_acceleration_summation_density,
+ *                      -m_b (p_a / rho_a^2 + p_b / rho_b^2) (pressure_acceleration.jl:8-15) */
+static double pressure_factor(const oracle_params *P, double m_b, double rho_a, double rho_b,
+                              double p_a, double p_b)
+{
+    if (P->density_calculator == 1) return -m_b * (p_a / (rho_a * rho_a) + p_b / (rho_b * rho_b));
+    return -m_b * ((p_a + p_b) / (rho_a * rho_b));
+}
+
 /* ---------------------------------------------------------------------------------------
  * WCSPH interact! (weakly_compressible_sph/rhs.jl:5-127) with ContinuityDensity, no
  * correction, pressure_acceleration_continuity_density (pressure_acceleration.jl:34-41),
@@ -703,14 +717,14 @@ int oracle_interact_fluid_fluid(const oracle_params *P, int64_t nf, const double
             for (int d = 0; d < nd; d++) v_b[d] = v[b * (nd + 1) + d];
             double rho_b = v[b * (nd + 1) + nd];
             double p_b = pf[b];
-            /* pressure_acceleration_continuity_density */
-            double pfac = -m_b * ((p_a + p_b) / (rho_a * rho_b));
+            double pfac = pressure_factor(P, m_b, rho_a, rho_b, p_a, p_b);
             for (int d = 0; d < nd; d++) dv_particle[d] += (pfac * grad_kernel[d]) * 1.0;
             /* dv_viscosity! -> ArtificialViscosityMonaghan (fluid neighbour) */
             if (P->use_viscosity)
                 oracle_av_monaghan(P, pos_diff, distance, m_b, rho_a, rho_b, v_a, v_b,
                                    grad_kernel, dv_particle);
-            /* continuity_equation! */
+            /* continuity_equation!: nothing with SummationDensity (fluid.jl:133-139) */
+            if (P->density_calculator == 1) continue;
             double v_diff[3] = {0, 0, 0};
             for (int d = 0; d < nd; d++) v_diff[d] = v_a[d] - v_b[d];
             drho_particle += (rho_a / rho_b) * m_b * dot3(nd, v_diff, grad_kernel);
@@ -777,7 +791,7 @@ int oracle_interact_fluid_wall(const oracle_params *P, int64_t nf, const double 
             double m_b = mw[b];
             double rho_b = rhow[b];
             double p_b = pw[b];
-            double pfac = -m_b * ((p_a + p_b) / (rho_a * rho_b));
+            double pfac = pressure_factor(P, m_b, rho_a, rho_b, p_a, p_b);
             for (int d = 0; d < nd; d++) dv_particle[d] += (pfac * grad_kernel[d]) * 1.0;
             if (P->wall_viscosity && wvel) {
                 const double *v_w = wvel + b * nd;
@@ -791,6 +805,7 @@ int oracle_interact_fluid_wall(const oracle_params *P, int64_t nf, const double 
                                     grad_kernel, m[a], m_b, rho_a, rho_b, v_a, v_w,
                                     dv_particle);
             }
+            if (P->density_calculator == 1) continue;
             double v_diff[3] = {0, 0, 0};
             for (int d = 0; d < nd; d++) v_diff[d] = v_a[d] - 0.0;
             drho_particle += (rho_a / rho_b) * m_b * dot3(nd, v_diff, grad_kernel);
@@ -826,9 +841,57 @@ double oracle_adaptive_sound_speed(const oracle_params *P, int64_t nf, const dou
     return c;
 }
 
-int oracle_kick(const oracle_params *P0, int64_t nf, const double *v, const double *x,
-                const double *m, int64_t nw, const double *xw, const double *mw, double *dv,
-                double *pf_out, double *pw_out, double *rhow_out)
+/* summation_density! (density_calculators.jl:26-48) of the fluid, with compute_density!
+ * (fluid.jl:133-139) under SummationDensity: set_zero!(density); then for every neighbour
+ * system the fluid interacts with, in the semidiscretization's order (foreach_system: the
+ * fluid itself, then the wall), every neighbour b within the compact support -- the particle
+ * itself included, whose distance 0 gives W(0) -- adds hydrodynamic_mass(b) * W(distance)
+ * (smoothing_kernel with the compact-support guard, smoothing_kernels.jl:30-34).  The wall
+ * neighbours' mass is boundary_model.hydrodynamic_mass (wall_boundary/system.jl:173-175).
+ * x: nd x nf, xw: nd x nw (nw may be 0), rho: nf out. */
+int oracle_summation_density(const oracle_params *P, int64_t nf, const double *x,
+                             const double *m, int64_t nw, const double *xw, const double *mw,
+                             double *rho)
+{
+    int nd = P->ndims;
+    double h = P->smoothing_length;
+    double R = 2.0 * h;
+    double lo[3], hi[3];
+    for (int64_t a = 0; a < nf; a++) rho[a] = 0.0;
+    bbox(nd, nf, x, nw, xw, lo, hi);
+    grid_t g;
+    if (grid_build(&g, nd, R, lo, hi, nf, x)) return -1;
+#pragma omp parallel for schedule(static)
+    for (int64_t a = 0; a < nf; a++) {
+        const double *xa = x + a * nd;
+        double density = rho[a];
+        FOREACH_NEIGHBOR(&g, xa, x, { density += m[b] * oracle_kernel(P->kernel, nd, distance, h); });
+        rho[a] = density;
+    }
+    grid_free(&g);
+    if (nw > 0) {
+        if (grid_build(&g, nd, R, lo, hi, nw, xw)) return -1;
+#pragma omp parallel for schedule(static)
+        for (int64_t a = 0; a < nf; a++) {
+            const double *xa = x + a * nd;
+            double density = rho[a];
+            FOREACH_NEIGHBOR(&g, xa, xw, { density += mw[b] * oracle_kernel(P->kernel, nd, distance, h); });
+            rho[a] = density;
+        }
+        grid_free(&g);
+    }
+    return 0;
+}
+
+/* Rows of v per fluid particle: velocity and density (ContinuityDensity), or the velocity
+ * alone (SummationDensity: v_nvariables, wcsph/system.jl:232-242). */
+static int v_rows(const oracle_params *P) { return P->ndims + (P->density_calculator == 1 ? 0 : 1); }
+
+/* kick! for ContinuityDensity: v and dv hold nd+1 rows per particle. */
+static int kick_continuity(const oracle_params *P0, int64_t nf, const double *v,
+                           const double *x, const double *m, int64_t nw, const double *xw,
+                           const double *mw, double *dv, double *pf_out, double *pw_out,
+                           double *rhow_out)
 {
     /* update_quantities! runs first in the kick (semidiscretization.jl:616-651): the
      * adaptive sound speed of this kick, used by the state equation (fluid and walls), the
@@ -851,7 +914,7 @@ int oracle_kick(const oracle_params *P0, int64_t nf, const double *v, const doub
         if (!wvel) { rc = -1; goto out; }
         if ((rc = oracle_wall_velocity(P, nw, xw, nf, x, v, wvel))) goto out;
     }
-    if (P->use_density_diffusion == 2) {
+    if (P->use_density_diffusion == 2 && P->density_calculator != 1) {
         /* update_quantities! -> update!(::DensityDiffusionAntuono) */
         ddgrad = (double *)malloc(sizeof(double) * (size_t)(nf * nd + 1));
         if (!ddgrad) { rc = -1; goto out; }
@@ -876,11 +939,65 @@ out:
     return rc;
 }
 
-/* drift! -> set_velocity!(::AbstractFluidSystem), semidiscretization.jl:561-571 */
-void oracle_drift(int nd, int64_t nf, const double *v, double *du)
+/* kick! (semidiscretization.jl:589-612).  ContinuityDensity: v, dv are (nd+1) x nf.
+ * SummationDensity: v, dv are nd x nf; compute_density! runs first in update_quantities!
+ * (wcsph/system.jl:291-301), and the pass order is otherwise the same, with the summed
+ * density in the state equation, the Adami pass and the pair terms (their density rows carry
+ * it here; there is no density rate).  rho_out (may be NULL): the density the kick used
+ * (system.cache.density under SummationDensity, the density row of v otherwise). */
+int oracle_kick_rho(const oracle_params *P, int64_t nf, const double *v, const double *x,
+                    const double *m, int64_t nw, const double *xw, const double *mw, double *dv,
+                    double *pf_out, double *pw_out, double *rhow_out, double *rho_out)
+{
+    int nd = P->ndims;
+    if (P->density_calculator != 1) {
+        int rc = kick_continuity(P, nf, v, x, m, nw, xw, mw, dv, pf_out, pw_out, rhow_out);
+        if (rc == 0 && rho_out)
+            for (int64_t a = 0; a < nf; a++) rho_out[a] = v[a * (nd + 1) + nd];
+        return rc;
+    }
+    size_t ne = (size_t)(nf > 0 ? nf : 1) * (size_t)(nd + 1);
+    double *vx = (double *)malloc(sizeof(double) * ne);
+    double *dvx = (double *)malloc(sizeof(double) * ne);
+    double *rho = (double *)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1));
+    int rc = -1;
+    if (vx && dvx && rho && oracle_summation_density(P, nf, x, m, nw, xw, mw, rho) == 0) {
+        for (int64_t a = 0; a < nf; a++) {
+            for (int d = 0; d < nd; d++) vx[a * (nd + 1) + d] = v[a * nd + d];
+            vx[a * (nd + 1) + nd] = rho[a];
+        }
+        rc = kick_continuity(P, nf, vx, x, m, nw, xw, mw, dvx, pf_out, pw_out, rhow_out);
+        if (rc == 0) {
+            for (int64_t a = 0; a < nf; a++)
+                for (int d = 0; d < nd; d++) dv[a * nd + d] = dvx[a * (nd + 1) + d];
+            if (rho_out) memcpy(rho_out, rho, sizeof(double) * (size_t)nf);
+        }
+    }
+    free(vx);
+    free(dvx);
+    free(rho);
+    return rc;
+}
+
+int oracle_kick(const oracle_params *P, int64_t nf, const double *v, const double *x,
+                const double *m, int64_t nw, const double *xw, const double *mw, double *dv,
+                double *pf_out, double *pw_out, double *rhow_out)
+{
+    return oracle_kick_rho(P, nf, v, x, m, nw, xw, mw, dv, pf_out, pw_out, rhow_out, NULL);
+}
+
+/* drift! -> set_velocity!(::AbstractFluidSystem), semidiscretization.jl:561-571: du = the
+ * velocity rows of v (vrows rows per particle: nd + 1 with ContinuityDensity, nd with
+ * SummationDensity). */
+void oracle_drift_rows(int nd, int vrows, int64_t nf, const double *v, double *du)
 {
     for (int64_t a = 0; a < nf; a++)
-        for (int d = 0; d < nd; d++) du[a * nd + d] = v[a * (nd + 1) + d] + 0.0;
+        for (int d = 0; d < nd; d++) du[a * nd + d] = v[a * vrows + d] + 0.0;
+}
+
+void oracle_drift(int nd, int64_t nf, const double *v, double *du)
+{
+    oracle_drift_rows(nd, nd + 1, nf, v, du);
 }
 
 /* ---------------------------------------------------------------------------------------
@@ -912,13 +1029,14 @@ void oracle_ck2n54_coefficients(double *A, double *B, double *C)
     for (int i = 0; i < 5; i++) { A[i] = CK_A[i]; B[i] = CK_B[i]; C[i] = CK_C[i]; }
 }
 
-/* Advance (v, u) by nsteps fixed steps of size dt.  v: (nd+1) x nf, u: nd x nf. */
+/* Advance (v, u) by nsteps fixed steps of size dt.  v: (nd+1) x nf (nd x nf with
+ * SummationDensity), u: nd x nf. */
 int oracle_solve_ck2n54(const oracle_params *P, int64_t nf, double *v, double *u,
                         const double *m, int64_t nw, const double *xw, const double *mw,
                         double dt, int64_t nsteps)
 {
     int nd = P->ndims;
-    int64_t nv = nf * (nd + 1), nu = nf * nd;
+    int64_t nv = nf * v_rows(P), nu = nf * nd;
     double *kv = (double *)malloc(sizeof(double) * (size_t)(nv > 0 ? nv : 1));
     double *ku = (double *)malloc(sizeof(double) * (size_t)(nu > 0 ? nu : 1));
     double *tv = (double *)calloc((size_t)(nv > 0 ? nv : 1), sizeof(double));
@@ -928,7 +1046,7 @@ int oracle_solve_ck2n54(const oracle_params *P, int64_t nf, double *v, double *u
     for (int64_t s = 0; s < nsteps; s++) {
         for (int i = 0; i < 5; i++) {
             if ((rc = oracle_kick(P, nf, v, u, m, nw, xw, mw, kv, NULL, NULL, NULL))) goto out;
-            oracle_drift(nd, nf, v, ku);
+            oracle_drift_rows(nd, v_rows(P), nf, v, ku);
             if (i == 0) {
 #pragma omp parallel for schedule(static)
                 for (int64_t k = 0; k < nv; k++) { tv[k] = dt * kv[k]; v[k] = fma(CK_B[0], tv[k], v[k]); }

@@ -22,7 +22,7 @@ def test_library_exports_every_declared_symbol(tp):
     for s in syms:
         assert hasattr(lib, s), f"{s} declared in include/tpsph.h but not exported"
     assert set(tp.native.EXPORTED_SYMBOLS) == set(syms)
-    assert lib.tpsph_abi_version() == 4
+    assert lib.tpsph_abi_version() == 5
 
 
 def test_library_exports_every_symbol_of_every_header(tp):
@@ -88,6 +88,8 @@ def _create(tp, **overrides):
     ("cell_divisions", 5, "cell_divisions"),
     ("use_density_diffusion", 3, "use_density_diffusion"),
     ("use_density_diffusion", -1, "use_density_diffusion"),
+    ("density_calculator", 2, "density_calculator"),
+    ("density_calculator", -1, "density_calculator"),
 ])
 def test_create_rejects_invalid_parameters(tp, field, value, msg):
     rc, err = _create(tp, **{field: value})
@@ -107,6 +109,14 @@ def test_create_rejects_invalid_adaptive_state_equation(tp, fields, msg):
     rc, err = _create(tp, adaptive_sound_speed=1, **fields)
     assert rc == tp.native.TPSPH_ERR_ARGUMENT
     assert msg in err
+
+
+@pytest.mark.parametrize("fields", [{"pair_search": 1}, {"precision": 1}])
+def test_create_rejects_summation_density_off_the_cell_walk(tp, fields):
+    """SummationDensity runs the cell-walk pair search in FP64."""
+    rc, err = _create(tp, density_calculator=1, **fields)
+    assert rc == tp.native.TPSPH_ERR_ARGUMENT
+    assert "SummationDensity" in err
 
 
 def test_semidiscretization_rejects_unknown_pair_search(tp):

@@ -9,7 +9,8 @@ from .schemes import (AdamiPressureExtrapolation, ArtificialViscosityMonaghan,
                       BoundaryModelDummyParticles, ContinuityDensity,
                       DensityDiffusionAntuono, DensityDiffusionMolteniColagrossi,
                       SchoenbergCubicSplineKernel,
-                      StateEquationAdaptiveCole, StateEquationCole, ViscosityAdami,
+                      StateEquationAdaptiveCole, StateEquationCole, SummationDensity,
+                      ViscosityAdami,
                       WallBoundarySystem, WeaklyCompressibleSPHSystem, WendlandC2Kernel)
 from .semidiscretization import (CK2N54_A, CK2N54_B, CK2N54_C, CarpenterKennedy2N54,
                                  DynamicalODEProblem, Semidiscretization, calculate_dt, drift,
@@ -22,7 +23,7 @@ __all__ = [
     "AdamiPressureExtrapolation", "ArtificialViscosityMonaghan", "BoundaryModelDummyParticles",
     "ContinuityDensity", "DensityDiffusionAntuono", "DensityDiffusionMolteniColagrossi",
     "SchoenbergCubicSplineKernel",
-    "StateEquationAdaptiveCole", "StateEquationCole", "ViscosityAdami", "WallBoundarySystem", "WeaklyCompressibleSPHSystem", "WendlandC2Kernel",
+    "StateEquationAdaptiveCole", "StateEquationCole", "SummationDensity", "ViscosityAdami", "WallBoundarySystem", "WeaklyCompressibleSPHSystem", "WendlandC2Kernel",
     "CK2N54_A", "CK2N54_B", "CK2N54_C", "CarpenterKennedy2N54", "DynamicalODEProblem",
     "Semidiscretization", "calculate_dt", "drift", "kick", "kick_pairs", "semidiscretize", "solve",
     "solve_tstops", "InitialCondition",

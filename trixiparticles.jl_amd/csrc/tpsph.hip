@@ -125,9 +125,10 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
 #pragma unroll
             for (int d = 0; d < ND; d++) {
                 x[d] = u[(size_t)i * ND + d];
-                vv[d] = v[(size_t)i * (ND + 1) + d];
+                vv[d] = v[(size_t)i * P.vrows + d];
             }
-            const double rho = v[(size_t)i * (ND + 1) + ND];
+            // SummationDensity: density and pressure are written by k_sum_finish
+            const double rho = P.summation ? 1.0 : v[(size_t)i * (ND + 1) + ND];
             const double mass = m[i];
             if (MODE == 1 || soa) {
                 xs[s] = x[0];
@@ -234,7 +235,9 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 
-template <int ND>
+// SUM: SummationDensity (the pressure acceleration of pair_fluid<ND, true>, velocity rows
+// of dv only).
+template <int ND, bool SUM = false>
 __global__ void __launch_bounds__(kBlock)
     k_fluid(int *flag, DevParams P, FluidView F, WallView Wv,
             const unsigned char *__restrict__ wall_near, double *__restrict__ dv)
@@ -259,16 +262,18 @@ __global__ void __launch_bounds__(kBlock)
         p_a = b.w;
     }
     const double inv_rho_a = 1.0 / rho_a;
+    if constexpr (SUM) p_a = p_a / (rho_a * rho_a);  // the target's p / rho^2
     double ga[3] = {0.0, 0.0, 0.0};  // DensityDiffusionAntuono: this particle's gradient
-    if (P.ddgrad && active) {
+    if (!SUM && P.ddgrad && active) {
 #pragma unroll
         for (int d = 0; d < ND; d++) ga[d] = P.ddgrad[(size_t)s * ND + d];
     }
 
     double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
-    neighbour_sweep<ND, false>(P, F.start, F.x, F.y, F.z, P.hs_fluid, active, F, Wv, lst, xa,
-                               va, rho_a, p_a, inv_rho_a, acc, drho, 0.0,
-                               P.ddgrad ? ga : nullptr);
+    neighbour_sweep<ND, false, kCap, kBlock, SUM>(P, F.start, F.x, F.y, F.z, P.hs_fluid, active,
+                                                  F, Wv, lst, xa, va, rho_a, p_a, inv_rho_a,
+                                                  acc, drho, 0.0,
+                                                  (!SUM && P.ddgrad) ? ga : nullptr);
     double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
     if (P.nw > 0) {
         // walls are static: only cells flagged at create can have wall neighbours
@@ -281,15 +286,72 @@ __global__ void __launch_bounds__(kBlock)
         }
         if (__any(near)) {
             const double m_a = (P.visc_w == 2 && active) ? F.vol[s].y : 1.0;
-            neighbour_sweep<ND, true>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, near, F, Wv,
-                                      lst, xa, va, rho_a, p_a, inv_rho_a, accw, drhow, m_a);
+            neighbour_sweep<ND, true, kCap, kBlock, SUM>(P, Wv.start, Wv.x, Wv.y, Wv.z,
+                                                         P.hs_wall, near, F, Wv, lst, xa, va,
+                                                         rho_a, p_a, inv_rho_a, accw, drhow, m_a);
         }
     }
     if (!active) return;
     const int i = F.perm[s];
+    constexpr int rows = SUM ? ND : ND + 1;
 #pragma unroll
-    for (int d = 0; d < ND; d++) dv[(size_t)i * (ND + 1) + d] = (acc[d] + accw[d]) + P.g[d];
-    dv[(size_t)i * (ND + 1) + ND] = drho + drhow;
+    for (int d = 0; d < ND; d++) dv[(size_t)i * rows + d] = (acc[d] + accw[d]) + P.g[d];
+    if constexpr (!SUM) dv[(size_t)i * rows + ND] = drho + drhow;
+}
+
+// SummationDensity (summation_density!, density_calculators.jl:26-48): the density of every
+// cell-sorted fluid particle s, sum_b m_b W(|x_s - x_b|) over the fluid particles within the
+// compact support (itself included, W(0)) and then over the wall particles, the neighbour
+// systems in the semidiscretization's order.  Written to rho (cell-sorted); k_sum_finish
+// then sets the records, so that no lane reads a record another lane writes.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_sum_density(DevParams P, FluidView F, WallView Wv,
+                  const unsigned char *__restrict__ wall_near, double *__restrict__ rho)
+{
+    const int s = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= P.nf) return;
+    const double x0 = F.x[s], x1 = F.y[s], x2 = (ND == 3) ? F.z[s] : 0.0;
+    double sum = 0.0;
+    Pencils<ND> it;
+    it.init(P, x0, x1, x2, P.hs_fluid);
+    int jb, je;
+    while (it.next(P, F.start, jb, je))
+        for (int j = jb; j < je; j++) {
+            const double d2 = dist2<ND>(x0 - F.x[j], x1 - F.y[j], (ND == 3) ? x2 - F.z[j] : 0.0);
+            if (d2 <= P.R2) sum += F.vol[j].y * wendland_c2(P, sqrt(d2));
+        }
+    if (P.nw > 0) {
+        const int c0 = cell_1d(x0, P.lo[0], P.inv_cs[0], P.n[0]);
+        const int c1 = cell_1d(x1, P.lo[1], P.inv_cs[1], P.n[1]);
+        const int c2 = (ND == 3) ? cell_1d(x2, P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+        if (wall_near[c0 + P.n[0] * (c1 + P.n[1] * c2)]) {
+            Pencils<ND> iw;
+            iw.init(P, x0, x1, x2, P.hs_wall);
+            while (iw.next(P, Wv.start, jb, je))
+                for (int w = jb; w < je; w++) {
+                    const double4 xw = Wv.pos[w];
+                    const double d2 = dist2<ND>(x0 - xw.x, x1 - xw.y, (ND == 3) ? x2 - xw.z : 0.0);
+                    if (d2 <= P.R2) sum += xw.w * wendland_c2(P, sqrt(d2));
+                }
+        }
+    }
+    rho[s] = sum;
+}
+
+// The records of the summed densities: rho, the state equation's pressure and V = m / rho.
+__global__ void __launch_bounds__(kBlock)
+    k_sum_finish(DevParams P, int nf, const double *__restrict__ rho, double4 *__restrict__ rec,
+                 double2 *__restrict__ vol)
+{
+    P = with_sound(P);
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= nf) return;
+    const double r = rho[s];
+    rec[2 * (size_t)s].w = r;
+    rec[2 * (size_t)s + 1].w = state_equation(P, r);
+    const double m = vol[s].y;
+    vol[s] = make_double2(m / r, m);
 }
 
 // Flag every cell that has a wall particle within R of some point of the cell (static).
@@ -1128,13 +1190,14 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // drift!: du = v (semidiscretization.jl:561-571; delta_v = 0 without shifting)
+// (vrows rows of v per particle: ND + 1, or ND under SummationDensity)
 template <int ND>
-__global__ void k_drift(int nf, const double *__restrict__ v, double *__restrict__ du)
+__global__ void k_drift(int nf, int vrows, const double *__restrict__ v, double *__restrict__ du)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nf) return;
 #pragma unroll
-    for (int d = 0; d < ND; d++) du[(size_t)i * ND + d] = v[(size_t)i * (ND + 1) + d];
+    for (int d = 0; d < ND; d++) du[(size_t)i * ND + d] = v[(size_t)i * vrows + d];
 }
 
 // One 2N-storage stage of the partition (v, u) with slopes (dv, v): the drift slope is read
@@ -1217,14 +1280,14 @@ __global__ void __launch_bounds__(kBlock)
 // ------------------------------------------------------------------------------------------
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
-    k_vmax(int nf, const double *__restrict__ v, double *__restrict__ part)
+    k_vmax(int nf, int vrows, const double *__restrict__ v, double *__restrict__ part)
 {
     __shared__ double s[kBlock];
     double m = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < nf; i += gridDim.x * kBlock) {
-        double d = v[(size_t)i * (ND + 1)] * v[(size_t)i * (ND + 1)];
+        double d = v[(size_t)i * vrows] * v[(size_t)i * vrows];
 #pragma unroll
-        for (int k = 1; k < ND; k++) d += v[(size_t)i * (ND + 1) + k] * v[(size_t)i * (ND + 1) + k];
+        for (int k = 1; k < ND; k++) d += v[(size_t)i * vrows + k] * v[(size_t)i * vrows + k];
         m = fmax(m, d);
     }
     s[threadIdx.x] = m;
@@ -1530,12 +1593,14 @@ __global__ void k_unpermute(int n, const int *__restrict__ perm, const T *__rest
     out[perm[s]] = sorted[s];
 }
 
-// fluid pressure of the cell-sorted records (p = .w of the second double4 of each record)
-__global__ void k_record_pressure(int n, const double4 *__restrict__ rec, double *__restrict__ p)
+// fluid pressure (p = .w of the second double4 of each record, half = 1) or density (.w of
+// the first, half = 0) of the cell-sorted records
+__global__ void k_record_pressure(int n, const double4 *__restrict__ rec, double *__restrict__ p,
+                                  int half = 1)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    p[s] = rec[2 * (size_t)s + 1].w;
+    p[s] = rec[2 * (size_t)s + half].w;
 }
 
 
@@ -1631,13 +1696,22 @@ int prepare_impl(tpsph_handle *h, const double *v, const double *u, bool gathere
         // records are then gathered again with it
         TimedScope t(h, TPSPH_TIME_GATHER);
         const int nb = gate_blocks(nf);
-        hipLaunchKernelGGL((k_vmax<ND>), dim3(nb), dim3(kBlock), 0, h->stream, nf, v, h->vmax_part);
+        hipLaunchKernelGGL((k_vmax<ND>), dim3(nb), dim3(kBlock), 0, h->stream, nf, h->P.vrows, v, h->vmax_part);
         hipLaunchKernelGGL((k_sound_speed), dim3(1), dim3(kBlock), 0, h->stream, h->P, nb,
                            h->vmax_part, h->dyn, h->params.alpha, h->params.delta);
         TRY(launch_check(h, "k_vmax/k_sound_speed"));
         gathered = false;
     }
     TRY(update_fluid<ND>(h, v, u, gathered));
+    if (h->P.summation && nf > 0) {
+        // compute_density! before the state equation (wcsph/system.jl:291-301, 323-326)
+        TimedScope t(h, TPSPH_TIME_GATHER);
+        hipLaunchKernelGGL((k_sum_density<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
+                           h->P, h->fluid_view(), h->wall_view(), h->wall_near, h->sum_rho);
+        hipLaunchKernelGGL((k_sum_finish), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, h->P,
+                           nf, h->sum_rho, h->rec, h->vol);
+        TRY(launch_check(h, "k_sum_density"));
+    }
     if (nw > 0) {
         TimedScope t(h, TPSPH_TIME_WALL);
         if (h->anbr)
@@ -1689,6 +1763,9 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
         } else if (h->search == TPSPH_SEARCH_LISTS) {
             hipLaunchKernelGGL((k_fluid_list<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
                 h->flag, h->P, h->fluid_view(), h->wall_view(), h->ord, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
+        } else if (h->P.summation) {
+            hipLaunchKernelGGL((k_fluid<ND, true>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
+                h->flag, h->P, h->fluid_view(), h->wall_view(), h->wall_near, dv);
         } else {
             hipLaunchKernelGGL((k_fluid<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, h->flag,
                 h->P, h->fluid_view(), h->wall_view(), h->wall_near, dv);
@@ -1717,7 +1794,10 @@ int tpsph::derive_params(const tpsph_params *params, int nf, int nw, const doubl
     P.nf = nf;
     P.nw = nw;
     P.use_visc = params->use_viscosity ? 1 : 0;
-    P.use_dd = params->use_density_diffusion ? 1 : 0;
+    P.summation = params->density_calculator == TPSPH_DENSITY_SUMMATION ? 1 : 0;
+    P.vrows = nd + (P.summation ? 0 : 1);
+    // density diffusion acts through the continuity equation, which SummationDensity has not
+    P.use_dd = (params->use_density_diffusion && !P.summation) ? 1 : 0;
     P.clip_fluid = params->clip_fluid_eos ? 1 : 0;
     P.clip_wall = params->clip_wall_pressure ? 1 : 0;
     P.h = hl;
@@ -1740,7 +1820,7 @@ int tpsph::derive_params(const tpsph_params *params, int nf, int nw, const doubl
     P.beta = params->beta;
     P.eps_h2 = params->epsilon * (hl * hl);
     P.av_c = params->alpha * P.c * hl;
-    P.dd2 = params->use_density_diffusion ? 2.0 * params->delta * hl * P.c : 0.0;
+    P.dd2 = P.use_dd ? 2.0 * params->delta * hl * P.c : 0.0;
     P.ddgrad = nullptr;  // set below for DensityDiffusionAntuono
     P.eps_hh = std::nextafter(hl * hl, INFINITY) - hl * hl;
     for (int d = 0; d < 3; d++) P.g[d] = d < nd ? params->acceleration[d] : 0.0;
@@ -1866,6 +1946,13 @@ int tpsph::validate(const tpsph_params *p, int64_t nf, int64_t nw, const double 
     if (!(p->skin < 1.0)) return bad("skin must be below 1 (a fraction of the compact support)");
     if (p->precision != TPSPH_PRECISION_FP64 && p->precision != TPSPH_PRECISION_MIXED)
         return bad("precision must be TPSPH_PRECISION_FP64 or TPSPH_PRECISION_MIXED");
+    if (p->density_calculator != TPSPH_DENSITY_CONTINUITY &&
+        p->density_calculator != TPSPH_DENSITY_SUMMATION)
+        return bad("density_calculator must be one of enum tpsph_density_calculator");
+    if (p->density_calculator == TPSPH_DENSITY_SUMMATION &&
+        (p->pair_search == TPSPH_SEARCH_LISTS || p->precision != TPSPH_PRECISION_FP64))
+        return bad("SummationDensity runs the cell-walk pair search (TPSPH_SEARCH_CELLS or "
+                   "AUTO) in FP64");
     if (nf > 0 && !fm) return bad("fluid_mass is NULL");
     if (nw > 0 && (!wx || !wm)) return bad("wall_coordinates / wall_mass is NULL");
     return TPSPH_OK;
@@ -1899,6 +1986,27 @@ int tpsph::run_ck2n54(tpsph_handle *h, double *v, double *u, double dt, int64_t 
         TRY(dalloc(h, &h->sol_dv, v_len(h)));
         TRY(dalloc(h, &h->sol_tv, v_len(h)));
         TRY(dalloc(h, &h->sol_tu, u_len(h)));
+    }
+    if (h->P.summation) {
+        // SummationDensity: v is the velocity, so the drift slope of u is v itself; the stage
+        // updates u (with the v of the stage state) before v, then the next kick gathers and
+        // sums the densities of the new state
+        const int64_t n = (int64_t)h->nf * h->nd;
+        const unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+        if (store_v) return fail(h, TPSPH_ERR_ARGUMENT, "reverse mode needs ContinuityDensity");
+        for (int64_t s = 0; s < nsteps; s++) {
+            for (int i = 0; i < 5; i++) {
+                TRY(kick_dispatch(h, h->sol_dv, v, u, false));
+                TimedScope ts(h, TPSPH_TIME_STAGE);
+                hipLaunchKernelGGL((k_rk_stage), dim3(grid), dim3(kBlock), 0, h->stream, n, u,
+                                   h->sol_tu, v, kCkA[i], kCkB[i], dt, (int)(i == 0));
+                hipLaunchKernelGGL((k_rk_stage), dim3(grid), dim3(kBlock), 0, h->stream, n, v,
+                                   h->sol_tv, h->sol_dv, kCkA[i], kCkB[i], dt, (int)(i == 0));
+                TRY(launch_check(h, "k_rk_stage"));
+            }
+        }
+        h->have_kick = false;
+        return TPSPH_OK;
     }
     // every stage kernel also writes the cell-sorted records of the new state, so the kicks
     // after the first skip the gather
@@ -2190,7 +2298,7 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     const int64_t ncells = P.ncells;
     h->search = params->pair_search;
     if (h->search == TPSPH_SEARCH_AUTO)
-        h->search = P.half_skin > 0.0 ? TPSPH_SEARCH_LISTS : TPSPH_SEARCH_CELLS;
+        h->search = (P.half_skin > 0.0 && !P.summation) ? TPSPH_SEARCH_LISTS : TPSPH_SEARCH_CELLS;
     if (h->search == TPSPH_SEARCH_LISTS) {
         // capacities: 1.5 x the expected count in the list ball (particle volume m / rho0)
         // plus 16; walls fill at most 7/8 (3D) or 3/4 (2D) of the ball around a corner
@@ -2255,10 +2363,11 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         return cleanup_fail(TPSPH_ERR_HIP);
     }
     if (params->wall_viscosity && nw > 0) CTRY(dalloc(h, &h->wvel, 3 * (size_t)nw));
-    if (params->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO && nf > 0) {
+    if (params->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO && !P.summation && nf > 0) {
         CTRY(dalloc(h, &h->ddgrad, (size_t)nd * nf));
         P.ddgrad = h->ddgrad;
     }
+    if (P.summation) CTRY(dalloc(h, &h->sum_rho, (size_t)std::max(nf, 1)));
     if (params->adaptive_sound_speed) {
         CTRY(dalloc(h, &h->dyn, 4));
         CTRY(dalloc(h, &h->vmax_part, kGateBlocks));
@@ -2421,14 +2530,14 @@ int tpsph_drift(tpsph_handle *h, double *du_ode, const double *v_ode, const doub
     if (!du_ode || !v_ode) return fail(h, TPSPH_ERR_ARGUMENT, "du_ode and v_ode must not be NULL");
     if (!is_host(h)) {
         if (h->nd == 2)
-            hipLaunchKernelGGL((k_drift<2>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, v_ode, du_ode);
+            hipLaunchKernelGGL((k_drift<2>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, h->P.vrows, v_ode, du_ode);
         else
-            hipLaunchKernelGGL((k_drift<3>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, v_ode, du_ode);
+            hipLaunchKernelGGL((k_drift<3>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream, h->nf, h->P.vrows, v_ode, du_ode);
         return launch_check(h, "k_drift");
     }
     for (int i = 0; i < h->nf; i++)
         for (int d = 0; d < h->nd; d++)
-            du_ode[(size_t)i * h->nd + d] = v_ode[(size_t)i * (h->nd + 1) + d];
+            du_ode[(size_t)i * h->nd + d] = v_ode[(size_t)i * h->P.vrows + d];
     return TPSPH_OK;
 }
 
@@ -2458,6 +2567,32 @@ int tpsph_get_pressures(tpsph_handle *h, double *fluid_pressure, double *wall_pr
     if (rc == TPSPH_OK) rc = emit(wall_density, h->rhow, h->w_perm, h->nw);
     (void)hipFree(tmp);
     (void)hipFree(pf_sorted);
+    return rc;
+}
+
+int tpsph_get_density(tpsph_handle *h, double *fluid_density)
+{
+    if (!h) return fail(nullptr, TPSPH_ERR_ARGUMENT, "handle is NULL");
+    if (!fluid_density) return fail(h, TPSPH_ERR_ARGUMENT, "fluid_density is NULL");
+    if (!h->have_kick) return fail(h, TPSPH_ERR_STATE, "no kick has been evaluated yet");
+    if (h->nf == 0) return TPSPH_OK;
+    double *sorted = nullptr, *tmp = nullptr;
+    TRY(dalloc(h, &sorted, (size_t)h->nf));
+    int rc = dalloc(h, &tmp, (size_t)h->nf);
+    if (rc == TPSPH_OK) {
+        hipLaunchKernelGGL((k_record_pressure), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
+                           h->nf, h->rec, sorted, 0);
+        hipLaunchKernelGGL((k_unpermute<double>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
+                           h->nf, h->f_perm, sorted, tmp);
+        rc = launch_check(h, "k_record_pressure/k_unpermute");
+    }
+    if (rc == TPSPH_OK &&
+        (hipMemcpyAsync(fluid_density, tmp, (size_t)h->nf * sizeof(double), hipMemcpyDefault,
+                        h->stream) != hipSuccess ||
+         hipStreamSynchronize(h->stream) != hipSuccess))
+        rc = fail(h, TPSPH_ERR_HIP, "reading the densities failed");
+    (void)hipFree(sorted);
+    (void)hipFree(tmp);
     return rc;
 }
 
@@ -2644,9 +2779,10 @@ int tpsph_kick_noalloc(const tpsph_params *params, int64_t n_fluid, int64_t n_wa
     auto bad = [](const std::string &m) { return fail(nullptr, TPSPH_ERR_ARGUMENT, "tpsph_kick_noalloc: " + m); };
     if (params->memory != TPSPH_MEM_DEVICE) return bad("needs device pointers (TPSPH_MEM_DEVICE)");
     if (params->adaptive_sound_speed || params->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO ||
-        params->wall_viscosity || params->precision != TPSPH_PRECISION_FP64)
-        return bad("supports StateEquationCole, Molteni-Colagrossi diffusion, walls without "
-                   "viscosity and FP64");
+        params->wall_viscosity || params->precision != TPSPH_PRECISION_FP64 ||
+        params->density_calculator != TPSPH_DENSITY_CONTINUITY)
+        return bad("supports ContinuityDensity, StateEquationCole, Molteni-Colagrossi diffusion, "
+                   "walls without viscosity and FP64");
     if (n_fluid > 0 && (!dv_ode || !v_ode || !u_ode))
         return bad("dv_ode, v_ode and u_ode must not be NULL");
     double lo[3] = {0.0, 0.0, 0.0}, hi[3] = {0.0, 0.0, 0.0};
@@ -2686,6 +2822,9 @@ int tpsph_integrals(tpsph_handle *h, const double *v_ode, const double *u_ode, d
     if (!h) return fail(nullptr, TPSPH_ERR_ARGUMENT, "handle is NULL");
     if (!out) return fail(h, TPSPH_ERR_ARGUMENT, "out must not be NULL");
     if (h->nf > 0 && (!v_ode || !u_ode)) return fail(h, TPSPH_ERR_ARGUMENT, "v_ode and u_ode must not be NULL");
+    if (h->P.summation)
+        return fail(h, TPSPH_ERR_ARGUMENT, "tpsph_integrals reads the density rows of v_ode "
+                                           "(ContinuityDensity)");
     const double *v = v_ode, *u = u_ode;
     if (is_host(h) && h->nf > 0) {
         TRY(ensure_staging(h));
@@ -2769,7 +2908,7 @@ void tpsph_destroy(tpsph_handle *h)
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu, h->integ, h->nrebuild, h->adj_kb, h->adj_coef, h->adj_th,
                     h->adj_gw, h->adj_red, h->adj_vb, h->adj_ub, h->adj_tvb, h->adj_tub,
-                    h->adj_ck, h->adj_st};
+                    h->adj_ck, h->adj_st, h->sum_rho};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);

@@ -405,10 +405,12 @@ int check_supported(tpsph_handle *h)
     if (is_host(h))
         return fail(h, TPSPH_ERR_ARGUMENT, "reverse mode needs device pointers (TPSPH_MEM_DEVICE)");
     if (p.adaptive_sound_speed || p.use_density_diffusion == TPSPH_DIFFUSION_ANTUONO ||
-        p.wall_viscosity || p.precision != TPSPH_PRECISION_FP64)
+        p.wall_viscosity || p.precision != TPSPH_PRECISION_FP64 ||
+        p.density_calculator != TPSPH_DENSITY_CONTINUITY)
         return fail(h, TPSPH_ERR_ARGUMENT,
-                    "reverse mode supports StateEquationCole with a fixed sound speed, "
-                    "Molteni-Colagrossi or no density diffusion, walls without viscosity and FP64");
+                    "reverse mode supports ContinuityDensity, StateEquationCole with a fixed "
+                    "sound speed, Molteni-Colagrossi or no density diffusion, walls without "
+                    "viscosity and FP64");
     return TPSPH_OK;
 }
 

@@ -384,9 +384,10 @@ int tpsph_batch_create(const tpsph_params *params, int32_t n_cases, const tpsph_
     if (params->memory != TPSPH_MEM_DEVICE)
         return fail(nullptr, TPSPH_ERR_ARGUMENT, "the ensemble takes device pointers (TPSPH_MEM_DEVICE)");
     if (params->adaptive_sound_speed || params->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO ||
-        params->wall_viscosity || params->precision != TPSPH_PRECISION_FP64)
+        params->wall_viscosity || params->precision != TPSPH_PRECISION_FP64 ||
+        params->density_calculator != TPSPH_DENSITY_CONTINUITY)
         return fail(nullptr, TPSPH_ERR_ARGUMENT,
-                    "the ensemble runs StateEquationCole, Monaghan viscosity, "
+                    "the ensemble runs ContinuityDensity, StateEquationCole, Monaghan viscosity, "
                     "Molteni-Colagrossi or no diffusion, walls without viscosity, FP64");
     int64_t nf = 0, nw = 0;
     for (int c = 0; c < n_cases; c++) {

@@ -79,6 +79,10 @@ struct DevParams {
     // gradient-correction pass (smoothing_kernel_grad, abstract_system.jl:142-155)
     double *ddgrad;
     double eps_hh;
+    // SummationDensity: every kick sums the density of each fluid particle over its fluid and
+    // wall neighbours (k_sum_density) instead of integrating it; v and dv then hold the
+    // velocity rows only.  vrows: rows of v and dv per fluid particle (ND + 1, or ND)
+    int summation, vrows;
 };
 
 // The sound-speed dependent constants of this kick: P's own, or the adaptive ones.  Kernels
@@ -362,7 +366,10 @@ __device__ __forceinline__ WallNbr load_wall(const WallView &Wv, int j)
 // psi_ab = 2 (rho_a - rho_b - (grad rho_a + grad rho_b).r_ab / 2) r_ab / r^2
 // (density_diffusion.jl:155-174) replaces the Molteni-Colagrossi 2 (rho_a - rho_b);
 // nullptr for Molteni-Colagrossi (or no density diffusion).
-template <int ND>
+// SUM: SummationDensity's pressure acceleration -m_b (p_a/rho_a^2 + p_b/rho_b^2) grad W
+// (pressure_acceleration.jl:8-15) with p_a = the target's p/rho^2 passed in the p_a slot;
+// the density rate is not used then (no continuity equation, fluid.jl:133-139).
+template <int ND, bool SUM = false>
 __device__ __forceinline__ void pair_fluid(const DevParams &P, const FluidNbr &n,
                                            const double *xa, const double *va, double rho_a,
                                            double p_a, double inv_rho_a, double *acc,
@@ -377,7 +384,7 @@ __device__ __forceinline__ void pair_fluid(const DevParams &P, const FluidNbr &n
     double vr = v0 * p0 + v1 * p1;
     if constexpr (ND == 3) vr += v2 * p2;
     const double rho_b = n.a.w;
-    double fac = -n.v.x * inv_rho_a * (p_a + n.b.w);
+    double fac = SUM ? -n.v.y * (p_a + n.b.w / (rho_b * rho_b)) : -n.v.x * inv_rho_a * (p_a + n.b.w);
     if (P.use_visc && vr < 0.0) {
         // (m_b alpha c mu + m_b beta mu^2) / rho_mean, rho_mean = (rho_a + rho_b) / 2
         const double q = d2 + P.eps_h2;
@@ -407,7 +414,8 @@ __device__ __forceinline__ void pair_fluid(const DevParams &P, const FluidNbr &n
 // (viscosity.jl:219-280, with eta_a = nu_a rho_a from the fluid's kinematic viscosity and
 // eta_b = nu rho_b, viscosity_model dispatch of wall_boundary/system.jl:286-289 and
 // fluid.jl:109-112); v_b is the wall's no-slip velocity and rho_b = m_b / V_b.
-template <int ND>
+// SUM: as in pair_fluid, with the wall's rho_b = m_b / V_b of the Adami pass.
+template <int ND, bool SUM = false>
 __device__ __forceinline__ void pair_wall(const DevParams &P, const WallNbr &n,
                                           const double *xa, const double *va, double rho_a,
                                           double p_a, double inv_rho_a, double *acc,
@@ -420,7 +428,13 @@ __device__ __forceinline__ void pair_wall(const DevParams &P, const WallNbr &n,
     const double kdr = P.kgrad_c * (w * w * w);
     double vr = va[0] * p0 + va[1] * p1;
     if constexpr (ND == 3) vr += va[2] * p2;
-    double fac = -n.pv.y * inv_rho_a * (p_a + n.pv.x) * kdr;
+    double fac;
+    if constexpr (SUM) {
+        const double rho_w = n.a.w / n.pv.y;
+        fac = -n.a.w * (p_a + n.pv.x / (rho_w * rho_w)) * kdr;
+    } else {
+        fac = -n.pv.y * inv_rho_a * (p_a + n.pv.x) * kdr;
+    }
     if (P.visc_w) {
         const double v0 = va[0] - n.v[0], v1 = va[1] - n.v[1];
         const double v2 = (ND == 3) ? va[2] - n.v[2] : 0.0;
@@ -496,7 +510,7 @@ __device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const Flui
 // candidates per iteration without branches: the arrays cx/cy/cz carry kPad NaN entries
 // past their end, so reads beyond a range stay in bounds and never pass the test, and every
 // candidate is written to the list while the count only advances on a hit.
-template <int ND, bool WALL, int CAP = kCap, int STRIDE = kBlock>
+template <int ND, bool WALL, int CAP = kCap, int STRIDE = kBlock, bool SUM = false>
 __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *__restrict__ start,
                                                 const double *__restrict__ cx,
                                                 const double *__restrict__ cy,
@@ -542,14 +556,14 @@ __device__ __forceinline__ void neighbour_sweep(const DevParams &P, const int *_
                 WallNbr cur = load_wall(Wv, lst[0]);
                 for (int k = 0; k < cnt; k++) {
                     const WallNbr nxt = load_wall(Wv, lst[min(k + 1, cnt - 1) * STRIDE]);
-                    pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, m_a);
+                    pair_wall<ND, SUM>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, m_a);
                     cur = nxt;
                 }
             } else {
                 FluidNbr cur = load_fluid(F, lst[0]);
                 for (int k = 0; k < cnt; k++) {
                     const FluidNbr nxt = load_fluid(F, lst[min(k + 1, cnt - 1) * STRIDE]);
-                    pair_fluid<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, ga);
+                    pair_fluid<ND, SUM>(P, cur, xa, va, rho_a, p_a, inv_rho_a, acc, drho, ga);
                     cur = nxt;
                 }
             }
@@ -750,6 +764,7 @@ struct tpsph_handle {
     double *wvel = nullptr;      // no-slip wall velocities (3 per wall particle)
     double *ddgrad = nullptr;    // DensityDiffusionAntuono: renormalised density gradients
     double *vmax_part = nullptr; // per-block maxima of v.v (kGateBlocks)
+    double *sum_rho = nullptr;   // SummationDensity: summed densities (cell-sorted)
     // Verlet lists of the wall particles over the fluid (Adami pass), active wall particles
     int *anbr = nullptr, *acnt = nullptr, *wactive = nullptr;
     int lw = 0, cap_a = 0;
@@ -904,7 +919,7 @@ int run_ck2n54(tpsph_handle *h, double *v, double *u, double dt, int64_t nsteps,
                double *store_v, double *store_u);
 
 inline bool is_host(const tpsph_handle *h) { return h->params.memory == TPSPH_MEM_HOST; }
-inline size_t v_len(const tpsph_handle *h) { return (size_t)h->nf * (h->nd + 1); }
+inline size_t v_len(const tpsph_handle *h) { return (size_t)h->nf * h->P.vrows; }
 inline size_t u_len(const tpsph_handle *h) { return (size_t)h->nf * h->nd; }
 
 inline int ensure_staging(tpsph_handle *h)

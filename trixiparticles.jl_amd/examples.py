@@ -41,7 +41,7 @@ def dam_break(ndims=2, particle_spacing=0.01, fluid_size=None, tank_size=None, *
               alpha=0.02, delta=0.1, gravity=9.81, n_layers=3, open_top=True,
               jitter=0.01, seed=0, smoothing_factor=1.2, exponent=7.0, water_height=0.6,
               clip_wall_pressure=True, adaptive_sound_speed=False, wall_viscosity=None,
-              density_diffusion=None):
+              density_diffusion=None, density_calculator=None):
     """Dam break with the workload defaults: 2D 1.2 x 0.6 m water in a 3.2 x 1.0 m tank, 3D
     1.0 x 0.5 x 0.5 m water in a 3.2 x 1.0 x 1.0 m tank.  ``adaptive_sound_speed`` selects
     the StateEquationAdaptiveCole of examples/fluid/dam_break_3d.jl (its defaults: Mach 0.1,
@@ -49,7 +49,8 @@ def dam_break(ndims=2, particle_spacing=0.01, fluid_size=None, tank_size=None, *
     is the wall model's viscosity (no-slip walls; ``None`` = free slip, as in the examples).
     ``density_diffusion`` replaces DensityDiffusionMolteniColagrossi(delta), e.g. by
     DensityDiffusionAntuono(delta=0.1), the alternative examples/fluid/dam_break_2d.jl:62-66
-    names."""
+    names.  ``density_calculator`` replaces ContinuityDensity(), e.g. by SummationDensity()
+    (whose state has no density rows and whose kick has no density diffusion term)."""
     if fluid_size is None:
         fluid_size = (1.2, 0.6) if ndims == 2 else (1.0, 0.5, 0.5)
     if tank_size is None:
@@ -69,7 +70,8 @@ def dam_break(ndims=2, particle_spacing=0.01, fluid_size=None, tank_size=None, *
     h = smoothing_factor * particle_spacing
     kernel = WendlandC2Kernel(ndims)
     fluid_system = WeaklyCompressibleSPHSystem(
-        tank.fluid, ContinuityDensity(), state_equation, kernel, h,
+        tank.fluid, ContinuityDensity() if density_calculator is None else density_calculator,
+        state_equation, kernel, h,
         acceleration=acceleration, viscosity=ArtificialViscosityMonaghan(alpha=alpha, beta=0.0),
         density_diffusion=DensityDiffusionMolteniColagrossi(delta=delta)
         if density_diffusion is None else density_diffusion)

@@ -31,6 +31,9 @@ PAIR_SEARCH = {"auto": SEARCH_AUTO, "lists": SEARCH_LISTS, "cells": SEARCH_CELLS
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 
+DENSITY_CONTINUITY = 0
+DENSITY_SUMMATION = 1
+
 TIME_CELL_LIST = 0
 TIME_GATHER = 1
 TIME_WALL = 2
@@ -41,7 +44,7 @@ TIMED_KERNELS = {"cell_list": 0, "gather": 1, "wall": 2, "fluid": 3, "stage": 4,
 
 EXPORTED_SYMBOLS = (
     "tpsph_abi_version", "tpsph_create", "tpsph_kick", "tpsph_drift", "tpsph_get_pressures",
-    "tpsph_get_sound_speed",
+    "tpsph_get_density", "tpsph_get_sound_speed",
     "tpsph_neighbors_csr", "tpsph_kick_pairs_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54", "tpsph_synchronize",
     "tpsph_enable_timing", "tpsph_kernel_time", "tpsph_last_error", "tpsph_destroy",
     "tpsph_kick_noalloc", "tpsph_integrals", "tpsph_rebuild_count",
@@ -83,7 +86,7 @@ class TpsphParams(C.Structure):
         ("min_sound_speed", C.c_double),
         ("max_sound_speed", C.c_double),
         ("wall_viscosity", C.c_int32),
-        ("reserved1", C.c_int32),
+        ("density_calculator", C.c_int32),
         ("wall_alpha", C.c_double),
         ("wall_beta", C.c_double),
         ("wall_epsilon", C.c_double),
@@ -122,6 +125,7 @@ def load(path: str | None = None):
     lib.tpsph_kick.argtypes = [P, P, P, P, C.c_double]
     lib.tpsph_drift.argtypes = [P, P, P, P, C.c_double]
     lib.tpsph_get_pressures.argtypes = [P, P, P, P]
+    lib.tpsph_get_density.argtypes = [P, P]
     lib.tpsph_get_sound_speed.argtypes = [P, dp]
     lib.tpsph_neighbors_csr.argtypes = [P, P, C.c_int32, P, P, C.c_int64]
     lib.tpsph_kick_pairs_csr.argtypes = [P, C.c_int32, P, P, C.c_int64]
@@ -145,6 +149,7 @@ def load(path: str | None = None):
     lib.tpsph_destroy.argtypes = [P]
     lib.tpsph_destroy.restype = None
     for name in ("tpsph_create", "tpsph_kick", "tpsph_drift", "tpsph_get_pressures",
+                 "tpsph_get_density", "tpsph_get_sound_speed",
                  "tpsph_neighbors_csr", "tpsph_kick_pairs_csr", "tpsph_rk_stage", "tpsph_solve_ck2n54",
                  "tpsph_synchronize", "tpsph_enable_timing", "tpsph_kernel_time",
                  "tpsph_kick_noalloc", "tpsph_integrals", "tpsph_rebuild_count",

@@ -10,7 +10,7 @@ Reference constructors mirrored here:
 * ``ArtificialViscosityMonaghan`` -- ``src/schemes/fluid/viscosity.jl:68-132``;
 * ``DensityDiffusionMolteniColagrossi`` -- ``weakly_compressible_sph/density_diffusion.jl:41-55``;
 * ``DensityDiffusionAntuono`` -- ``weakly_compressible_sph/density_diffusion.jl:89-211``;
-* ``ContinuityDensity`` -- ``src/general/density_calculators.jl:188``;
+* ``ContinuityDensity`` / ``SummationDensity`` -- ``src/general/density_calculators.jl:1-48``;
 * ``WeaklyCompressibleSPHSystem`` -- ``weakly_compressible_sph/system.jl:65-174``;
 * ``AdamiPressureExtrapolation`` / ``BoundaryModelDummyParticles`` --
   ``src/schemes/boundary/wall_boundary/dummy_particles.jl:52-150``;
@@ -32,7 +32,7 @@ from .setups import InitialCondition
 __all__ = [
     "WendlandC2Kernel", "SchoenbergCubicSplineKernel", "StateEquationCole",
     "ArtificialViscosityMonaghan", "DensityDiffusionMolteniColagrossi",
-    "DensityDiffusionAntuono", "ContinuityDensity",
+    "DensityDiffusionAntuono", "ContinuityDensity", "SummationDensity",
     "WeaklyCompressibleSPHSystem", "AdamiPressureExtrapolation",
     "BoundaryModelDummyParticles", "WallBoundarySystem",
 ]
@@ -195,7 +195,22 @@ def _julia_float(x: float) -> str:
 
 
 class ContinuityDensity:
-    """``ContinuityDensity()`` marker (density_calculators.jl:188)."""
+    """``ContinuityDensity()`` marker (density_calculators.jl:14-24): the density is
+    integrated (row ndims+1 of ``v_ode``)."""
+
+    def __repr__(self):
+        return "ContinuityDensity()"
+
+
+class SummationDensity:
+    """``SummationDensity()`` marker (density_calculators.jl:1-11): the density is summed over
+    the fluid and wall neighbours every kick (``summation_density!``,
+    density_calculators.jl:26-48); ``v_ode`` holds the velocity alone
+    (wcsph/system.jl:236-238) and the pressure acceleration is
+    ``pressure_acceleration_summation_density`` (pressure_acceleration.jl:8-15)."""
+
+    def __repr__(self):
+        return "SummationDensity()"
 
 
 class AdamiPressureExtrapolation:
@@ -270,8 +285,9 @@ class WeaklyCompressibleSPHSystem:
             raise ValueError(f"`acceleration` must be of length {nd} for a {nd}D problem")
         if density_calculator is None:
             density_calculator = ContinuityDensity()
-        if not isinstance(density_calculator, ContinuityDensity):
-            raise NotImplementedError("the HIP right-hand side integrates ContinuityDensity")
+        if not isinstance(density_calculator, (ContinuityDensity, SummationDensity)):
+            raise NotImplementedError("the HIP right-hand side implements ContinuityDensity "
+                                      "and SummationDensity")
         if not isinstance(state_equation, StateEquationCole):
             raise NotImplementedError("the HIP right-hand side implements StateEquationCole "
                                       "and StateEquationAdaptiveCole")
@@ -299,8 +315,10 @@ class WeaklyCompressibleSPHSystem:
         self.viscosity = viscosity
         self.density_diffusion = density_diffusion
         self.mass = np.asarray(initial_condition.mass, dtype=np.float64).copy()
-        # ``system.pressure``: filled by the library on demand (reference order)
+        # ``system.pressure`` and, with SummationDensity, ``system.cache.density``: filled by
+        # the library on demand (reference order)
         self.pressure = np.zeros(initial_condition.n_particles)
+        self.density = np.asarray(initial_condition.density, dtype=np.float64).copy()
 
     @property
     def ndims(self):
@@ -310,9 +328,14 @@ class WeaklyCompressibleSPHSystem:
     def n_particles(self):
         return self.initial_condition.n_particles
 
+    @property
+    def summation(self):
+        return isinstance(self.density_calculator, SummationDensity)
+
     def v_nvariables(self):
-        """``v_nvariables`` (wcsph/system.jl:240-242): velocity + density rows."""
-        return self.ndims + 1
+        """``v_nvariables`` (wcsph/system.jl:232-242): velocity + density rows with
+        ContinuityDensity, velocity rows alone with SummationDensity."""
+        return self.ndims + (0 if self.summation else 1)
 
     def compact_support(self):
         return self.smoothing_kernel.compact_support(self.smoothing_length)

@@ -2,8 +2,9 @@
 ``CarpenterKennedy2N54`` driver, mirroring ``src/general/semidiscretization.jl``.
 
 The ODE state keeps the reference's layout (``semidiscretization.jl:128-135``):
-``v_ode`` is ``(ndims+1) x n_fluid`` (velocity + density, ``wcsph/system.jl:240-242``) and
-``u_ode`` is ``ndims x n_fluid``; the wall block of both is empty.  Arrays are stored
+``v_ode`` is ``(ndims+1) x n_fluid`` (velocity + density, ``wcsph/system.jl:240-242``;
+``ndims x n_fluid`` with SummationDensity, ``:236-238``) and ``u_ode`` is
+``ndims x n_fluid``; the wall block of both is empty.  Arrays are stored
 column-major like Julia's, i.e. as C-contiguous arrays of shape ``(n_fluid, ndims+1)`` /
 ``(n_fluid, ndims)`` whose rows are the reference's columns.
 
@@ -135,6 +136,8 @@ class Semidiscretization:
         p.pair_search = native.PAIR_SEARCH[self.pair_search]
         p.precision = native.PRECISION_MIXED if self.precision == "mixed" else \
             native.PRECISION_FP64
+        p.density_calculator = native.DENSITY_SUMMATION if f.summation else \
+            native.DENSITY_CONTINUITY
         p.clip_fluid_eos = int(se.clip_negative_pressure)
         p.smoothing_length = f.smoothing_length
         p.reference_density = se.reference_density
@@ -246,7 +249,10 @@ class Semidiscretization:
     def initial_state(self):
         """``write_v0!``/``write_u0!`` (fluid.jl:79-97, wcsph/system.jl:440-445)."""
         ic = self.fluid.initial_condition
-        v0 = np.concatenate([ic.velocity, ic.density.reshape(1, -1)], axis=0).T
+        if self.fluid.summation:  # the velocity alone (wcsph/system.jl:236-238)
+            v0 = ic.velocity.T
+        else:
+            v0 = np.concatenate([ic.velocity, ic.density.reshape(1, -1)], axis=0).T
         u0 = ic.coordinates.T
         return self._array(v0), self._array(u0)
 
@@ -284,6 +290,21 @@ class Semidiscretization:
         else:
             self._get_pressures_device()
         return f.pressure, pw, rw
+
+    def density(self):
+        """``current_density(v, system)`` of the last kick in the caller's order
+        (``tpsph_get_density``): the summed densities with SummationDensity (also copied to
+        ``fluid.density``, the reference's ``system.cache.density``)."""
+        f = self.fluid
+        if self.backend == "hip":
+            import torch
+            d = torch.empty(f.n_particles, dtype=torch.float64, device=f"cuda:{self.device}")
+            native.check(native.load().tpsph_get_density(self.handle, d.data_ptr()), self.handle)
+            f.density[:] = d.cpu().numpy()
+        else:
+            native.check(native.load().tpsph_get_density(self.handle, f.density.ctypes.data),
+                         self.handle)
+        return f.density.copy()
 
     def _get_pressures_device(self):
         import torch
