@@ -116,6 +116,13 @@ def load(path: str | None = None):
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with trixiparticles.jl_amd/build.py "
                            "(there is no CPU fallback for the right-hand side)")
+    # torch ships its own HIP runtime under the same soname as the system's: whichever is
+    # loaded first serves the whole process.  Load torch's first (when torch is installed),
+    # so that the device buffers torch allocates and this library's calls share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     P = C.c_void_p
     dp = C.POINTER(C.c_double)
