@@ -354,7 +354,8 @@ __global__ void __launch_bounds__(kBlock)
     vol[s] = make_double2(m / r, m);
 }
 
-// Flag every cell that has a wall particle within R of some point of the cell (static).
+// Flag every cell that has a wall particle within the wall list radius of some point of the
+// cell (static).
 template <int ND>
 __global__ void k_wall_near(DevParams P, int nw, const double4 *__restrict__ wpos,
                             unsigned char *__restrict__ near)
@@ -362,7 +363,9 @@ __global__ void k_wall_near(DevParams P, int nw, const double4 *__restrict__ wpo
     int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nw) return;
     const double4 x = wpos[w];
-    const double reach = P.R + P.hs_wall;
+    // the wall list radius R + skin/2 (and margins) of the Verlet lists, which covers the
+    // radius R of the cell walk
+    const double reach = P.R + P.hs_wall + P.hs_list_wall;
     const int x0 = cell_1d(x.x - reach, P.lo[0], P.inv_cs[0], P.n[0]);
     const int x1 = cell_1d(x.x + reach, P.lo[0], P.inv_cs[0], P.n[0]);
     const int y0 = cell_1d(x.y - reach, P.lo[1], P.inv_cs[1], P.n[1]);
@@ -383,10 +386,26 @@ __global__ void k_wall_near(DevParams P, int nw, const double4 *__restrict__ wpo
 // lanes of a wavefront read one contiguous 256-byte row per slot.  A particle with more
 // candidates than the capacity stores count -1 and walks the cells in the kick instead.
 // ------------------------------------------------------------------------------------------
-template <int ND, bool WALL>
+// The union rows of the brick of a target, for the LDS indices of its list (the records the
+// brick kernel stages): row r of the union covers the cells (uy0 + r % nuy, uz0 + r / nuy)
+// and holds the cell-sorted particles gb[r] ... in LDS slots lds[r] .. lds[r + 1] - 1.
+struct BrickRows {
+    const int *gb, *lds;
+    int uy0, uz0, nuy;
+};
+
+// Verlet list of fluid particle s (sorted index) in column col: the candidates within the
+// list radius, in the pencils' order.  BRICK: also the 16-bit LDS indices of the entries in
+// the union of the target's brick, four per 8-byte group (the slots past the end of the list
+// point at the far record); ok turns false if an entry lies outside the union (the brick
+// then does not stage).
+template <int ND, bool WALL, bool BRICK = false>
 __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &F,
                                            const WallView &Wv, int s, int col, int ls, int cap,
-                                           int *__restrict__ nbr, int *__restrict__ cnt)
+                                           int *__restrict__ nbr, int *__restrict__ cnt,
+                                           const BrickRows &br = BrickRows{},
+                                           unsigned long long *__restrict__ nbr16 = nullptr,
+                                           bool *ok = nullptr)
 {
     const double xa = F.x[s], ya = F.y[s], za = (ND == 3) ? F.z[s] : 0.0;
     const int *start = WALL ? Wv.start : F.start;
@@ -397,7 +416,22 @@ __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &
     Pencils<ND> it;
     it.init(P, xa, ya, za, WALL ? P.hs_list_wall : P.hs_list_fluid);
     int n = 0, j, je;
+    int rg = 0, rl = 0, rl1 = 0;      // BRICK: the union row of the current pencil
+    bool row_ok = false, good = true;
+    unsigned long long pack = 0ull;    // BRICK: the LDS indices of the open group
+    constexpr unsigned long long kFarSlot = (unsigned long long)(kBrickCap - 1);
     while (it.next(P, start, j, je)) {
+        if constexpr (BRICK) {
+            // the pencil is the cell row (iy - 1, iz) of the fluid grid
+            const int ry = (it.iy - 1) - br.uy0, rz = ((ND == 3) ? it.iz : 0) - br.uz0;
+            const int r = ry + br.nuy * rz;
+            row_ok = ry >= 0 && ry < br.nuy && rz >= 0 && r < P.maxr;
+            if (row_ok) {
+                rg = br.gb[r];
+                rl = br.lds[r];
+                rl1 = br.lds[r + 1];
+            }
+        }
         // four candidates per step, their loads in flight together: the SoA arrays carry
         // kPad entries past their end, so the loads stay in bounds, and j + k < je masks the
         // candidates past the range (the list order is the candidates' order)
@@ -410,33 +444,82 @@ __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 if (j + k < je && d2[k] <= rl2 && (WALL || j + k != s)) {
-                    if (n < cap) nbr[(size_t)n * ls + col] = j + k;
+                    if (n < cap) {
+                        nbr[(size_t)n * ls + col] = j + k;
+                        if constexpr (BRICK) {
+                            const int e = rl + (j + k - rg);
+                            const bool in = row_ok && j + k >= rg && e < rl1 && e < kBrickCap - 1;
+                            good = good && in;
+                            pack |= (in ? (unsigned long long)e : kFarSlot) << (16 * (n & 3));
+                            if ((n & 3) == 3) {
+                                nbr16[(size_t)(n >> 2) * ls + col] = pack;
+                                pack = 0ull;
+                            }
+                        }
+                    }
                     n++;
                 }
             }
         }
     }
+    if constexpr (BRICK) {
+        if (n <= cap && (n & 3) != 0) {  // pad the last group with the far record
+            for (int k = n & 3; k < 4; k++) pack |= kFarSlot << (16 * k);
+            nbr16[(size_t)(n >> 2) * ls + col] = pack;
+        }
+        *ok = good && n <= cap;
+    }
     cnt[col] = n <= cap ? n : -1;
 }
 
-// Fluid and wall lists of every fluid particle (the wall lists only with walls), stored in
-// column t for the particle ord[t].
+// Fluid and wall lists of every fluid particle, stored in column t for the particle ord[t].
+// The wall lists only for particles in cells flagged near a wall (wall_near, whose reach
+// covers the wall list radius).  With bricks (nbr16 != nullptr) the fluid lists also get
+// their LDS indices in the union of the particle's brick (brick_union_body ran before), and
+// a brick with an entry outside its union, or with an overflowed list, is marked not to
+// stage (bntot = INT_MAX).
 template <int ND>
 __device__ __forceinline__ void
     lists_body(int bx, int gx, const int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
             int ls, int cap_f, int *__restrict__ nbr, int *__restrict__ cnt, int cap_w,
             int *__restrict__ wnbr, int *__restrict__ wcnt, int *__restrict__ wtargets,
-            int *__restrict__ nwtargets, int *__restrict__ noverflow)
+            int *__restrict__ nwtargets, int *__restrict__ noverflow,
+            const unsigned char *__restrict__ wall_near, const int *__restrict__ brow_gb,
+            const int *__restrict__ brow_lds, int *__restrict__ bntot,
+            unsigned long long *__restrict__ nbr16)
 {
     if (skip(flag)) return;
     for (int t = bx * kBlock + threadIdx.x; t < P.nf; t += gx * kBlock) {
         const int s = ord[t];
-        build_list<ND, false>(P, F, Wv, s, t, ls, cap_f, nbr, cnt);
+        const int c0 = cell_1d(F.x[s], P.lo[0], P.inv_cs[0], P.n[0]);
+        const int c1 = cell_1d(F.y[s], P.lo[1], P.inv_cs[1], P.n[1]);
+        const int c2 = (ND == 3) ? cell_1d(F.z[s], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
+        int b = -1;
+        if (nbr16) {
+            const int bxi = c0 / P.bc[0], byi = c1 / P.bc[1], bzi = c2 / P.bc[2];
+            b = bxi + P.nb[0] * (byi + P.nb[1] * bzi);
+            if (bntot[b] > kBrickCap - 1) b = -1;  // the brick's union does not stage
+        }
+        if (b >= 0) {
+            const int byi = c1 / P.bc[1], bzi = c2 / P.bc[2];
+            const BrickRows br{brow_gb + (size_t)b * P.maxr, brow_lds + (size_t)b * (P.maxr + 1),
+                               byi * P.bc[1] - P.ur[1], (ND == 3) ? bzi * P.bc[2] - P.ur[2] : 0,
+                               P.bc[1] + 2 * P.ur[1]};
+            bool ok = true;
+            build_list<ND, false, true>(P, F, Wv, s, t, ls, cap_f, nbr, cnt, br, nbr16, &ok);
+            if (!ok) bntot[b] = INT_MAX;
+        } else {
+            build_list<ND, false>(P, F, Wv, s, t, ls, cap_f, nbr, cnt);
+        }
         // an overflowed list walks the cells, which need the SoA coordinates every kick
         if (cnt[t] < 0) atomicAdd(noverflow, 1);
         if (P.nw > 0) {
-            build_list<ND, true>(P, F, Wv, s, t, ls, cap_w, wnbr, wcnt);
-            if (wtargets && wcnt[t] != 0) wtargets[atomicAdd(nwtargets, 1)] = t;
+            if (wall_near[c0 + P.n[0] * (c1 + P.n[1] * c2)]) {
+                build_list<ND, true>(P, F, Wv, s, t, ls, cap_w, wnbr, wcnt);
+                if (wtargets && wcnt[t] != 0) wtargets[atomicAdd(nwtargets, 1)] = t;
+            } else {
+                wcnt[t] = 0;
+            }
         }
     }
 }
@@ -706,84 +789,21 @@ __device__ __forceinline__ void
     }
 }
 
-// LDS indices of the fluid lists of every target (the same entries as nbr, in the same
-// order), and the marks of the bricks that cannot stage.
-template <int ND>
-__device__ __forceinline__ void
-    brick_lists_body(int bx, int gx, const int *flag, DevParams P, FluidView F, const int *__restrict__ ord,
-                  const int *__restrict__ nbr, const int *__restrict__ cnt,
-                  const int *__restrict__ wcnt, const int *__restrict__ brow_gb,
-                  const int *__restrict__ brow_lds, int *__restrict__ bntot, int ls,
-                  unsigned short *__restrict__ nbr16)
-{
-    if (skip(flag)) return;
-    const int nuy = P.bc[1] + 2 * P.ur[1];
-    for (int t = bx * kBlock + threadIdx.x; t < P.nf; t += gx * kBlock) {
-        const int s = ord[t];
-        const int c0 = cell_1d(F.x[s], P.lo[0], P.inv_cs[0], P.n[0]);
-        const int c1 = cell_1d(F.y[s], P.lo[1], P.inv_cs[1], P.n[1]);
-        const int c2 = (ND == 3) ? cell_1d(F.z[s], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
-        const int bx = c0 / P.bc[0], by = c1 / P.bc[1], bz = c2 / P.bc[2];
-        const int b = bx + P.nb[0] * (by + P.nb[1] * bz);
-        const int n = cnt[t];
-        bool ok = n >= 0;
-        const int uy0 = by * P.bc[1] - P.ur[1], uz0 = (ND == 3) ? bz * P.bc[2] - P.ur[2] : 0;
-        const int *gb = brow_gb + (size_t)b * P.maxr;
-        const int *lds = brow_lds + (size_t)b * (P.maxr + 1);
-        // one group of four slots per step (one 8-byte store); the slots past the end of the
-        // list point at the far record
-        ushort4 *l4 = reinterpret_cast<ushort4 *>(nbr16);
-        const int ng = (max(n, 0) + 3) >> 2;
-        for (int g = 0; ok && g < ng; g++) {
-            int j[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) j[k] = 4 * g + k < n ? nbr[(size_t)(4 * g + k) * ls + t] : -1;
-            unsigned short e16[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                e16[k] = (unsigned short)(kBrickCap - 1);
-                if (j[k] < 0) continue;
-                const int cy = cell_1d(F.y[j[k]], P.lo[1], P.inv_cs[1], P.n[1]);
-                const int cz = (ND == 3) ? cell_1d(F.z[j[k]], P.lo[2], P.inv_cs[2], P.n[2]) : 0;
-                const int ry = cy - uy0, rz = cz - uz0;
-                const int r = ry + nuy * rz;
-                if (ry < 0 || ry >= nuy || rz < 0 || r >= P.maxr) {
-                    ok = false;
-                    continue;
-                }
-                const int e = lds[r] + (j[k] - gb[r]);
-                if (j[k] < gb[r] || e >= lds[r + 1] || e >= kBrickCap - 1) {
-                    ok = false;
-                    continue;
-                }
-                e16[k] = (unsigned short)e;
-            }
-            if (ok) l4[(size_t)g * ls + t] = make_ushort4(e16[0], e16[1], e16[2], e16[3]);
-        }
-        if (!ok) bntot[b] = INT_MAX;
-    }
-}
-
-// The list builds of a rebuild in two launches: the fluid and wall Verlet lists with the
-// brick unions, then the brick (LDS-index) lists with the Adami lists of the walls.  Each
-// launch gives the first part its own block range and the second part the rest.
 struct ListsArgs {
     const int *ord;
     int ls, cap_f;
     int *nbr, *cnt;
     int cap_w;
     int *wnbr, *wcnt, *wtargets, *nwtargets, *noverflow;
+    const unsigned char *wall_near;
+    const int *brow_gb, *brow_lds;
+    int *bntot;
+    unsigned long long *nbr16;  // nullptr without bricks
 };
 struct UnionArgs {
     const int *start, *bstart;
     int nbricks;
     int *brow_gb, *brow_lds, *bntot, *nel, *nne;
-};
-struct BrickListsArgs {
-    const int *ord, *nbr, *cnt, *wcnt, *brow_gb, *brow_lds;
-    int *bntot;
-    int ls;
-    unsigned short *nbr16;
 };
 struct WallListsArgs {
     const double4 *wpos;
@@ -794,32 +814,31 @@ struct WallListsArgs {
     double *wvel;
 };
 
+// The rebuild's union tables of the bricks (first nblocks_union blocks) beside the Verlet
+// lists of the wall particles (the Adami pass): both read only the new cell-sorted order.
 template <int ND>
 __global__ void __launch_bounds__(kBlock)
-    k_brick_union_and_lists(const int *flag, DevParams P, FluidView F, WallView Wv, ListsArgs a,
-                            UnionArgs b, int nblocks_lists)
+    k_brick_union_and_walls(const int *flag, DevParams P, FluidView F, UnionArgs b,
+                            WallListsArgs w, int nblocks_union)
 {
-    if ((int)blockIdx.x < nblocks_lists)
-        lists_body<ND>(blockIdx.x, nblocks_lists, flag, P, F, Wv, a.ord, a.ls, a.cap_f, a.nbr,
-                       a.cnt, a.cap_w, a.wnbr, a.wcnt, a.wtargets, a.nwtargets, a.noverflow);
+    if ((int)blockIdx.x < nblocks_union)
+        brick_union_body<ND>(blockIdx.x, nblocks_union, flag, P, b.start, b.bstart, b.nbricks,
+                             b.brow_gb, b.brow_lds, b.bntot, b.nel, b.nne);
     else
-        brick_union_body<ND>(blockIdx.x - nblocks_lists, gridDim.x - nblocks_lists, flag, P,
-                             b.start, b.bstart, b.nbricks, b.brow_gb, b.brow_lds, b.bntot, b.nel,
-                             b.nne);
-}
-
-template <int ND>
-__global__ void __launch_bounds__(kBlock)
-    k_brick_lists_and_walls(const int *flag, DevParams P, FluidView F, BrickListsArgs a,
-                            WallListsArgs w, int nblocks_bricks)
-{
-    if ((int)blockIdx.x < nblocks_bricks)
-        brick_lists_body<ND>(blockIdx.x, nblocks_bricks, flag, P, F, a.ord, a.nbr, a.cnt, a.wcnt,
-                             a.brow_gb, a.brow_lds, a.bntot, a.ls, a.nbr16);
-    else
-        wall_lists_body<ND>(blockIdx.x - nblocks_bricks, gridDim.x - nblocks_bricks, flag, P, F,
+        wall_lists_body<ND>(blockIdx.x - nblocks_union, gridDim.x - nblocks_union, flag, P, F,
                             w.wpos, w.lw, w.cap, w.nbr, w.cnt, w.active, w.nactive, w.pw, w.rhow,
                             w.wpv, w.wvel);
+}
+
+// The fluid particles' Verlet lists (with their LDS indices when the bricks stage) and wall
+// lists, after the union tables.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_lists(const int *flag, DevParams P, FluidView F, WallView Wv, ListsArgs a)
+{
+    lists_body<ND>(blockIdx.x, gridDim.x, flag, P, F, Wv, a.ord, a.ls, a.cap_f, a.nbr, a.cnt,
+                   a.cap_w, a.wnbr, a.wcnt, a.wtargets, a.nwtargets, a.noverflow, a.wall_near,
+                   a.brow_gb, a.brow_lds, a.bntot, a.nbr16);
 }
 
 // Staged union: one record per particle as four 16-byte fields and one 8-byte field, each in
@@ -1657,25 +1676,22 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
             TRY(scan(h, h->flag, h->bcount, h->bstart, h->nbricks + 1));
             hipLaunchKernelGGL((k_brick_scatter), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
                 h->flag, nf, h->f_cell, h->f_rank, h->bstart, h->ord, h->bcount);
-            const int gl = gate_blocks(nf);
             const int gu = h->bricks ? std::max(1, std::min(h->nbricks, kGateBlocks)) : 0;
-            hipLaunchKernelGGL((k_brick_union_and_lists<ND>), dim3(gl + gu), dim3(kBlock), 0,
-                h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
-                ListsArgs{h->ord, h->ls, h->cap_f, h->nbr, h->cnt, h->cap_w, h->wnbr, h->wcnt,
-                          h->wtargets, h->counters + 2, h->counters + 3},
-                UnionArgs{h->f_start, h->bstart, h->nbricks, h->brow_gb, h->brow_lds, h->bntot,
-                          h->nel, h->counters},
-                gl);
-            const int gb = h->bricks ? gate_blocks(nf) : 0;
             const int gw = h->nw > 0 ? gate_blocks(h->nw) : 0;
-            if (gb + gw > 0)
-                hipLaunchKernelGGL((k_brick_lists_and_walls<ND>), dim3(gb + gw), dim3(kBlock), 0,
+            if (gu + gw > 0)
+                hipLaunchKernelGGL((k_brick_union_and_walls<ND>), dim3(gu + gw), dim3(kBlock), 0,
                     h->stream, h->flag, h->P, h->fluid_view(),
-                    BrickListsArgs{h->ord, h->nbr, h->cnt, h->wcnt, h->brow_gb, h->brow_lds,
-                                   h->bntot, h->ls, h->nbr16},
+                    UnionArgs{h->f_start, h->bstart, h->nbricks, h->brow_gb, h->brow_lds, h->bntot,
+                              h->nel, h->counters},
                     WallListsArgs{h->wpos, h->lw, h->cap_a, h->anbr, h->acnt, h->wactive,
                                   h->counters + 1, h->pw, h->rhow, h->wpv, h->wvel},
-                    gb);
+                    gu);
+            hipLaunchKernelGGL((k_lists<ND>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
+                h->flag, h->P, h->fluid_view(), h->wall_view(),
+                ListsArgs{h->ord, h->ls, h->cap_f, h->nbr, h->cnt, h->cap_w, h->wnbr, h->wcnt,
+                          h->wtargets, h->counters + 2, h->counters + 3, h->wall_near,
+                          h->brow_gb, h->brow_lds, h->bntot,
+                          h->bricks ? reinterpret_cast<unsigned long long *>(h->nbr16) : nullptr});
             TRY(launch_check(h, "k_lists"));
         }
         // the fluid kernel of this kick lowers the flag
@@ -1880,12 +1896,16 @@ int tpsph::derive_params(const tpsph_params *params, int nf, int nw, const doubl
         return fail(nullptr, TPSPH_ERR_ARGUMENT, "cell grid too large (domain / smoothing_length)");
     P.ncells = (int)ncells;
     {
-        // bricks of 3 (3D) or 8 (2D) cell spans per side: about 3.1 R (8.3 R) wide, at
-        // h = 1.2 dx about 420 (400) targets and a union of about 1700 (600) particles
+        // bricks of 3 (3D) or 8 (2D) cell spans per side, about 3.1 R (8.3 R) wide, in 3D
+        // shortened along x to 2.75 spans: at h = 1.2 dx a 3D brick holds up to 448 targets
+        // and a union of at most about 2020 particles at rest, inside the LDS records of the
+        // brick kernel (full 3-span bricks: 512 targets, and the unions of 5-10 % of the
+        // bricks exceed the records and take the slower global path)
         const int per = nd == 3 ? 3 : 8;
         int64_t nb = 1;
         for (int d = 0; d < 3; d++) {
             P.bc[d] = d < nd ? per * (d == 0 ? xdiv : div) : 1;
+            if (nd == 3 && d == 0) P.bc[d] = std::max(1, P.bc[d] * 11 / 12);
             P.nb[d] = (P.n[d] + P.bc[d] - 1) / P.bc[d];
             nb *= P.nb[d];
         }
