@@ -122,22 +122,33 @@ __host__ __device__ __forceinline__ double dpressure_drho(const DevParams &P, do
 }
 
 // Every candidate of one particle: its Verlet list (list != nullptr, n >= 0), else the cell
-// walk around (x, y, z) with cells grown by hs.  fn(j) applies the exact radius test.
-template <int ND, typename Fn>
+// walk around (x, y, z) with cells grown by hs.  fn(j) applies the exact radius test.  LANES
+// lanes share the particle: lane q takes list entries q, q + LANES, ...; the cell walk of an
+// overflowed list runs on lane 0 alone.
+template <int ND, int LANES = 1, typename Fn>
 __device__ __forceinline__ void for_each_candidate(const DevParams &P, const int *__restrict__ list,
                                                    int n, int stride, int col,
                                                    const int *__restrict__ start, double hs,
-                                                   double x, double y, double z, Fn &&fn)
+                                                   double x, double y, double z, int q, Fn &&fn)
 {
     if (list && n >= 0) {
-        for (int k = 0; k < n; k++) fn(list[(size_t)k * stride + col]);
-    } else {
+        for (int k = q; k < n; k += LANES) fn(list[(size_t)k * stride + col]);
+    } else if (q == 0) {
         Pencils<ND> it;
         it.init(P, x, y, z, hs);
         int jb, je;
         while (it.next(P, start, jb, je))
             for (int j = jb; j < je; j++) fn(j);
     }
+}
+
+// Sum of v over the LANES lanes of a group (a fixed butterfly: every lane gets the total).
+template <int LANES>
+__device__ __forceinline__ double group_sum(double v)
+{
+#pragma unroll
+    for (int o = 1; o < LANES; o <<= 1) v += __shfl_xor(v, o);
+    return v;
 }
 
 // The adjoint weights of the rates in cell-sorted order: kb[s] = {Gbar (3), rbar}.
@@ -200,7 +211,7 @@ __global__ void __launch_bounds__(kBlock)
     const double4 xw = wpos[w];
     const double p_w = pw[w], rho_w = rhow[w], V_w = wpv[w].y, m_w = xw.w;
     double pbar = 0.0, rbar = 0.0, vol = 0.0, gsum = 0.0;
-    for_each_candidate<ND>(P, alist, n, lw, w, F.start, P.hs_fluid, xw.x, xw.y, xw.z, [&](int j) {
+    for_each_candidate<ND>(P, alist, n, lw, w, F.start, P.hs_fluid, xw.x, xw.y, xw.z, 0, [&](int j) {
         const double4 a = F.rec[2 * (size_t)j];
         // the Adami pass's difference x_w - x_f
         const double p0 = xw.x - a.x, p1 = xw.y - a.y, p2 = (ND == 3) ? xw.z - a.z : 0.0;
@@ -239,11 +250,14 @@ __global__ void __launch_bounds__(kBlock)
     gw[w] += cN * gsum;
 }
 
-// Adjoint of the fluid terms, one lane per fluid particle c: its own fluid-fluid and
-// fluid-wall terms, its partners' fluid-fluid terms, its part in the Adami sums of the wall
-// particles near it, the state equation and gravity.  Adds (ACC) or writes d L / d state_c
-// into vb (v, rho) and ub (x) in the caller's order, and adds the partials along alpha,
-// delta and g into th[3 i ..].
+// Adjoint of the fluid terms of fluid particle c, four lanes per particle (lane q takes the
+// list entries q, q + 4, ...; the partial sums are added across the four lanes in a fixed
+// order): its own fluid-fluid and fluid-wall terms, its partners' fluid-fluid terms, its part
+// in the Adami sums of the wall particles near it, the state equation and gravity.  Adds
+// (ACC) or writes d L / d state_c into vb (v, rho) and ub (x) in the caller's order, and adds
+// the partials along alpha, delta and g into th[3 i ..].
+constexpr int kAdjLanes = 4;
+
 template <int ND, bool ACC>
 __global__ void __launch_bounds__(kBlock)
     k_adj_fluid(int *flag, DevParams P, AdjConsts K, FluidView F, WallView Wv,
@@ -254,8 +268,11 @@ __global__ void __launch_bounds__(kBlock)
                 double *__restrict__ ub, double *__restrict__ th)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;  // the rebuild (if any) is done
-    const int t = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (t >= P.nf) return;
+    const int q = threadIdx.x % kAdjLanes;
+    const int tt = (xcd_tile(blockIdx.x, gridDim.x) * kBlock + (int)threadIdx.x) / kAdjLanes;
+    // the loop bound is uniform per group of lanes, so the partners shuffle together
+    const bool active = tt < P.nf;
+    const int t = active ? tt : P.nf - 1;
     const int s = ord ? ord[t] : t;
     const double4 a = F.rec[2 * (size_t)s], b = F.rec[2 * (size_t)s + 1];
     const double2 vc = F.vol[s];
@@ -267,7 +284,8 @@ __global__ void __launch_bounds__(kBlock)
     double xb[3] = {0.0, 0.0, 0.0}, vbr[3] = {0.0, 0.0, 0.0};
     double rhob = 0.0, pb = 0.0, alpha = 0.0, delta = 0.0;
     const int n = nbr ? cnt[t] : -1;
-    for_each_candidate<ND>(P, nbr, n, ls, t, F.start, P.hs_fluid, a.x, a.y, a.z, [&](int j) {
+    for_each_candidate<ND, kAdjLanes>(P, nbr, active ? n : 0, ls, t, F.start, P.hs_fluid, a.x, a.y,
+                                      a.z, active ? q : kAdjLanes, [&](int j) {
         const double4 na = F.rec[2 * (size_t)j];
         double r[3] = {xc[0] - na.x, xc[1] - na.y, (ND == 3) ? xc[2] - na.z : 0.0};
         const double d2 = dist2<ND>(r[0], r[1], r[2]);
@@ -294,19 +312,20 @@ __global__ void __launch_bounds__(kBlock)
         const double rn[3] = {-r[0], -r[1], -r[2]};
         const double vn[3] = {-vab[0], -vab[1], -vab[2]};
         const double Gn[3] = {gn.x, gn.y, gn.z};
-        const PairAdj q = pair_adj<ND, true>(P, K, rn, d2, vn, vr, na.w, rho_c, nb.w, p_c, V_c,
-                                             m_c, Gn, gn.w);
+        const PairAdj qa = pair_adj<ND, true>(P, K, rn, d2, vn, vr, na.w, rho_c, nb.w, p_c, V_c,
+                                              m_c, Gn, gn.w);
 #pragma unroll
         for (int k = 0; k < ND; k++) {
-            xb[k] -= q.r[k];
-            vbr[k] += q.vr * r[k];
+            xb[k] -= qa.r[k];
+            vbr[k] += qa.vr * r[k];
         }
-        rhob += q.rho_b;
-        pb += q.p;
+        rhob += qa.rho_b;
+        pb += qa.p;
     });
     if (P.nw > 0) {
         const int nw = wnbr ? wcnt[t] : -1;
-        for_each_candidate<ND>(P, wnbr, nw, ls, t, Wv.start, P.hs_wall, a.x, a.y, a.z, [&](int w) {
+        for_each_candidate<ND, kAdjLanes>(P, wnbr, active ? nw : 0, ls, t, Wv.start, P.hs_wall,
+                                          a.x, a.y, a.z, active ? q : kAdjLanes, [&](int w) {
             const double4 xw = Wv.pos[w];
             double r[3] = {xc[0] - xw.x, xc[1] - xw.y, (ND == 3) ? xc[2] - xw.z : 0.0};
             const double d2 = dist2<ND>(r[0], r[1], r[2]);
@@ -341,6 +360,16 @@ __global__ void __launch_bounds__(kBlock)
             pb += o.p;
         });
     }
+#pragma unroll
+    for (int k = 0; k < ND; k++) {
+        xb[k] = group_sum<kAdjLanes>(xb[k]);
+        vbr[k] = group_sum<kAdjLanes>(vbr[k]);
+    }
+    rhob = group_sum<kAdjLanes>(rhob);
+    pb = group_sum<kAdjLanes>(pb);
+    alpha = group_sum<kAdjLanes>(alpha);
+    delta = group_sum<kAdjLanes>(delta);
+    if (!active || q != 0) return;
     rhob += pb * dpressure_drho(P, rho_c);
     double gpart = G[0] * K.ghat[0] + G[1] * K.ghat[1];  // gravity: dv += g ghat
     if constexpr (ND == 3) gpart += G[2] * K.ghat[2];
@@ -476,7 +505,7 @@ int adjoint_passes(tpsph_handle *h, double *vb, double *ub)
                            h->adj_kb, h->adj_coef, h->adj_gw);
         TRY(launch_check(h, "k_adj_wall"));
     }
-    hipLaunchKernelGGL((k_adj_fluid<ND, ACC>), dim3(nblocks(h->nf)), dim3(kBlock), 0, h->stream,
+    hipLaunchKernelGGL((k_adj_fluid<ND, ACC>), dim3(nblocks(kAdjLanes * (int64_t)h->nf)), dim3(kBlock), 0, h->stream,
                        h->flag, h->P, K, h->fluid_view(), h->wall_view(),
                        lists ? h->ord : nullptr, lists ? h->nbr : nullptr,
                        lists ? h->cnt : nullptr, lists ? h->wnbr : nullptr,
