@@ -47,11 +47,31 @@ struct AdjConsts {
     double ghat[3];
 };
 
-// (host and device, so that it can be checked on the CPU)
+// The geometry of one pair at squared distance d2 (shared by the pair's two terms):
+// distance d and 1/d, the kernel factor kdr (grad W = kdr r_ab), (d kdr / d distance) / d
+// (d kdr / d r_ab = dkdr r_ab) and 1 / (d2 + eps h^2) of the viscosity.  sqrt and the
+// reciprocals are the refined hardware approximations of the forward pass (within a few ulp).
+struct PairGeom {
+    double d, inv_d, kdr, dkdr, inv_q;
+};
+
+__device__ __forceinline__ PairGeom pair_geom(const DevParams &P, double d2)
+{
+    PairGeom g;
+    g.d = sqrt_pos(d2);
+    g.inv_d = rcp_pos(g.d);
+    const double w = 1.0 - 0.5 * P.h_inv * g.d;
+    g.kdr = P.kgrad_c * (w * w * w);
+    g.dkdr = P.kgrad_c * (-1.5 * P.h_inv) * (w * w) * g.inv_d;
+    g.inv_q = rcp_pos(d2 + P.eps_h2);
+    return g;
+}
+
 // d phi_ab / d(inputs) of one pair term in the equation of particle a with partner b:
-//   r_ab = x_a - x_b (r), distance^2 d2, v_ab (vab, zero for a wall partner),
-//   vr = v_ab . r_ab, the densities, pressures, V_b = m_b / rho_b and m_b of the pair,
-//   and the adjoint weights G (of dv_a) and gr (of drho_a).
+//   r_ab = x_a - x_b (r), the pair geometry pg, v_ab (vab, zero for a wall partner),
+//   vr = v_ab . r_ab, the densities with their reciprocals, the pressures, V_b = m_b / rho_b
+//   and m_b of the pair, 1 / (rho_a + rho_b), and the adjoint weights G (of dv_a) and gr (of
+//   drho_a).
 // Outputs: d phi / d r_ab, d phi / d vr (d phi / d v_a = vr r_ab, d phi / d v_b = -vr r_ab),
 // d phi / d rho_a, d phi / d rho_b (total: also through V_b), d phi / d p_a = d phi / d p_b,
 // and the derivatives along alpha and delta.  FLUID: fluid partner (viscosity and density
@@ -64,32 +84,27 @@ struct PairAdj {
 };
 
 template <int ND, bool FLUID>
-__host__ __device__ __forceinline__ PairAdj pair_adj(const DevParams &P, const AdjConsts &K,
-                                            const double *r, double d2, const double *vab,
-                                            double vr, double rho_a, double rho_b, double p_a,
-                                            double p_b, double V_b, double m_b,
-                                            const double *G, double gr)
+__device__ __forceinline__ PairAdj pair_adj(const DevParams &P, const AdjConsts &K,
+                                            const double *r, const PairGeom &pg,
+                                            const double *vab, double vr, double rho_a,
+                                            double inv_rho_a, double rho_b, double inv_rho_b,
+                                            double inv_sum, double p_a, double p_b,
+                                            double V_b, double m_b, const double *G, double gr)
 {
     PairAdj o;
-    const double d = sqrt(d2);
-    const double w = 1.0 - 0.5 * P.h_inv * d;
-    const double kdr = P.kgrad_c * (w * w * w);
-    // (d kdr / d distance) / distance: d kdr / d r_ab = dkdr r_ab
-    const double dkdr = P.kgrad_c * (-1.5 * P.h_inv) * (w * w) / d;
+    const double kdr = pg.kdr, dkdr = pg.dkdr;
     double A = G[0] * r[0] + G[1] * r[1];
     if constexpr (ND == 3) A += G[2] * r[2];
-    const double sum_rho = rho_a + rho_b;
-    double fac = -V_b * (p_a + p_b) / rho_a;
+    double fac = -V_b * (p_a + p_b) * inv_rho_a;
     double visc = 0.0, dvisc_dvr = 0.0, dvisc_dq = 0.0, dvisc_dalpha = 0.0;
     if (FLUID && P.use_visc && vr < 0.0) {
         // (m_b alpha c mu + m_b beta mu^2) / rho_mean, mu = h vr / q, q = d2 + eps h^2
-        const double q = d2 + P.eps_h2;
-        const double mu = P.h * vr / q;
-        const double two_m = 2.0 * m_b / sum_rho;
+        const double mu = P.h * vr * pg.inv_q;
+        const double two_m = 2.0 * m_b * inv_sum;
         visc = two_m * (P.alpha * P.c * mu + P.beta * (mu * mu));
         const double dmu = two_m * (P.alpha * P.c + 2.0 * P.beta * mu);
-        dvisc_dvr = dmu * P.h / q;
-        dvisc_dq = -dmu * mu / q;
+        dvisc_dvr = dmu * P.h * pg.inv_q;
+        dvisc_dq = -dmu * mu * pg.inv_q;
         dvisc_dalpha = two_m * P.c * mu;
     }
     fac += visc;
@@ -103,11 +118,11 @@ __host__ __device__ __forceinline__ PairAdj pair_adj(const DevParams &P, const A
     for (int k = 0; k < 3; k++) o.r[k] = 0.0;
 #pragma unroll
     for (int k = 0; k < ND; k++) o.r[k] = kdr * fac * G[k] + o.vr * vab[k] + radial * r[k];
-    o.p = -kA * V_b / rho_a;
-    o.rho_a = kA * (V_b * (p_a + p_b) / (rho_a * rho_a) - visc / sum_rho) +
+    o.p = -kA * V_b * inv_rho_a;
+    o.rho_a = kA * (V_b * (p_a + p_b) * (inv_rho_a * inv_rho_a) - visc * inv_sum) +
               gr * V_b * kdr * (dd2 + vr);
-    const double dphi_dVb = -kA * (p_a + p_b) / rho_a + gr * kdr * E;
-    o.rho_b = -kA * visc / sum_rho - gr * V_b * kdr * dd2 - dphi_dVb * V_b / rho_b;
+    const double dphi_dVb = -kA * (p_a + p_b) * inv_rho_a + gr * kdr * E;
+    o.rho_b = -kA * visc * inv_sum - gr * V_b * kdr * dd2 - dphi_dVb * V_b * inv_rho_b;
     o.alpha = kA * dvisc_dalpha;
     o.delta = FLUID ? gr * V_b * kdr * K.dd2_per_delta * (rho_a - rho_b) : 0.0;
     return o;
@@ -210,6 +225,7 @@ __global__ void __launch_bounds__(kBlock)
     }
     const double4 xw = wpos[w];
     const double p_w = pw[w], rho_w = rhow[w], V_w = wpv[w].y, m_w = xw.w;
+    const double inv_rho_w = rcp_pos(rho_w);
     double pbar = 0.0, rbar = 0.0, vol = 0.0, gsum = 0.0;
     for_each_candidate<ND>(P, alist, n, lw, w, F.start, P.hs_fluid, xw.x, xw.y, xw.z, 0, [&](int j) {
         const double4 a = F.rec[2 * (size_t)j];
@@ -230,8 +246,9 @@ __global__ void __launch_bounds__(kBlock)
         double vr = va[0] * r[0] + va[1] * r[1];
         if constexpr (ND == 3) vr += va[2] * r[2];
         const double G[3] = {g.x, g.y, g.z};
-        const PairAdj o = pair_adj<ND, false>(P, K, r, d2, va, vr, a.w, rho_w, b.w, p_w, V_w,
-                                              m_w, G, g.w);
+        const PairGeom pg = pair_geom(P, d2);
+        const PairAdj o = pair_adj<ND, false>(P, K, r, pg, va, vr, a.w, rcp_pos(a.w), rho_w,
+                                              inv_rho_w, 0.0, b.w, p_w, V_w, m_w, G, g.w);
         pbar += o.p;
         rbar += o.rho_b;
     });
@@ -278,7 +295,7 @@ __global__ void __launch_bounds__(kBlock)
     const double2 vc = F.vol[s];
     const double4 g = kb[s];
     const double xc[3] = {a.x, a.y, a.z}, v_c[3] = {b.x, b.y, b.z};
-    const double rho_c = a.w, p_c = b.w, V_c = vc.x, m_c = vc.y;
+    const double rho_c = a.w, p_c = b.w, V_c = vc.x, m_c = vc.y, inv_rho_c = rcp_pos(rho_c);
     const double G[3] = {g.x, g.y, g.z};
     const double gr = g.w;
     double xb[3] = {0.0, 0.0, 0.0}, vbr[3] = {0.0, 0.0, 0.0};
@@ -296,9 +313,12 @@ __global__ void __launch_bounds__(kBlock)
         double vab[3] = {v_c[0] - nb.x, v_c[1] - nb.y, (ND == 3) ? v_c[2] - nb.z : 0.0};
         double vr = vab[0] * r[0] + vab[1] * r[1];
         if constexpr (ND == 3) vr += vab[2] * r[2];
+        // the geometry and the density reciprocals of the pair serve both of its terms
+        const PairGeom pg = pair_geom(P, d2);
+        const double inv_rho_j = rcp_pos(na.w), inv_sum = rcp_pos(rho_c + na.w);
         // own term: c's rates with partner j
-        const PairAdj o = pair_adj<ND, true>(P, K, r, d2, vab, vr, rho_c, na.w, p_c, nb.w,
-                                             nv.x, nv.y, G, gr);
+        const PairAdj o = pair_adj<ND, true>(P, K, r, pg, vab, vr, rho_c, inv_rho_c, na.w,
+                                             inv_rho_j, inv_sum, p_c, nb.w, nv.x, nv.y, G, gr);
 #pragma unroll
         for (int k = 0; k < ND; k++) {
             xb[k] += o.r[k];
@@ -312,8 +332,8 @@ __global__ void __launch_bounds__(kBlock)
         const double rn[3] = {-r[0], -r[1], -r[2]};
         const double vn[3] = {-vab[0], -vab[1], -vab[2]};
         const double Gn[3] = {gn.x, gn.y, gn.z};
-        const PairAdj qa = pair_adj<ND, true>(P, K, rn, d2, vn, vr, na.w, rho_c, nb.w, p_c, V_c,
-                                              m_c, Gn, gn.w);
+        const PairAdj qa = pair_adj<ND, true>(P, K, rn, pg, vn, vr, na.w, inv_rho_j, rho_c,
+                                              inv_rho_c, inv_sum, nb.w, p_c, V_c, m_c, Gn, gn.w);
 #pragma unroll
         for (int k = 0; k < ND; k++) {
             xb[k] -= qa.r[k];
@@ -333,7 +353,7 @@ __global__ void __launch_bounds__(kBlock)
             const double2 pv = Wv.pv[w];
             const double2 cf = coef[w];
             // the Adami sums of w: N_w and V_w depend on p_c, rho_c and x_c
-            const double dist = sqrt(d2);
+            const double dist = sqrt(d2);  // d2 may be 0 here
             const double W = wendland_c2(P, dist);
             const double ww = 1.0 - 0.5 * P.h_inv * dist;
             const double kdrW = dist < P.two_h ? P.kgrad_c * (ww * ww * ww) : 0.0;
@@ -348,9 +368,10 @@ __global__ void __launch_bounds__(kBlock)
             // own fluid-wall term (v_w = 0)
             double vr = v_c[0] * r[0] + v_c[1] * r[1];
             if constexpr (ND == 3) vr += v_c[2] * r[2];
-            const double rho_w = xw.w / pv.y;
-            const PairAdj o = pair_adj<ND, false>(P, K, r, d2, v_c, vr, rho_c, rho_w, p_c, pv.x,
-                                                  pv.y, xw.w, G, gr);
+            const double inv_rho_w = pv.y / xw.w, rho_w = xw.w / pv.y;
+            const PairGeom pg = pair_geom(P, d2);
+            const PairAdj o = pair_adj<ND, false>(P, K, r, pg, v_c, vr, rho_c, inv_rho_c, rho_w,
+                                                  inv_rho_w, 0.0, p_c, pv.x, pv.y, xw.w, G, gr);
 #pragma unroll
             for (int k = 0; k < ND; k++) {
                 xb[k] += o.r[k];
