@@ -40,6 +40,7 @@ using namespace tpsph;
 namespace {
 
 constexpr int kRedBlocks = 256;  // blocks of the deterministic parameter reduction
+constexpr int kAdjLanes = 4;     // lanes per particle of the adjoint passes
 
 // Constants of the parameter derivatives: d dd2 / d delta and the unit gravity direction.
 struct AdjConsts {
@@ -202,7 +203,7 @@ __global__ void __launch_bounds__(kBlock)
     kb[iperm[i]] = make_double4(k[0], k[1], ND == 3 ? k[2] : 0.0, k[ND]);
 }
 
-// Adjoint of the Adami pass, one lane per wall particle (wall-sorted order w): the adjoints
+// Adjoint of the Adami pass, four lanes per wall particle (wall-sorted order w): the adjoints
 // of p_w and rho_w from the fluid-wall terms of the fluid near w, through the inverse state
 // equation and the clipping to the adjoint of the unclipped p_w = N_w / V_w with
 //   N_w = sum_f (p_offset + p_f + rho_f g.(x_w - x_f)) W_wf,   V_w = sum_f W_wf,
@@ -216,18 +217,21 @@ __global__ void __launch_bounds__(kBlock)
                const int *__restrict__ acnt, int lw, const double4 *__restrict__ kb,
                double2 *__restrict__ coef, double *__restrict__ gw)
 {
-    const int w = blockIdx.x * kBlock + threadIdx.x;
-    if (w >= P.nw) return;
+    // four lanes per wall particle (the loop bound is uniform per group of lanes, so the
+    // partners shuffle together); lane q takes the list entries q, q + 4, ...
+    const int q = threadIdx.x % kAdjLanes;
+    const int w = (blockIdx.x * kBlock + (int)threadIdx.x) / kAdjLanes;
+    if (w >= P.nw) return;  // whole groups
     const int n = alist ? acnt[w] : -1;
     if (n == 0) {  // no fluid within reach: p_w and rho_w are constants
-        coef[w] = make_double2(0.0, 0.0);
+        if (q == 0) coef[w] = make_double2(0.0, 0.0);
         return;
     }
     const double4 xw = wpos[w];
     const double p_w = pw[w], rho_w = rhow[w], V_w = wpv[w].y, m_w = xw.w;
     const double inv_rho_w = rcp_pos(rho_w);
     double pbar = 0.0, rbar = 0.0, vol = 0.0, gsum = 0.0;
-    for_each_candidate<ND>(P, alist, n, lw, w, F.start, P.hs_fluid, xw.x, xw.y, xw.z, 0, [&](int j) {
+    for_each_candidate<ND, kAdjLanes>(P, alist, n, lw, w, F.start, P.hs_fluid, xw.x, xw.y, xw.z, q, [&](int j) {
         const double4 a = F.rec[2 * (size_t)j];
         // the Adami pass's difference x_w - x_f
         const double p0 = xw.x - a.x, p1 = xw.y - a.y, p2 = (ND == 3) ? xw.z - a.z : 0.0;
@@ -252,6 +256,11 @@ __global__ void __launch_bounds__(kBlock)
         pbar += o.p;
         rbar += o.rho_b;
     });
+    pbar = group_sum<kAdjLanes>(pbar);
+    rbar = group_sum<kAdjLanes>(rbar);
+    vol = group_sum<kAdjLanes>(vol);
+    gsum = group_sum<kAdjLanes>(gsum);
+    if (q != 0) return;
     // inverse state equation: d rho_w / d p_w = rho_w / (gamma (p_w - p_bg + B))
     const double ptot = pbar + rbar * rho_w / (P.gamma * (p_w - P.p_bg + P.B));
     double cN = 0.0, cV = 0.0;
@@ -273,8 +282,6 @@ __global__ void __launch_bounds__(kBlock)
 // in the Adami sums of the wall particles near it, the state equation and gravity.  Adds
 // (ACC) or writes d L / d state_c into vb (v, rho) and ub (x) in the caller's order, and adds
 // the partials along alpha, delta and g into th[3 i ..].
-constexpr int kAdjLanes = 4;
-
 template <int ND, bool ACC>
 __global__ void __launch_bounds__(kBlock)
     k_adj_fluid(int *flag, DevParams P, AdjConsts K, FluidView F, WallView Wv,
@@ -520,7 +527,7 @@ int adjoint_passes(tpsph_handle *h, double *vb, double *ub)
     const AdjConsts K = adj_consts(h);
     const bool lists = h->search == TPSPH_SEARCH_LISTS;
     if (h->nw > 0) {
-        hipLaunchKernelGGL((k_adj_wall<ND>), dim3(nblocks(h->nw)), dim3(kBlock), 0, h->stream,
+        hipLaunchKernelGGL((k_adj_wall<ND>), dim3(nblocks(kAdjLanes * (int64_t)h->nw)), dim3(kBlock), 0, h->stream,
                            h->P, K, h->fluid_view(), h->wpos, h->pw, h->rhow, h->wpv,
                            lists ? h->anbr : nullptr, lists ? h->acnt : nullptr, h->lw,
                            h->adj_kb, h->adj_coef, h->adj_gw);
