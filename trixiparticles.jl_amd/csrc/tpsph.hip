@@ -106,7 +106,7 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
     P = with_sound(P);
     const int f = *flag;
     // MODE 0 runs first in every kick: it resets the work counter of the brick kernel
-    if (MODE == 0 && work && blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
+    if (MODE == 0 && work && blockIdx.x == 0 && threadIdx.x == 0) work[0] = work[1] = 0;
     if ((MODE == 0) == (f != 0)) return;
     const size_t nf = (size_t)P.nf;
     // with Verlet lists the SoA coordinates are read between rebuilds only by the cell walk
@@ -879,8 +879,9 @@ __device__ __forceinline__ FluidNbr lds_fluid(const BrickSmem<ND> &S, int e)
 // Fluid-fluid kernel over the bricks (one block of kBrickThreads per non-empty brick, taken
 // from a work counter that k_gather<0> resets).  Targets with wall candidates are completed
 // by k_fluid_walls.
-// GLOBAL_ONLY: only the bricks that do not stage, taken by blocks in turn (no work counter);
-// the mixed-precision pass (tpsph_mixed.hip) leaves them to this kernel.
+// GLOBAL_ONLY: only the bricks that do not stage, which the mixed-precision pass
+// (tpsph_mixed.hip) leaves to this kernel: the work[1] bricks it listed in bfall, taken by
+// blocks in turn.
 template <int ND, bool GLOBAL_ONLY = false>
 __global__ void __launch_bounds__(kBrickThreads)
     k_fluid_brick(int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
@@ -890,7 +891,7 @@ __global__ void __launch_bounds__(kBrickThreads)
                   const int *__restrict__ bntot, const unsigned short *__restrict__ nbr16,
                   const int *__restrict__ nbr, const int *__restrict__ cnt,
                   const int *__restrict__ wnbr, const int *__restrict__ wcnt, int ls,
-                  double *__restrict__ dv)
+                  double *__restrict__ dv, const int *__restrict__ bfall = nullptr)
 {
     P = with_sound(P);
     // cell-walk fallback lists of a global brick: as many entries per lane as the staging
@@ -901,7 +902,7 @@ __global__ void __launch_bounds__(kBrickThreads)
     const int tid = threadIdx.x;
     if (blockIdx.x == 0 && tid == 0) *flag = 0;  // the rebuild (if any) is done
     int *lst = reinterpret_cast<int *>(&S) + tid;  // fallback lists, stride kBrickThreads
-    const int ne = *nne;
+    const int ne = GLOBAL_ONLY ? work[1] : *nne;
     for (int turn = blockIdx.x;; turn += gridDim.x) {
         int item = turn;
         if (!GLOBAL_ONLY) {
@@ -910,7 +911,7 @@ __global__ void __launch_bounds__(kBrickThreads)
             item = S.item;
         }
         if (item >= ne) break;
-        const int b = nel[item];
+        const int b = GLOBAL_ONLY ? bfall[item] : nel[item];
         const int t0 = bstart[b], t1 = bstart[b + 1];
         const int ntot = bntot[b];
         const bool staged = ntot <= kBrickCap - 1;
@@ -1257,7 +1258,7 @@ __global__ void __launch_bounds__(kBlock)
         v[k] = fma(b, t, v[k]);
     }
     __syncthreads();  // the block's new u and v are complete
-    if (work && blockIdx.x == 0 && threadIdx.x == 0) *work = 0;
+    if (work && blockIdx.x == 0 && threadIdx.x == 0) work[0] = work[1] = 0;
     const int i = i0 + (int)threadIdx.x;
     bool moved = false;  // left its skin
     if (i < nf) {
@@ -1762,7 +1763,7 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
             hipLaunchKernelGGL((k_fluid_brick<ND, true>), dim3(std::max(1, std::min(h->nbricks, h->n_cu))),
                 dim3(kBrickThreads), 0, h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
                 h->ord, h->bstart, h->nel, h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
-                h->nbr16, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
+                h->nbr16, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv, h->bfall);
             if (nw > 0)
                 hipLaunchKernelGGL((k_fluid_walls<ND>), dim3(gate_blocks(4 * (int64_t)nf)), dim3(kBlock), 0, h->stream,
                     h->P, h->fluid_view(), h->wall_view(), h->ord, h->wtargets, h->counters + 2,
@@ -2433,7 +2434,10 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
             CTRY(dalloc(h, &h->brow_lds, (size_t)h->nbricks * (P.maxr + 1)));
             CTRY(dalloc(h, &h->bntot, h->nbricks));
             CTRY(dalloc(h, &h->nel, h->nbricks));
-            CTRY(dalloc(h, &h->bwork, 1));
+            // [0] the brick kernels' work counter, [1] the bricks the mixed-precision pass
+            // leaves to the FP64 pass (listed in bfall); both cleared by the kick's gather
+            CTRY(dalloc(h, &h->bwork, 2));
+            if (h->mixed) CTRY(dalloc(h, &h->bfall, (size_t)h->nbricks));
             CTRY(dalloc(h, &h->nbr16, (size_t)h->cap_f * h->ls));
         }
         CTRY(dalloc(h, &h->bcount, (size_t)h->nbricks + 1));
@@ -2928,7 +2932,7 @@ void tpsph_destroy(tpsph_handle *h)
                     h->rhow, h->st_v, h->st_u, h->st_dv, h->st_du, h->sol_dv, h->sol_tv,
                     h->sol_tu, h->integ, h->nrebuild, h->adj_kb, h->adj_coef, h->adj_th,
                     h->adj_gw, h->adj_red, h->adj_vb, h->adj_ub, h->adj_tvb, h->adj_tub,
-                    h->adj_ck, h->adj_st, h->sum_rho};
+                    h->adj_ck, h->adj_st, h->sum_rho, h->bfall};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);

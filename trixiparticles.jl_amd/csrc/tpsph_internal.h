@@ -759,6 +759,7 @@ struct tpsph_handle {
     int nbricks = 0;
     bool bricks = false;
     int *brow_gb = nullptr, *brow_lds = nullptr, *bntot = nullptr, *nel = nullptr, *bwork = nullptr;
+    int *bfall = nullptr;  // mixed precision: the bricks left to the FP64 pass
     unsigned short *nbr16 = nullptr;
     double *dyn = nullptr;       // adaptive sound speed: {c, B, av_c, dd2}
     double *wvel = nullptr;      // no-slip wall velocities (3 per wall particle)

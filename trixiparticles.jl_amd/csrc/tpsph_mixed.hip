@@ -123,8 +123,9 @@ __device__ __forceinline__ void pair32_term(const Pair32 &q, const float4 &a, co
 }
 
 // Fluid-fluid pass over the staged bricks, one block of kBrickThreads per brick taken from
-// the work counter (two blocks per CU).  Bricks that do not stage are left to
-// k_fluid_brick<ND, true> (FP64, global memory), launched after this kernel.
+// the work counter work[0] (two blocks per CU).  Bricks that do not stage are listed in bfall
+// (work[1] of them) for k_fluid_brick<ND, true> (FP64, global memory), launched after this
+// kernel.
 template <int ND>
 __global__ void __launch_bounds__(kBrickThreads, 8)
     k_fluid_brick32(int *flag, Pair32 q, FluidView F, const int *__restrict__ ord,
@@ -133,7 +134,7 @@ __global__ void __launch_bounds__(kBrickThreads, 8)
                     const int *__restrict__ brow_gb, const int *__restrict__ brow_lds,
                     const int *__restrict__ bntot, const unsigned short *__restrict__ nbr16,
                     const int *__restrict__ cnt, int ls, const int *__restrict__ wcnt,
-                    double *__restrict__ dv)
+                    double *__restrict__ dv, int *__restrict__ bfall)
 {
     if (q.dyn) {  // the sound speed of this kick
         q.two_av_c = (float)(2.0 * q.dyn[2]);
@@ -150,7 +151,9 @@ __global__ void __launch_bounds__(kBrickThreads, 8)
         if (item >= ne) break;
         const int b = nel[item];
         const int ntot = bntot[b];
-        if (ntot <= kBrickCap - 1) {  // uniform per block
+        if (ntot > kBrickCap - 1) {  // left to the FP64 pass (uniform per block)
+            if (tid == 0) bfall[atomicAdd(&work[1], 1)] = b;
+        } else {
             const int t0 = bstart[b], t1 = bstart[b + 1];
             for (int r = tid; r <= q.maxr; r += kBrickThreads) {
                 if (r < q.maxr) S.row_gb[r] = brow_gb[(size_t)b * q.maxr + r];
@@ -249,11 +252,11 @@ int tpsph::launch_fluid_brick_mixed(tpsph_handle *h, double *dv)
         hipLaunchKernelGGL((k_fluid_brick32<2>), dim3(grid), dim3(kBrickThreads), 0, h->stream,
                            h->flag, pair32(h->P), h->fluid_view(), h->ord, h->bstart, h->nel,
                            h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot, h->nbr16,
-                           h->cnt, h->ls, h->wcnt, dv);
+                           h->cnt, h->ls, h->wcnt, dv, h->bfall);
     else
         hipLaunchKernelGGL((k_fluid_brick32<3>), dim3(grid), dim3(kBrickThreads), 0, h->stream,
                            h->flag, pair32(h->P), h->fluid_view(), h->ord, h->bstart, h->nel,
                            h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot, h->nbr16,
-                           h->cnt, h->ls, h->wcnt, dv);
+                           h->cnt, h->ls, h->wcnt, dv, h->bfall);
     return launch_check(h, "k_fluid_brick32");
 }
