@@ -321,7 +321,9 @@ def section_ensemble(tp, lib, native, args, n_cases=256, steps=300, alone_steps=
                                       "own_handle": t_one_handle * 1e3,
                                       "steps_per_case": alone_steps},
         "speedup_vs_one_at_a_time": best_alone / t_batch_short,
-        "launch_shapes": {k: tuning[k] for k in ("fluid_block", "wall_block")} if tuning else None,
+        "launch_shapes": {k: tuning.get(k) for k in ("cell_divisions", "x_cell_divisions",
+                                                     "fluid_block", "wall_block")}
+        if tuning else None,
     }
 
 

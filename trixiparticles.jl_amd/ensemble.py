@@ -99,13 +99,23 @@ class EnsembleCase:
 
 def dam_break_ensemble(n_cases=256, *, seed=1, particle_spacing=0.01, width=(0.3, 1.2),
                        aspect=(0.5, 2.0), alpha=(0.01, 0.05), delta=(0.05, 0.15),
-                       tank_width_factor=2.5, tank_height_factor=5.0 / 3.0):
+                       tank_width_factor=2.5, tank_height_factor=5.0 / 3.0,
+                       cell_divisions=None, x_cell_divisions=None):
     """Uneven 2D dam breaks drawn from one seed (SplitMix64, four draws per case): water
     column width w ~ U(width), height aspect * w with aspect ~ U(aspect), alpha ~ U(alpha),
     delta ~ U(delta); the open tank is tank_width_factor * w wide and tank_height_factor *
     height high (the 3.2 x 1.0 tank of the 1.2 x 0.6 template keeps 5/3 of the height); the
     sound speed is 20 sqrt(g height) as in the template.  Sizes are rounded to the particle
-    spacing by RectangularTank.  Case k jitters its lattice with seed + 1 + k."""
+    spacing by RectangularTank.  Case k jitters its lattice with seed + 1 + k.
+    ``cell_divisions`` / ``x_cell_divisions``: cells per compact support of every case's grid
+    (the cell tiles of the packed cell walk); ``None`` takes the autotuner's committed choice
+    (library defaults without one).  The tiling changes the order of each case's sums, so a
+    case compares bit for bit with itself alone on the same tiling."""
+    tuned = load_tuning() or {}
+    if cell_divisions is None:
+        cell_divisions = tuned.get("cell_divisions", 0)
+    if x_cell_divisions is None:
+        x_cell_divisions = tuned.get("x_cell_divisions", 0)
     u = splitmix64_uniform(seed, 4 * n_cases).reshape(n_cases, 4)
     cases = []
     for k in range(n_cases):
@@ -118,7 +128,8 @@ def dam_break_ensemble(n_cases=256, *, seed=1, particle_spacing=0.01, width=(0.3
                        tank_size=(tank_width_factor * w, tank_height_factor * H),
                        alpha=al, delta=de, water_height=H, seed=seed + 1 + k)
         semi = Semidiscretization(db.fluid_system, db.boundary_system, pair_search="cells",
-                                  skin=0.0)
+                                  skin=0.0, cell_divisions=cell_divisions,
+                                  x_cell_divisions=x_cell_divisions)
         cases.append(EnsembleCase(db, semi, w, a, al, de))
     return cases
 
