@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define TPSPH_ABI_VERSION 5
+#define TPSPH_ABI_VERSION 6
 
 enum tpsph_status {
     TPSPH_OK = 0,
@@ -371,6 +371,23 @@ int tpsph_integrals(tpsph_handle *h, const double *v_ode, const double *u_ode, d
  * tpsph_create: a kick rebuilds when a particle has moved by more than half the skin since
  * the last rebuild.  Synchronises the handle's stream. */
 int tpsph_rebuild_count(tpsph_handle *h, int64_t *count);
+
+/* SortingCallback's affect! for the fluid system (sort_particles! and sort_system!,
+ * src/callbacks/sorting.jl:118-158): puts the caller's fluid particles into the cell-sorted
+ * order of their current positions, in place.  Row j of the sorted v_ode / u_ode is row
+ * perm[j] (0-based) of the state passed in; the handle's fluid masses are permuted with the
+ * state (the reference leaves the masses unsorted, sorting.jl:131, which mixes the
+ * non-uniform masses of a hydrostatic set-up up: the caller permutes its own copy of the
+ * masses with perm too).  The order is the library's cell order (cells of the fluid grid,
+ * x fastest, by caller index within a cell), so after the sort the library's internal
+ * permutation is the identity and the gathers and scatters of every kick are contiguous.
+ * Each particle's rates equal, bit for bit, those of the unsorted state on a handle that
+ * rebuilds its cell list at the same positions (the sums run in the same order).  perm may
+ * be NULL;
+ * otherwise it holds n_fluid int64 values in the memory kind of the handle.  The pressures
+ * of the sorted state are ready for tpsph_get_pressures afterwards.  A trajectory recorded
+ * by tpsph_adjoint_forward is dropped (its states are in the old order). */
+int tpsph_sort_particles(tpsph_handle *h, double *v_ode, double *u_ode, int64_t *perm);
 
 /* Wait for all work queued on the handle's stream. */
 int tpsph_synchronize(tpsph_handle *h);

@@ -22,7 +22,7 @@ def test_library_exports_every_declared_symbol(tp):
     for s in syms:
         assert hasattr(lib, s), f"{s} declared in include/tpsph.h but not exported"
     assert set(tp.native.EXPORTED_SYMBOLS) == set(syms)
-    assert lib.tpsph_abi_version() == 5
+    assert lib.tpsph_abi_version() == 6
 
 
 def test_library_exports_every_symbol_of_every_header(tp):

@@ -15,6 +15,7 @@ from .schemes import (AdamiPressureExtrapolation, ArtificialViscosityMonaghan,
 from .semidiscretization import (CK2N54_A, CK2N54_B, CK2N54_C, CarpenterKennedy2N54,
                                  DynamicalODEProblem, Semidiscretization, calculate_dt, drift,
                                  kick, kick_pairs, semidiscretize, solve, solve_tstops)
+from .callbacks import SortingCallback
 from .postprocess import compute_pressure, interpolate_line, interpolate_points, max_x_coord
 from .setups import InitialCondition, RectangularShape, RectangularTank, splitmix64_uniform
 from . import native
@@ -29,6 +30,6 @@ __all__ = [
     "solve_tstops", "InitialCondition",
     "RectangularShape", "RectangularTank", "splitmix64_uniform", "DamBreak", "dam_break",
     "marrone_dam_break_2d", "compute_pressure", "interpolate_line", "interpolate_points",
-    "max_x_coord",
+    "max_x_coord", "SortingCallback",
     "native",
 ]

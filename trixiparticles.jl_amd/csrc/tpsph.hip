@@ -1613,6 +1613,25 @@ __global__ void k_unpermute(int n, const int *__restrict__ perm, const T *__rest
     out[perm[s]] = sorted[s];
 }
 
+// tpsph_sort_particles: row j of the sorted state is row perm[j] of the caller's state (v
+// with `vrows` doubles per particle, u with ND, the mass with one); po[j] = perm[j]
+template <int ND>
+__global__ void k_permute_state(int n, int vrows, const int *__restrict__ perm,
+                                const double *__restrict__ v, const double *__restrict__ u,
+                                const double *__restrict__ m, double *__restrict__ vo,
+                                double *__restrict__ uo, double *__restrict__ mo,
+                                int64_t *__restrict__ po)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int i = perm[j];
+    for (int d = 0; d < vrows; d++) vo[(size_t)j * vrows + d] = v[(size_t)i * vrows + d];
+#pragma unroll
+    for (int d = 0; d < ND; d++) uo[(size_t)j * ND + d] = u[(size_t)i * ND + d];
+    mo[j] = m[i];
+    if (po) po[j] = i;
+}
+
 // fluid pressure (p = .w of the second double4 of each record, half = 1) or density (.w of
 // the first, half = 0) of the cell-sorted records
 __global__ void k_record_pressure(int n, const double4 *__restrict__ rec, double *__restrict__ p,
@@ -2881,6 +2900,70 @@ int tpsph_rebuild_count(tpsph_handle *h, int64_t *count)
     }
     *count = n;
     return TPSPH_OK;
+}
+
+int tpsph_sort_particles(tpsph_handle *h, double *v_ode, double *u_ode, int64_t *perm)
+{
+    if (!h) return fail(nullptr, TPSPH_ERR_ARGUMENT, "handle is NULL");
+    const int nf = h->nf, nd = h->nd;
+    if (nf == 0) return TPSPH_OK;
+    if (!v_ode || !u_ode) return fail(h, TPSPH_ERR_ARGUMENT, "v_ode and u_ode must not be NULL");
+    TRY(ensure_staging(h));
+    const size_t vl = v_len(h), ul = u_len(h);
+    const bool host = is_host(h);
+    // device copies of the state before the sort (st_v, st_u) and of the masses (st_dv);
+    // host mode writes the sorted state to temporaries and copies it back
+    double *vo = v_ode, *uo = u_ode, *tmp = nullptr;
+    int64_t *po = perm;
+    const hipMemcpyKind in_kind = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    HIP_TRY(h, hipMemcpyAsync(h->st_v, v_ode, vl * sizeof(double), in_kind, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(h->st_u, u_ode, ul * sizeof(double), in_kind, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(h->st_dv, h->f_mass, (size_t)nf * sizeof(double),
+                              hipMemcpyDeviceToDevice, h->stream));
+    if (host) {
+        TRY(dalloc(h, &tmp, vl + ul + (size_t)nf));
+        vo = tmp;
+        uo = tmp + vl;
+        po = perm ? reinterpret_cast<int64_t *>(tmp + vl + ul) : nullptr;
+    }
+    // the cell-sorted order of the current positions (the flag stays raised until the
+    // preparation below has rebuilt everything from the sorted state)
+    hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->flag, 1);
+    int rc = nd == 2 ? build_cells<2>(h, h->flag, nf, h->st_u, h->f_cell, h->f_rank, h->f_count,
+                                      h->f_start, h->f_perm)
+                     : build_cells<3>(h, h->flag, nf, h->st_u, h->f_cell, h->f_rank, h->f_count,
+                                      h->f_start, h->f_perm);
+    if (rc == TPSPH_OK) {
+        if (nd == 2)
+            hipLaunchKernelGGL((k_permute_state<2>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, nf,
+                               h->P.vrows, h->f_perm, h->st_v, h->st_u, h->st_dv, vo, uo, h->f_mass, po);
+        else
+            hipLaunchKernelGGL((k_permute_state<3>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, nf,
+                               h->P.vrows, h->f_perm, h->st_v, h->st_u, h->st_dv, vo, uo, h->f_mass, po);
+        rc = launch_check(h, "k_permute_state");
+    }
+    // rebuild from the sorted state: the cell-sorted order is the one just computed (within
+    // a cell the order is by caller index, which now increases along it), so the internal
+    // permutation is the identity; pressures and wall values of the sorted state are ready
+    // for tpsph_get_pressures, and the next kick need not rebuild
+    if (rc == TPSPH_OK) rc = prepare_dispatch(h, vo, uo);
+    if (rc == TPSPH_OK) {
+        hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->flag, 0);
+        rc = launch_check(h, "k_set_i32");
+    }
+    if (rc == TPSPH_OK && host) {
+        if (hipMemcpyAsync(v_ode, vo, vl * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipMemcpyAsync(u_ode, uo, ul * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            (perm && hipMemcpyAsync(perm, po, (size_t)nf * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                    h->stream) != hipSuccess) ||
+            hipStreamSynchronize(h->stream) != hipSuccess)
+            rc = fail(h, TPSPH_ERR_HIP, "copy of the sorted state to the host failed");
+    }
+    if (tmp) (void)hipFree(tmp);
+    // a recorded trajectory (reverse mode) belongs to the old order
+    h->adj_ready = false;
+    h->have_kick = rc == TPSPH_OK;
+    return rc;
 }
 
 int tpsph_synchronize(tpsph_handle *h)

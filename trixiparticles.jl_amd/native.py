@@ -49,6 +49,7 @@ EXPORTED_SYMBOLS = (
     "tpsph_enable_timing", "tpsph_kernel_time", "tpsph_last_error", "tpsph_destroy",
     "tpsph_kick_noalloc", "tpsph_integrals", "tpsph_rebuild_count",
     "tpsph_kick_vjp", "tpsph_adjoint_forward", "tpsph_adjoint_reverse", "tpsph_axpby",
+    "tpsph_sort_particles",
 )
 
 
@@ -148,6 +149,7 @@ def load(path: str | None = None):
                                           C.c_int64, P, P]
     lib.tpsph_adjoint_reverse.argtypes = [P, P, P, P, P, P]
     lib.tpsph_rebuild_count.argtypes = [P, C.POINTER(C.c_int64)]
+    lib.tpsph_sort_particles.argtypes = [P, P, P, P]
     lib.tpsph_synchronize.argtypes = [P]
     lib.tpsph_enable_timing.argtypes = [P, C.c_int32]
     lib.tpsph_kernel_time.argtypes = [P, C.c_int32, C.POINTER(C.c_int64), dp]
@@ -161,7 +163,7 @@ def load(path: str | None = None):
                  "tpsph_synchronize", "tpsph_enable_timing", "tpsph_kernel_time",
                  "tpsph_kick_noalloc", "tpsph_integrals", "tpsph_rebuild_count",
                  "tpsph_kick_vjp", "tpsph_adjoint_forward", "tpsph_adjoint_reverse",
-                 "tpsph_axpby"):
+                 "tpsph_axpby", "tpsph_sort_particles"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

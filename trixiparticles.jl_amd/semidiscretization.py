@@ -322,6 +322,45 @@ class Semidiscretization:
             w.boundary_model.pressure[:] = pw.cpu().numpy()
             w.boundary_model.density[:] = rw.cpu().numpy()
 
+    # -- SortingCallback
+    def sort_particles(self, v_ode, u_ode):
+        """``sort_particles!`` + ``sort_system!`` for the fluid system
+        (callbacks/sorting.jl:118-158) through ``tpsph_sort_particles``: ``v_ode`` and
+        ``u_ode`` are put into the cell-sorted order of the current positions in place, and
+        the host copies the reference permutes (``pressure``, ``density``) are permuted with
+        them, as is the fluid ``mass`` (which the reference leaves in the old order,
+        sorting.jl:131).  ``particle_ids`` follows the particles: entry j is the index in the
+        initial condition of the particle now at row j.  Returns the permutation (numpy
+        int64): row j of the sorted state was row ``perm[j]``."""
+        f = self.fluid
+        nf = f.n_particles
+        lib = native.load()
+        if self.backend == "hip":
+            import torch
+            perm_d = torch.empty(nf, dtype=torch.int64, device=f"cuda:{self.device}")
+            native.check(lib.tpsph_sort_particles(self.handle, _ptr(v_ode), _ptr(u_ode),
+                                                  perm_d.data_ptr()), self.handle)
+            perm = perm_d.cpu().numpy()
+        else:
+            perm = np.empty(nf, dtype=np.int64)
+            native.check(lib.tpsph_sort_particles(self.handle, _ptr(v_ode), _ptr(u_ode),
+                                                  perm.ctypes.data), self.handle)
+        f.mass[:] = f.mass[perm]
+        f.pressure[:] = f.pressure[perm]
+        f.density[:] = f.density[perm]
+        self.particle_ids = self.particle_ids[perm]
+        return perm
+
+    @property
+    def particle_ids(self):
+        if getattr(self, "_particle_ids", None) is None:
+            self._particle_ids = np.arange(self.fluid.n_particles, dtype=np.int64)
+        return self._particle_ids
+
+    @particle_ids.setter
+    def particle_ids(self, ids):
+        self._particle_ids = ids
+
     def neighbors(self, u_ode, pair=native.PAIR_FLUID_FLUID):
         """Neighbour CSR ``(offsets, indices)`` of the current positions (host arrays)."""
         lib = native.load()
@@ -424,30 +463,75 @@ class CarpenterKennedy2N54:
     williamson_condition: bool = False
 
 
+def _split_callbacks(callback):
+    """Plain step functions ``f(v, u, t, step)`` and discrete callbacks (objects with
+    ``initialize`` / ``condition`` / ``affect``, such as ``SortingCallback``), in order."""
+    if callback is None:
+        return [], []
+    items = list(callback) if isinstance(callback, (list, tuple)) else [callback]
+    discrete = [c for c in items if hasattr(c, "condition") and hasattr(c, "affect")]
+    fns = [c for c in items if not (hasattr(c, "condition") and hasattr(c, "affect"))]
+    return fns, discrete
+
+
 def solve(ode: DynamicalODEProblem, alg=None, *, dt, nsteps=None, callback=None,
           stage_update="fused"):
     """Fixed-step ``solve(ode, CarpenterKennedy2N54(); dt, adaptive=false)``.
 
-    Without ``callback`` the steps run entirely on the device in one library call
-    (``tpsph_solve_ck2n54``).  With ``callback(v, u, t, step)`` every step is driven from
-    here through ``kick``/``drift`` and the stage update, the sequence OrdinaryDiffEq runs:
-    ``stage_update="fused"`` updates each register and state in one ``tpsph_rk_stage``;
-    ``"broadcasts"`` issues the single broadcasts of LowStorageRK2N's ``perform_step!``
-    (``tmp = dt k``, ``u = u + B tmp``, ``tmp = A tmp``, ``tmp = tmp + dt k``) as
-    ``tpsph_axpby`` calls, as a device-array type does under OrdinaryDiffEq.  Returns the
-    final ``(v, u)`` (copies of the initial state, advanced)."""
+    ``callback``: ``None``, a step function ``f(v, u, t, step)``, a discrete callback such
+    as ``SortingCallback``, or a list of these (the reference's ``CallbackSet``), run after
+    every step in list order.
+
+    Without step functions the steps run on the device (``tpsph_solve_ck2n54``), in one
+    library call per stretch of steps between two steps at which a discrete callback fires.
+    With step functions every step is driven from here through ``kick``/``drift`` and the
+    stage update, the sequence OrdinaryDiffEq runs: ``stage_update="fused"`` updates each
+    register and state in one ``tpsph_rk_stage``; ``"broadcasts"`` issues the single
+    broadcasts of LowStorageRK2N's ``perform_step!`` (``tmp = dt k``, ``u = u + B tmp``,
+    ``tmp = A tmp``, ``tmp = tmp + dt k``) as ``tpsph_axpby`` calls, as a device-array type
+    does under OrdinaryDiffEq.  Returns the final ``(v, u)`` (copies of the initial state,
+    advanced)."""
+    from .callbacks import Integrator
     if stage_update not in ("fused", "broadcasts"):
         raise ValueError("stage_update must be 'fused' or 'broadcasts'")
     semi = ode.p["semi"]
     t0, t1 = ode.tspan
     if nsteps is None:
         nsteps = int(round((t1 - t0) / dt))
+    nsteps = int(nsteps)
     v = ode.v0.clone() if hasattr(ode.v0, "clone") else ode.v0.copy()
     u = ode.u0.clone() if hasattr(ode.u0, "clone") else ode.u0.copy()
     lib = native.load()
-    if callback is None:
-        native.check(lib.tpsph_solve_ck2n54(semi.handle, _ptr(v), _ptr(u), float(t0),
-                                            float(dt), int(nsteps)), semi.handle)
+    fns, discrete = _split_callbacks(callback)
+    integ = Integrator(ode.p, (v, u), float(t0))
+    for cb in discrete:
+        cb.initialize(integ)
+
+    def after_step(n):
+        """Integrator state after accepted step n; the discrete callbacks that fire."""
+        integ.t = t0 + n * dt
+        integ.stats.naccept = n
+        integ.isfinished = n == nsteps
+        for cb in discrete:
+            if cb.condition(None, integ.t, integ):
+                cb.affect(integ)
+
+    if not fns:
+        step = 0
+        while step < nsteps:
+            # the next step at which a callback fires (the conditions read t and naccept only)
+            n = step + 1
+            while n < nsteps:
+                probe = Integrator(ode.p, (v, u), t0 + n * dt)
+                probe.stats.naccept = n
+                if any(cb.condition(None, probe.t, probe) for cb in discrete):
+                    break
+                n += 1
+            native.check(lib.tpsph_solve_ck2n54(semi.handle, _ptr(v), _ptr(u),
+                                                float(t0 + step * dt), float(dt), n - step),
+                         semi.handle)
+            step = n
+            after_step(step)
         return v, u
     dv, du = semi._zeros(*v.shape), semi._zeros(*u.shape)
     tv, tu = semi._zeros(*v.shape), semi._zeros(*u.shape)
@@ -457,7 +541,7 @@ def solve(ode: DynamicalODEProblem, alg=None, *, dt, nsteps=None, callback=None,
         native.check(lib.tpsph_axpby(semi.handle, int(np.prod(y.shape)), _ptr(y), float(a),
                                      _ptr(x), float(b)), semi.handle)
 
-    for step in range(int(nsteps)):
+    for step in range(nsteps):
         for i in range(5):
             ts = t + CK2N54_C[i] * dt
             if stage_update == "broadcasts" and i > 0:
@@ -475,7 +559,9 @@ def solve(ode: DynamicalODEProblem, alg=None, *, dt, nsteps=None, callback=None,
                                                     int(np.prod(y.shape)), int(i == 0)),
                                  semi.handle)
         t = t0 + (step + 1) * dt
-        callback(v, u, t, step + 1)
+        for f in fns:
+            f(v, u, t, step + 1)
+        after_step(step + 1)
     return v, u
 
 
