@@ -1669,10 +1669,20 @@ int build_cells(tpsph_handle *h, const int *flag, int n, const double *x, int *c
 
 // Fluid cell list (rebuilt only when flagged) + cell-sorted gather of (v, u).
 template <int ND>
-int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathered)
+int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathered,
+                 bool keep_order = false)
 {
     const int nf = h->nf;
     if (nf == 0) return TPSPH_OK;
+    if (keep_order) {
+        // the order and the lists stand: only the records of (v, u)
+        hipLaunchKernelGGL((k_set_i32), dim3(1), dim3(1), 0, h->stream, h->flag, 0);
+        hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
+            h->flag, h->bwork, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
+            h->snap, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+            h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr, nullptr);
+        return launch_check(h, "k_gather<0>");
+    }
     if (!gathered) {  // else the stage kernel wrote the records of (v, u)
         TimedScope t(h, TPSPH_TIME_GATHER);
         hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
@@ -1724,7 +1734,8 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
 // records of (v, u) with the fluid pressures, the rebuild of the cell list and the lists (if
 // a particle left its skin), and the Adami pass of the walls.
 template <int ND>
-int prepare_impl(tpsph_handle *h, const double *v, const double *u, bool gathered)
+int prepare_impl(tpsph_handle *h, const double *v, const double *u, bool gathered,
+                 bool keep_order = false)
 {
     const int nf = h->nf, nw = h->nw;
     if (h->dyn && nf > 0) {
@@ -1738,7 +1749,7 @@ int prepare_impl(tpsph_handle *h, const double *v, const double *u, bool gathere
         TRY(launch_check(h, "k_vmax/k_sound_speed"));
         gathered = false;
     }
-    TRY(update_fluid<ND>(h, v, u, gathered));
+    TRY(update_fluid<ND>(h, v, u, gathered, keep_order));
     if (h->P.summation && nf > 0) {
         // compute_density! before the state equation (wcsph/system.jl:291-301, 323-326)
         TimedScope t(h, TPSPH_TIME_GATHER);
@@ -2004,9 +2015,10 @@ int tpsph::kick_dispatch(tpsph_handle *h, double *dv, const double *v, const dou
     return h->nd == 2 ? kick_impl<2>(h, dv, v, u, gathered) : kick_impl<3>(h, dv, v, u, gathered);
 }
 
-int tpsph::prepare_dispatch(tpsph_handle *h, const double *v, const double *u)
+int tpsph::prepare_dispatch(tpsph_handle *h, const double *v, const double *u, bool keep_order)
 {
-    return h->nd == 2 ? prepare_impl<2>(h, v, u, false) : prepare_impl<3>(h, v, u, false);
+    return h->nd == 2 ? prepare_impl<2>(h, v, u, false, keep_order)
+                      : prepare_impl<3>(h, v, u, false, keep_order);
 }
 
 const double tpsph::kCkA[5] = {0.0, -567301805773.0 / 1357537059087.0,
@@ -2059,6 +2071,9 @@ int tpsph::run_ck2n54(tpsph_handle *h, double *v, double *u, double dt, int64_t 
                                           hipMemcpyDeviceToDevice, h->stream));
                 HIP_TRY(h, hipMemcpyAsync(store_u + k * u_len(h), u, u_len(h) * sizeof(double),
                                           hipMemcpyDeviceToDevice, h->stream));
+                // the stored states are what the caller keeps: the last kick and stage would
+                // only advance (v, u) past them
+                if (s == nsteps - 1 && i == 4) break;
             }
             TRY(kick_dispatch(h, h->sol_dv, v, u, gathered));
             TimedScope ts(h, TPSPH_TIME_STAGE);

@@ -595,12 +595,22 @@ int reverse_impl(tpsph_handle *h, const double *vT_bar, const double *uT_bar, do
         const double *ck = h->adj_ck + (size_t)k * (nv + nu);
         HIP_TRY(h, hipMemcpyAsync(wv, ck, nv * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
         HIP_TRY(h, hipMemcpyAsync(wu, ck + nv, nu * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        int64_t n0 = 0, n1 = 0;
+        TRY(tpsph_rebuild_count(h, &n0));
         TRY(run_ck2n54(h, wv, wu, h->adj_dt, steps, st_v, st_u));
+        TRY(tpsph_rebuild_count(h, &n1));
+        int moved = 1;
+        HIP_TRY(h, hipMemcpyAsync(&moved, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        // no rebuild in the recomputation and the last stored state (which no kick saw) within
+        // its skin: every stored state is within the skin of the current lists, so the reverse
+        // stages keep the order
+        const bool keep_order = n1 == n0 && moved == 0;
         for (int64_t s = steps - 1; s >= 0; s--) {
             for (int i = 4; i >= 0; i--) {
                 const size_t e = (size_t)(5 * s + i);
                 const double *yv = st_v + e * nv, *yu = st_u + e * nu;
-                TRY(prepare_dispatch(h, yv, yu));
+                TRY(prepare_dispatch(h, yv, yu, keep_order));
                 hipLaunchKernelGGL((k_adj_weights<ND, true>), dim3(nblocks(h->nf)), dim3(kBlock), 0,
                                    h->stream, h->nf, h->iperm, nullptr, h->adj_vb, h->adj_ub,
                                    h->adj_tvb, h->adj_tub, kCkA[i], kCkB[i], h->adj_dt, h->adj_kb);

@@ -909,13 +909,16 @@ int kick_dispatch(tpsph_handle *h, double *dv, const double *v, const double *u,
 
 // A kick without its fluid pass: the records of (v, u) with the fluid pressures, the rebuild
 // of the cell order and the lists (when a particle left its skin), the Adami pass.
-int prepare_dispatch(tpsph_handle *h, const double *v, const double *u);
+// keep_order: the caller knows that no particle of (v, u) has left its skin since the last
+// rebuild, so the rebuild launches are left out and the flag is lowered first.
+int prepare_dispatch(tpsph_handle *h, const double *v, const double *u, bool keep_order = false);
 
 // CarpenterKennedy2N54 (williamson_condition = false) coefficients A (A[0] unused) and B
 extern const double kCkA[5], kCkB[5];
 
 // nsteps CK2N54 steps of the device state (v, u) in place.  With store_v / store_u the state
-// every stage's kick sees is copied there first, stage k = 5 step + i at offset k v_len / u_len.
+// every stage's kick sees is copied there first, stage k = 5 step + i at offset k v_len / u_len,
+// and the run stops after storing the last one ((v, u) is then left at that stage state).
 int run_ck2n54(tpsph_handle *h, double *v, double *u, double dt, int64_t nsteps,
                double *store_v, double *store_u);
 

@@ -8,8 +8,9 @@
 //     instead of the domain's extent;
 //   - densities are staged as rho - rho0 (FP64 difference, rounded once), so the
 //     Molteni-Colagrossi term rho_a - rho_b keeps its relative precision;
-//   - the density rate is summed with Kahan's compensated summation, and the two lanes of a
-//     target combine their (sum, compensation) pairs in FP64;
+//   - the density rate is summed with Kahan's compensated summation (one compensated add per
+//     group of four list slots), and the two lanes of a target combine their (sum,
+//     compensation) pairs in FP64;
 //   - FP64 stays where it is: the state, the state equation (pressures come from the FP64
 //     density of the stage kernel), the wall sums (k_fluid_walls), the Adami pass, the
 //     stage update and the written dv / drho.
@@ -89,21 +90,24 @@ struct Kahan {
 };
 
 // One candidate in FP32 (pair_fluid_masked's terms).  Out-of-range candidates (and the far
-// record) get kdr = 0: their terms are exact zeros.
+// record) get kdr = 0: their terms are exact zeros.  The range test selects no branch: d2 is
+// clamped into [az2, R2] for the kernel factor, which the mask then zeroes.  The density-rate
+// term is added to drho_part, the plain FP32 sum of one list group (at most four terms) that
+// the caller adds to the compensated total.
 template <int ND>
 __device__ __forceinline__ void pair32_term(const Pair32 &q, const float4 &a, const float4 &b,
                                             float V_b, const float *xa, const float *va,
                                             float drho_a, float rho_a, float p_a,
-                                            float inv_rho_a, float *acc, Kahan &drho)
+                                            float inv_rho_a, float *acc, float &drho_part)
 {
     const float p0 = xa[0] - a.x, p1 = xa[1] - a.y, p2 = (ND == 3) ? xa[2] - a.z : 0.0f;
     float d2 = p0 * p0 + p1 * p1;
     if constexpr (ND == 3) d2 = d2 + p2 * p2;
     const bool in = (d2 <= q.R2) & (d2 >= q.az2);
-    const float d2s = in ? d2 : q.R2;
+    const float d2s = __builtin_fminf(__builtin_fmaxf(d2, q.az2), q.R2);
     const float r = d2s * __builtin_amdgcn_rsqf(d2s);
     const float w = 1.0f - q.h_inv_half * r;
-    const float kdr = in ? q.kgrad_c * (w * w * w) : 0.0f;
+    const float kdr = q.kgrad_c * (w * w * w) * (in ? 1.0f : 0.0f);
     const float v0 = va[0] - b.x, v1 = va[1] - b.y, v2 = (ND == 3) ? va[2] - b.z : 0.0f;
     float vr = v0 * p0 + v1 * p1;
     if constexpr (ND == 3) vr = vr + v2 * p2;
@@ -119,7 +123,7 @@ __device__ __forceinline__ void pair32_term(const Pair32 &q, const float4 &a, co
     acc[0] += fac * p0;
     acc[1] += fac * p1;
     if constexpr (ND == 3) acc[2] += fac * p2;
-    drho.add(V_b * kdr * (q.dd2 * (drho_a - a.w) + rho_a * vr));
+    drho_part += V_b * kdr * (q.dd2 * (drho_a - a.w) + rho_a * vr);
 }
 
 // Fluid-fluid pass over the staged bricks, one block of kBrickThreads per brick taken from
@@ -215,12 +219,14 @@ __global__ void __launch_bounds__(kBrickThreads, 8)
                     const ushort4 nxt4 = l4[(size_t)min(g + 2, ng - 1) * ls + col];
                     // one candidate at a time: eight waves per SIMD hide the LDS latency
                     const unsigned short e4[4] = {cur4.x, cur4.y, cur4.z, cur4.w};
+                    float part = 0.0f;
 #pragma unroll 1
                     for (int k = 0; k < 4; k++) {
                         const int e = e4[k];
                         pair32_term<ND>(q, S.pa[e], S.pb[e], S.vol[e], xa, va, drho_a, rho_a, p_a,
-                                        inv_rho_a, acc, drho);
+                                        inv_rho_a, acc, part);
                     }
+                    drho.add(part);
                     cur4 = nxt4;
                 }
                 // the partner's sums in FP64, the compensation folded in
