@@ -212,8 +212,13 @@ __global__ void __launch_bounds__(kBlock)
         double pressure = 0.0, volume = 0.0, vs[3] = {0.0, 0.0, 0.0};
         const int n = alist ? acnt[w] : -1;
         if (n >= 0) {
-            for (int k = q; k < n; k += LANES)
-                adami_term<ND>(P, F, xw, alist[(size_t)k * lw + w], pressure, volume, vs);
+            // the next list entry is loaded before the current one is evaluated
+            int j = q < n ? alist[(size_t)q * lw + w] : 0;
+            for (int k = q; k < n; k += LANES) {
+                const int jn = alist[(size_t)min(k + LANES, n - 1) * lw + w];
+                adami_term<ND>(P, F, xw, j, pressure, volume, vs);
+                j = jn;
+            }
         } else if (q == 0) {
             Pencils<ND> it;
             it.init(P, xw.x, xw.y, xw.z, P.hs_fluid);
@@ -1064,8 +1069,11 @@ __global__ void __launch_bounds__(kBlock)
         const int nw = active ? wcnt[t] : 0;
         double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
         const int m = max(nw, 0);
+        // the next list entry is loaded before the current one is evaluated
+        int jw = q < m ? wnbr[(size_t)q * ls + t] : 0;
         for (int k = q; k < m; k += kLanes) {
-            const WallNbr cur = load_wall(Wv, wnbr[(size_t)k * ls + t]);
+            const WallNbr cur = load_wall(Wv, jw);
+            jw = wnbr[(size_t)min(k + kLanes, m - 1) * ls + t];
             const double p0 = xa[0] - cur.a.x, p1 = xa[1] - cur.a.y;
             const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
             const double d2 = dist2<ND>(p0, p1, p2);

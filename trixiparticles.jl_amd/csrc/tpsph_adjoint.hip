@@ -148,7 +148,14 @@ __device__ __forceinline__ void for_each_candidate(const DevParams &P, const int
                                                    double x, double y, double z, int q, Fn &&fn)
 {
     if (list && n >= 0) {
-        for (int k = q; k < n; k += LANES) fn(list[(size_t)k * stride + col]);
+        // the next entry is loaded before the current one is evaluated
+        int j = q < n ? list[(size_t)q * stride + col] : 0;
+        for (int k = q; k < n; k += LANES) {
+            const int kn = min(k + LANES, n - 1);
+            const int jn = list[(size_t)kn * stride + col];
+            fn(j);
+            j = jn;
+        }
     } else if (q == 0) {
         Pencils<ND> it;
         it.init(P, x, y, z, hs);
@@ -310,13 +317,14 @@ __global__ void __launch_bounds__(kBlock)
     const int n = nbr ? cnt[t] : -1;
     for_each_candidate<ND, kAdjLanes>(P, nbr, active ? n : 0, ls, t, F.start, P.hs_fluid, a.x, a.y,
                                       a.z, active ? q : kAdjLanes, [&](int j) {
+        // every load of the candidate at once: one round trip instead of two
         const double4 na = F.rec[2 * (size_t)j];
-        double r[3] = {xc[0] - na.x, xc[1] - na.y, (ND == 3) ? xc[2] - na.z : 0.0};
-        const double d2 = dist2<ND>(r[0], r[1], r[2]);
-        if (!(d2 <= P.R2 && d2 >= P.az2)) return;
         const double4 nb = F.rec[2 * (size_t)j + 1];
         const double2 nv = F.vol[j];
         const double4 gn = kb[j];
+        double r[3] = {xc[0] - na.x, xc[1] - na.y, (ND == 3) ? xc[2] - na.z : 0.0};
+        const double d2 = dist2<ND>(r[0], r[1], r[2]);
+        if (!(d2 <= P.R2 && d2 >= P.az2)) return;
         double vab[3] = {v_c[0] - nb.x, v_c[1] - nb.y, (ND == 3) ? v_c[2] - nb.z : 0.0};
         double vr = vab[0] * r[0] + vab[1] * r[1];
         if constexpr (ND == 3) vr += vab[2] * r[2];
