@@ -63,8 +63,8 @@ def profiled_fp64_flops(kernel):
     committed summary (wavefront instructions x 64 lanes; an FMA counts 2), or None."""
     import re
     for path, text in profile_text(kernel):
-        m = re.search(r"- `%s<3>` \(\d+ dispatches, per dispatch\): ([^\n]*SQ_INSTS_VALU_FMA_F64[^\n]*)"
-                      % re.escape(kernel), text)
+        m = re.search(r"- `%s<3[^`]*>` \(\d+ dispatches, per dispatch[^)]*\): "
+                      r"([^\n]*SQ_INSTS_VALU_FMA_F64[^\n]*)" % re.escape(kernel), text)
         if not m:
             continue
         vals = dict(re.findall(r"(SQ_INSTS_VALU_\w+_F64) = ([\d.e+]+)", m.group(1)))
