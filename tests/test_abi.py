@@ -47,6 +47,12 @@ def test_reverse_mode_entry_points_reject_a_null_handle(tp):
     assert "NULL" in lib.tpsph_last_error(None).decode()
 
 
+def test_sort_entry_point_rejects_a_null_handle(tp):
+    lib = tp.native.load()
+    assert lib.tpsph_sort_particles(None, None, None, None) == tp.native.TPSPH_ERR_ARGUMENT
+    assert "NULL" in lib.tpsph_last_error(None).decode()
+
+
 def test_params_struct_layout_matches_header(tp):
     # 10 int32 + 9 doubles + acceleration[3] + offset + domain_min[3] + domain_max[3] + skin
     # + adaptive flag and its padding + 3 adaptive doubles
