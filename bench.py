@@ -17,11 +17,13 @@ Beside it, in the same line:
                  graph-captured steps vs the same cases one at a time
   adjoint        the 250k-particle 3D case: forward run vs the reverse sweep (checkpoints
                  every 10 steps, recomputation), time ratio
+  sorted_order   the headline case after one SortingCallback sort (tpsph_sort_particles):
+                 the caller's order becomes the library's cell order
   cpu_baseline   the FP64 C oracle with OpenMP on the host cores: median of 5 right-hand
                  sides after 2 warm-ups and one full CK2N54 step of the headline case
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
-  python bench.py --only headline|moving|precision|ensemble|adjoint   (one section)
+  python bench.py --only headline|moving|precision|ensemble|adjoint|sorted   (one section)
   python bench.py --rebuild-kicks N    # cost of a rebuild (separate JSON line, no bench)
 """
 import argparse
@@ -35,7 +37,7 @@ import time
 REPO = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
-SECTIONS = ("headline", "moving", "precision", "ensemble", "adjoint")
+SECTIONS = ("headline", "moving", "precision", "ensemble", "adjoint", "sorted")
 
 
 def profile_text(kernel):
@@ -223,6 +225,31 @@ def section_moving(tp, lib, native, args, spin_steps=1500, steps=100):
         "steps": steps,
         "rebuilds_per_step": (r1 - r0) / steps,
         "max_speed": vmax,
+        "kernel_ms_per_step": {k: ms / steps for k, (n, ms) in times.items()},
+    }
+
+
+def section_sorted(tp, lib, native, args, warmup=5, steps=200):
+    """The headline case with its fluid rows put into the library's cell order once (the
+    sort of SortingCallback, tpsph_sort_particles) before the timed steps."""
+    import torch
+    db = tp.dam_break(3, args.dx)
+    semi = tp.Semidiscretization(db.fluid_system, db.boundary_system)
+    ode = tp.semidiscretize(semi, (0.0, args.dt * (warmup + steps)))
+    v, u = ode.v0.clone(), ode.u0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    semi.sort_particles(v, u)
+    torch.cuda.synchronize()
+    sort_s = time.perf_counter() - t0
+    elapsed, times = run_solve(tp, lib, native, semi, v, u, args.dt, warmup, steps)
+    semi.close()
+    return {
+        "value": db.n_fluid * steps / elapsed,
+        "unit": "particle-steps/s",
+        "ms_per_step": elapsed * 1e3 / steps,
+        "steps": steps,
+        "sort_ms": sort_s * 1e3,
         "kernel_ms_per_step": {k: ms / steps for k, (n, ms) in times.items()},
     }
 
@@ -420,7 +447,8 @@ def main():
         return
     if args.only and args.only != "headline":
         fn = {"moving": section_moving, "precision": section_precision,
-              "ensemble": section_ensemble, "adjoint": section_adjoint}[args.only]
+              "ensemble": section_ensemble, "adjoint": section_adjoint,
+              "sorted": section_sorted}[args.only]
         print(json.dumps({args.only: fn(tp, lib, native, args)}), flush=True)
         return
 
@@ -432,7 +460,8 @@ def main():
     extra = {}
     if not args.only:
         for name, fn in (("moving_flow", section_moving), ("precision", section_precision),
-                         ("ensemble", section_ensemble), ("adjoint", section_adjoint)):
+                         ("ensemble", section_ensemble), ("adjoint", section_adjoint),
+                         ("sorted_order", section_sorted)):
             progress(name)
             try:
                 extra[name] = fn(tp, lib, native, args)
