@@ -251,3 +251,55 @@ def test_sorted_run_follows_the_oracle(tp, orc):
         assert rel(back_u[:, col] - u0[:, col], ur[:, col] - u0[:, col]) <= 1e-10, \
             f"u displacement column {col}"
     semi.close()
+
+
+@pytest.mark.gpu
+@needs_gpu
+def test_sort_keeps_mixed_precision_rates(tp):
+    """The mixed-precision pass sums in the cell order too: its rates move with the rows."""
+    _, semi_a = shuffled_case(tp, 3, 0.025, seed=5)
+    _, semi_b = shuffled_case(tp, 3, 0.025, seed=5)
+    for semi in (semi_a, semi_b):
+        semi.precision = "mixed"
+    va, ua = semi_a.initial_state()
+    dva = kick_np(tp, semi_a, va, ua)
+    vb, ub = semi_b.initial_state()
+    perm = semi_b.sort_particles(vb, ub)
+    dvb = kick_np(tp, semi_b, vb, ub)
+    assert np.array_equal(dvb, dva[perm])
+    semi_a.close()
+    semi_b.close()
+
+
+@pytest.mark.gpu
+@needs_gpu
+def test_sorting_with_step_functions_follows_the_oracle(tp, orc):
+    """SortingCallback beside a step function: the host-driven steps (kick!, drift! and the
+    stage update per stage) with sorts every 3 steps, mapped back through particle_ids,
+    against the oracle's unsorted trajectory."""
+    db, semi = shuffled_case(tp, 2, 0.02, seed=11)
+    ode = tp.semidiscretize(semi, (0.0, 10e-5))
+    v0, u0 = to_np(ode.v0).copy(), to_np(ode.u0).copy()
+    m0 = semi.fluid.mass.copy()
+    P = orc.params_from_semi(semi)
+    vr, ur = orc.solve_ck2n54(P, v0, u0, m0, db.boundary_system.coordinates.T,
+                              db.boundary_system.boundary_model.hydrodynamic_mass, 1e-5, 10)
+    seen = []
+    cb = tp.SortingCallback(interval=3)
+    v, u = tp.solve(ode, dt=1e-5, nsteps=10,
+                    callback=[lambda v, u, t, s: seen.append(s), cb])
+    assert seen == list(range(1, 11)) and cb.sorts == 4  # initial + after steps 3, 6, 9
+    ids = semi.particle_ids
+    back_v = np.empty_like(v0)
+    back_u = np.empty_like(u0)
+    back_v[ids] = to_np(v)
+    back_u[ids] = to_np(u)
+    for col in range(vr.shape[1]):
+        err = np.abs(back_v[:, col] - vr[:, col]).max() / np.abs(vr[:, col]).max()
+        assert err <= 1e-10, (col, err)
+    du = back_u - u0
+    dur = ur - u0
+    for col in range(ur.shape[1]):
+        err = np.abs(du[:, col] - dur[:, col]).max() / np.abs(dur[:, col]).max()
+        assert err <= 1e-10, (col, err)
+    semi.close()
