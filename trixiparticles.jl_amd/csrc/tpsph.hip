@@ -959,11 +959,16 @@ __global__ void __launch_bounds__(kBrickThreads)
             }
             __syncthreads();
         }
-        // Two lanes per target: lanes 2i and 2i+1 take alternate groups of four list slots
-        // (alternate slots of the global and wall lists); lane 2i adds its partner's sums.
-        const int half = tid & 1;
-        for (int base = t0; base < t1; base += kBrickThreads / 2) {
-            const int t = base + (tid >> 1);
+        // Four lanes per target: lane q of target i takes every fourth group of four list
+        // slots, starting at group q (every fourth slot of the global lists); the four
+        // partial sums are added across the lanes (a fixed butterfly) and lane 0 writes.
+        // With four lanes a brick's targets (about 385) take two passes of 256 whose waves
+        // spread evenly over the four SIMDs; two lanes left a quarter of the waves idle and
+        // one SIMD with the most work (8% slower).
+        constexpr int kL = 4;
+        const int half = tid & (kL - 1);
+        for (int base = t0; base < t1; base += kBrickThreads / kL) {
+            const int t = base + (tid / kL);
             const bool active = t < t1;
             const int col = active ? t : t0;
             const int s = ord[col];
@@ -994,8 +999,8 @@ __global__ void __launch_bounds__(kBrickThreads)
                 // a load and a register: the select made the compiler keep cur4 in scratch
                 // and issue a flat load, whose wait also held up every LDS read
                 ushort4 cur4 = l4[(size_t)min(half, max(ng - 1, 0)) * ls + col];
-                for (int g = half; g < ng; g += 2) {
-                    const ushort4 nxt4 = l4[(size_t)min(g + 2, ng - 1) * ls + col];
+                for (int g = half; g < ng; g += kL) {
+                    const ushort4 nxt4 = l4[(size_t)min(g + kL, ng - 1) * ls + col];
                     // the candidates are independent chains without branches: two at a time
                     // interleave within the register budget of four waves per SIMD
 #pragma unroll 1
@@ -1009,7 +1014,7 @@ __global__ void __launch_bounds__(kBrickThreads)
                     cur4 = nxt4;
                 }
             } else {
-                for (int k = half; k < m; k += 2) {
+                for (int k = half; k < m; k += kL) {
                     const FluidNbr nb = load_fluid(F, nbr[(size_t)k * ls + col]);
                     const double p0 = xa[0] - nb.a.x, p1 = xa[1] - nb.a.y;
                     const double p2 = (ND == 3) ? xa[2] - nb.a.z : 0.0;
@@ -1023,8 +1028,11 @@ __global__ void __launch_bounds__(kBrickThreads)
                         va, rho_a, p_a, inv_rho_a, acc, drho);
             }
 #pragma unroll
-            for (int d = 0; d < 3; d++) acc[d] += __shfl_xor(acc[d], 1);
-            drho += __shfl_xor(drho, 1);
+            for (int o = 1; o < kL; o <<= 1) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) acc[d] += __shfl_xor(acc[d], o);
+                drho += __shfl_xor(drho, o);
+            }
             if (active && half == 0) {
                 // with wall candidates k_fluid_walls adds the wall sum and then gravity
                 const int i = F.perm[s];
