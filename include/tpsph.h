@@ -82,7 +82,8 @@ enum tpsph_precision {
     TPSPH_PRECISION_FP64 = 0,  /* FP64 throughout (the reference's Float64)                */
     TPSPH_PRECISION_MIXED = 1  /* FP32 pair arithmetic on brick-relative coordinates with  */
                                /* compensated density-rate sums; FP64 positions, state,    */
-                               /* state equation, stage update and reductions              */
+                               /* state equation, stage update and reductions.  The fluid  */
+                               /* viscosity must have beta = 0 (TPSPH_ERR_ARGUMENT else)   */
 };
 
 /* Density calculator of the fluid (tpsph_params.density_calculator). */

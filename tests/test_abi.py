@@ -117,6 +117,13 @@ def test_create_rejects_invalid_adaptive_state_equation(tp, fields, msg):
     assert msg in err
 
 
+def test_create_rejects_mixed_precision_with_a_quadratic_viscosity(tp):
+    """The FP32 pair terms carry ArtificialViscosityMonaghan's alpha term only."""
+    rc, err = _create(tp, precision=1, use_viscosity=1, alpha=0.02, beta=0.5)
+    assert rc == tp.native.TPSPH_ERR_ARGUMENT
+    assert "beta = 0" in err
+
+
 @pytest.mark.parametrize("fields", [{"pair_search": 1}, {"precision": 1}])
 def test_create_rejects_summation_density_off_the_cell_walk(tp, fields):
     """SummationDensity runs the cell-walk pair search in FP64."""

@@ -108,6 +108,25 @@ def test_kick_matches_oracle_for_every_pair_search(tp, orc, nd, dx, search):
     check_columns(gpu_kick(tp, semi, v, u), dv_ref)
 
 
+@pytest.mark.parametrize("viscosity", ["none", "alpha_beta"])
+@pytest.mark.parametrize("nd,dx", [(2, 0.01), (3, 0.025)])
+def test_brick_kernel_viscosity_forms(tp, orc, nd, dx, viscosity):
+    """The brick kernel's pair loop is compiled once per viscosity form of the fluid
+    (none, ArtificialViscosityMonaghan with beta = 0, with beta != 0); the forms the other
+    tests do not reach match the oracle too, with approaching and receding pairs."""
+    db = tp.dam_break(nd, dx)
+    db.fluid_system.viscosity = (None if viscosity == "none" else
+                                 tp.ArtificialViscosityMonaghan(alpha=0.5, beta=0.3))
+    semi = tp.Semidiscretization(db.fluid_system, db.boundary_system,
+                                 parallelization_backend="hip", pair_search="lists")
+    v0, u0 = semi.initial_state()
+    v, u = v0.cpu().numpy().copy(), u0.cpu().numpy()
+    rng = np.random.default_rng(13)
+    v[:, :nd] = rng.normal(0.0, 0.5, size=(v.shape[0], nd))
+    dv_ref, *_ = oracle_kick(orc, semi, db, v, u)
+    check_columns(gpu_kick(tp, semi, v, u), dv_ref)
+
+
 def test_list_overflow_falls_back_to_the_cell_walk(tp, orc):
     """List capacities come from the mean particle volume m / rho0; with 40x the masses
     they are far too small, every list overflows and each particle walks the cells."""
@@ -184,6 +203,46 @@ def test_callback_driver_matches_native_solver(tp, stage_update):
     assert seen == [1, 2, 3, 4, 5]
     assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
     assert np.array_equal(u1.cpu().numpy(), u2.cpu().numpy())
+
+
+@pytest.mark.parametrize("nd,dx,wall_viscosity", [(2, 0.02, None), (3, 0.04, None),
+                                                  (3, 0.04, "adami")])
+def test_native_solver_in_motion_matches_callback_driver(tp, nd, dx, wall_viscosity):
+    """The device solver's stage kernel writes the records of the new state and raises the
+    next kick's rebuild flag.  From a state in motion that rebuilds the lists every few
+    steps it equals, bit for bit, the callback driver's kick!/drift! with the stage update of
+    tpsph_rk_stage, and a kick from the caller after the solve sees the current records."""
+    import ctypes as C
+    import torch
+    kw = {} if wall_viscosity is None else {"wall_viscosity": tp.ViscosityAdami(nu=1e-3)}
+    db, semi = setup_case(tp, nd, dx, **kw)
+    nsteps, dt = 12, 1e-4
+    ode = tp.semidiscretize(semi, (0.0, nsteps * dt))
+    rng = np.random.default_rng(5)
+    v0 = ode.v0.clone()
+    v0[:, :nd] = torch.from_numpy(rng.normal(0.0, 1.0, size=(v0.shape[0], nd))).to(v0)
+    ode.v0 = v0
+    lib = tp.native.load()
+    count = C.c_int64()
+    lib.tpsph_rebuild_count(semi.handle, C.byref(count))
+    r0 = count.value
+    v1, u1 = tp.solve(ode, dt=dt, nsteps=nsteps)
+    semi.synchronize()
+    lib.tpsph_rebuild_count(semi.handle, C.byref(count))
+    assert count.value - r0 >= 3, "the state should leave its skin several times"
+    v2, u2 = tp.solve(ode, dt=dt, nsteps=nsteps, callback=lambda v, u, t, s: None)
+    assert np.array_equal(v1.cpu().numpy(), v2.cpu().numpy())
+    assert np.array_equal(u1.cpu().numpy(), u2.cpu().numpy())
+    # a kick from the caller after the solve gives the rates a fresh handle gives for the
+    # solver's final state (in its own cell order: within 1e-12)
+    dv = torch.full_like(v1, float("nan"))
+    tp.kick(dv, v1, u1, {"semi": semi}, 0.0)
+    _, semi2 = setup_case(tp, nd, dx, **kw)
+    dv2 = torch.full_like(v1, float("nan"))
+    tp.kick(dv2, v1, u1, {"semi": semi2}, 0.0)
+    semi.synchronize()
+    semi2.synchronize()
+    check_columns(dv.cpu().numpy(), dv2.cpu().numpy())
 
 
 @pytest.mark.parametrize("nd", [2, 3])

@@ -886,8 +886,8 @@ __device__ __forceinline__ FluidNbr lds_fluid(const BrickSmem<ND> &S, int e)
 // by k_fluid_walls.
 // GLOBAL_ONLY: only the bricks that do not stage, which the mixed-precision pass
 // (tpsph_mixed.hip) leaves to this kernel: the work[1] bricks it listed in bfall, taken by
-// blocks in turn.
-template <int ND, bool GLOBAL_ONLY = false>
+// blocks in turn.  VISC: visc_mode(P), the viscosity form of the staged pair loop.
+template <int ND, bool GLOBAL_ONLY = false, int VISC = kViscAlpha>
 __global__ void __launch_bounds__(kBrickThreads)
     k_fluid_brick(int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
                   const int *__restrict__ bstart, const int *__restrict__ nel,
@@ -1008,8 +1008,8 @@ __global__ void __launch_bounds__(kBrickThreads)
                         // slots past the end of the list read the far record: exact zeros
                         const FluidNbr na = lds_fluid<ND>(S, h2 ? cur4.z : cur4.x);
                         const FluidNbr nb = lds_fluid<ND>(S, h2 ? cur4.w : cur4.y);
-                        pair_fluid_masked<ND>(P, na, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
-                        pair_fluid_masked<ND>(P, nb, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                        pair_fluid_masked<ND, VISC>(P, na, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                        pair_fluid_masked<ND, VISC>(P, nb, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
                     }
                     cur4 = nxt4;
                 }
@@ -1815,10 +1815,13 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
                     h->P, h->fluid_view(), h->wall_view(), h->ord, h->wtargets, h->counters + 2,
                     h->wnbr, h->wcnt, h->ls, dv);
         } else if (h->search == TPSPH_SEARCH_LISTS && h->bricks) {
-            hipLaunchKernelGGL((k_fluid_brick<ND>), dim3(std::max(1, std::min(h->nbricks, h->n_cu))),
+            const auto brick = visc_mode(h->P) == kViscNone ? k_fluid_brick<ND, false, kViscNone>
+                             : visc_mode(h->P) == kViscAlpha ? k_fluid_brick<ND, false, kViscAlpha>
+                                                             : k_fluid_brick<ND, false, kViscAlphaBeta>;
+            hipLaunchKernelGGL(brick, dim3(std::max(1, std::min(h->nbricks, h->n_cu))),
                 dim3(kBrickThreads), 0, h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
                 h->ord, h->bstart, h->nel, h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
-                h->nbr16, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv);
+                h->nbr16, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv, nullptr);
             if (nw > 0)
                 hipLaunchKernelGGL((k_fluid_walls<ND>), dim3(gate_blocks(4 * (int64_t)nf)), dim3(kBlock), 0, h->stream,
                     h->P, h->fluid_view(), h->wall_view(), h->ord, h->wtargets, h->counters + 2,
@@ -2020,6 +2023,9 @@ int tpsph::validate(const tpsph_params *p, int64_t nf, int64_t nw, const double 
         (p->pair_search == TPSPH_SEARCH_LISTS || p->precision != TPSPH_PRECISION_FP64))
         return bad("SummationDensity runs the cell-walk pair search (TPSPH_SEARCH_CELLS or "
                    "AUTO) in FP64");
+    if (p->precision == TPSPH_PRECISION_MIXED && p->use_viscosity && p->beta != 0.0)
+        return bad("precision TPSPH_PRECISION_MIXED evaluates ArtificialViscosityMonaghan with "
+                   "beta = 0 only");
     if (nf > 0 && !fm) return bad("fluid_mass is NULL");
     if (nw > 0 && (!wx || !wm)) return bad("wall_coordinates / wall_mass is NULL");
     return TPSPH_OK;

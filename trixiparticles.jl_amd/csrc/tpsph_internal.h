@@ -463,11 +463,24 @@ __device__ __forceinline__ void pair_wall(const DevParams &P, const WallNbr &n,
     drho = fma(n.pv.y * kdr, rho_a * vr, drho);
 }
 
+// The viscosity form of the fluid-fluid pairs as a compile-time constant of the branchless
+// pair loops: none, ArtificialViscosityMonaghan with beta = 0, or with beta != 0.
+constexpr int kViscNone = 0, kViscAlpha = 1, kViscAlphaBeta = 2;
+inline int visc_mode(const DevParams &P)
+{
+    return P.use_visc ? (P.beta != 0.0 ? kViscAlphaBeta : kViscAlpha) : kViscNone;
+}
+
 // pair_fluid without branches, for candidates evaluated in groups: an out-of-range (or
-// invalid) candidate gets kdr = 0, so its terms are exact zeros, and the viscosity term is
-// selected by the sign of v_ab.r_ab.  For an in-range candidate every operation is the one
-// of pair_fluid, so the sums agree bit for bit.
-template <int ND>
+// invalid) candidate gets kdr = 0 by a select, so its terms are exact zeros whatever the
+// square root of its distance gives (NaN for d2 = 0), and the viscosity acts through
+// min(v_ab.r_ab, 0), which is v_ab.r_ab for approaching pairs and zero otherwise (a zero
+// term added to fac leaves it unchanged).  VISC is visc_mode(P).  For an in-range candidate
+// every operation is the one of pair_fluid, so the sums agree bit for bit (up to the sign
+// of a zero).  Every intermediate of an out-of-range candidate stays finite or is selected
+// away: the far record (coordinates 1e100, zero volume) gives d2 about 1e200, whose
+// reciprocals are normal numbers.
+template <int ND, int VISC>
 __device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const FluidNbr &n,
                                                   bool valid, const double *xa,
                                                   const double *va, double rho_a, double p_a,
@@ -476,8 +489,7 @@ __device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const Flui
     const double p0 = xa[0] - n.a.x, p1 = xa[1] - n.a.y, p2 = (ND == 3) ? xa[2] - n.a.z : 0.0;
     const double d2 = dist2<ND>(p0, p1, p2);
     const bool in = valid & (d2 <= P.R2) & (d2 >= P.az2);
-    const double d2s = in ? d2 : P.R2;  // a normal argument for the square root
-    const double r = sqrt_pos(d2s);
+    const double r = sqrt_pos(d2);
     const double w = fma(-0.5 * P.h_inv, r, 1.0);
     const double kdr = in ? P.kgrad_c * (w * w * w) : 0.0;
     const double v0 = va[0] - n.b.x, v1 = va[1] - n.b.y, v2 = (ND == 3) ? va[2] - n.b.z : 0.0;
@@ -485,15 +497,15 @@ __device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const Flui
     if constexpr (ND == 3) vr += v2 * p2;
     const double rho_b = n.a.w;
     double fac = -n.v.x * inv_rho_a * (p_a + n.b.w);
-    if (P.use_visc) {
-        const double q = d2s + P.eps_h2;
+    if constexpr (VISC != kViscNone) {
+        const double q = d2 + P.eps_h2;
         const double sum_rho = rho_a + rho_b;
-        double fv = fma(2.0 * P.av_c * n.v.y * vr, rcp_pos(q * sum_rho), fac);
-        if (P.beta != 0.0) {
-            const double mu = P.h * vr * rcp_pos(q);
-            fv += 2.0 * n.v.y * P.beta * (mu * mu) * rcp_pos(sum_rho);
+        const double vn = fmin(vr, 0.0);
+        fac = fma(2.0 * P.av_c * n.v.y * vn, rcp_pos(q * sum_rho), fac);
+        if constexpr (VISC == kViscAlphaBeta) {
+            const double mu = P.h * vn * rcp_pos(q);
+            fac += 2.0 * n.v.y * P.beta * (mu * mu) * rcp_pos(sum_rho);
         }
-        fac = vr < 0.0 ? fv : fac;
     }
     fac *= kdr;
     acc[0] = fma(fac, p0, acc[0]);
