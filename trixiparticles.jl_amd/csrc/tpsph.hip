@@ -424,7 +424,7 @@ __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &
     int rg = 0, rl = 0, rl1 = 0;      // BRICK: the union row of the current pencil
     bool row_ok = false, good = true;
     unsigned long long pack = 0ull;    // BRICK: the LDS indices of the open group
-    constexpr unsigned long long kFarSlot = (unsigned long long)(kBrickCap - 1);
+    const unsigned long long kFarSlot = (unsigned long long)(P.brick_cap - 1);
     while (it.next(P, start, j, je)) {
         if constexpr (BRICK) {
             // the pencil is the cell row (iy - 1, iz) of the fluid grid
@@ -453,7 +453,7 @@ __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &
                         nbr[(size_t)n * ls + col] = j + k;
                         if constexpr (BRICK) {
                             const int e = rl + (j + k - rg);
-                            const bool in = row_ok && j + k >= rg && e < rl1 && e < kBrickCap - 1;
+                            const bool in = row_ok && j + k >= rg && e < rl1 && e < P.brick_cap - 1;
                             good = good && in;
                             pack |= (in ? (unsigned long long)e : kFarSlot) << (16 * (n & 3));
                             if ((n & 3) == 3) {
@@ -503,7 +503,7 @@ __device__ __forceinline__ void
         if (nbr16) {
             const int bxi = c0 / P.bc[0], byi = c1 / P.bc[1], bzi = c2 / P.bc[2];
             b = bxi + P.nb[0] * (byi + P.nb[1] * bzi);
-            if (bntot[b] > kBrickCap - 1) b = -1;  // the brick's union does not stage
+            if (bntot[b] > P.brick_cap - 1) b = -1;  // the brick's union does not stage
         }
         if (b >= 0) {
             const int byi = c1 / P.bc[1], bzi = c2 / P.bc[2];
@@ -855,18 +855,23 @@ __global__ void __launch_bounds__(kBlock, 6)
 // a read collide on fewer bank groups (measured 2% and 12% slower).  The last record is the
 // far record that pads the list groups (finite values, zero volume).  With a global brick
 // the same bytes hold the lanes' int lists.
-template <int ND>
+// ANT (DensityDiffusionAntuono): also the renormalised density gradient, g0 {gx, gy} and in
+// 3D g1 {gz}, in CAP = kBrickCapGrad records there (96 bytes each: the LDS holds fewer).
+template <int ND, bool ANT = false>
 struct BrickSmem {
+    static constexpr int CAP = (ANT && ND == 3) ? kBrickCapGrad : kBrickCap;
     // 3D: p0 {x, y}, p1 {z, rho}, p2 {vx, vy}, p3 {vz, p};  2D: p0 {x, y}, p1 {rho, vx},
     // p2 {vy, p} (p3 unused); vol: V = m / rho
-    double2 p0[kBrickCap], p1[kBrickCap], p2[kBrickCap], p3[ND == 3 ? kBrickCap : 1];
-    double vol[kBrickCap];
+    double2 p0[CAP], p1[CAP], p2[CAP], p3[ND == 3 ? CAP : 1];
+    double vol[CAP];
+    double2 g0[ANT ? CAP : 1];
+    double g1[(ANT && ND == 3) ? CAP : 1];
     int row_gb[kBrickMaxRows], row_lds[kBrickMaxRows + 1];
     int item;
 };
 
-template <int ND>
-__device__ __forceinline__ FluidNbr lds_fluid(const BrickSmem<ND> &S, int e)
+template <int ND, bool ANT>
+__device__ __forceinline__ FluidNbr lds_fluid(const BrickSmem<ND, ANT> &S, int e)
 {
     FluidNbr n;
     const double2 a = S.p0[e], b = S.p1[e], c = S.p2[e];
@@ -884,13 +889,24 @@ __device__ __forceinline__ FluidNbr lds_fluid(const BrickSmem<ND> &S, int e)
     return n;
 }
 
+// The density gradient of staged record e (DensityDiffusionAntuono).
+template <int ND, bool ANT>
+__device__ __forceinline__ void lds_grad(const BrickSmem<ND, ANT> &S, int e, double *g)
+{
+    const double2 a = S.g0[e];
+    g[0] = a.x;
+    g[1] = a.y;
+    g[2] = (ND == 3) ? S.g1[e] : 0.0;
+}
+
 // Fluid-fluid kernel over the bricks (one block of kBrickThreads per non-empty brick, taken
 // from a work counter that k_gather<0> resets).  Targets with wall candidates are completed
 // by k_fluid_walls.
 // GLOBAL_ONLY: only the bricks that do not stage, which the mixed-precision pass
 // (tpsph_mixed.hip) leaves to this kernel: the work[1] bricks it listed in bfall, taken by
-// blocks in turn.  VISC: visc_mode(P), the viscosity form of the staged pair loop.
-template <int ND, bool GLOBAL_ONLY = false, int VISC = kViscAlpha>
+// blocks in turn.  VISC: visc_mode(P), the viscosity form of the staged pair loop.  ANT:
+// DensityDiffusionAntuono (P.ddgrad), whose gradients are staged beside the records.
+template <int ND, bool GLOBAL_ONLY = false, int VISC = kViscAlpha, bool ANT = false>
 __global__ void __launch_bounds__(kBrickThreads)
     k_fluid_brick(int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
                   const int *__restrict__ bstart, const int *__restrict__ nel,
@@ -904,9 +920,11 @@ __global__ void __launch_bounds__(kBrickThreads)
     P = with_sound(P);
     // cell-walk fallback lists of a global brick: as many entries per lane as the staging
     // area holds (at most kCap)
-    constexpr int kFallbackCap = std::min<int>(kCap, sizeof(BrickSmem<ND>) / (sizeof(int) * kBrickThreads));
+    using Smem = BrickSmem<ND, ANT>;
+    constexpr int kCapB = Smem::CAP;
+    constexpr int kFallbackCap = std::min<int>(kCap, sizeof(Smem) / (sizeof(int) * kBrickThreads));
     static_assert(kFallbackCap >= 8, "the fallback lists must fit the staging area");
-    __shared__ BrickSmem<ND> S;
+    __shared__ Smem S;
     const int tid = threadIdx.x;
     if (blockIdx.x == 0 && tid == 0) *flag = 0;  // the rebuild (if any) is done
     int *lst = reinterpret_cast<int *>(&S) + tid;  // fallback lists, stride kBrickThreads
@@ -922,7 +940,7 @@ __global__ void __launch_bounds__(kBrickThreads)
         const int b = GLOBAL_ONLY ? bfall[item] : nel[item];
         const int t0 = bstart[b], t1 = bstart[b + 1];
         const int ntot = bntot[b];
-        const bool staged = ntot <= kBrickCap - 1;
+        const bool staged = ntot <= kCapB - 1;
         if (GLOBAL_ONLY && staged) continue;  // uniform per block; no LDS is touched
         if (staged) {
             for (int r = tid; r <= P.maxr; r += kBrickThreads) {
@@ -930,7 +948,11 @@ __global__ void __launch_bounds__(kBrickThreads)
                 S.row_lds[r] = brow_lds[(size_t)b * (P.maxr + 1) + r];
             }
             if (tid == 0) {  // the far record: coordinates kFar, rho 1, the rest 0
-                constexpr int f = kBrickCap - 1;
+                constexpr int f = kCapB - 1;
+                if constexpr (ANT) {
+                    S.g0[f] = make_double2(0.0, 0.0);
+                    if constexpr (ND == 3) S.g1[f] = 0.0;
+                }
                 S.p0[f] = make_double2(kFar, kFar);
                 if constexpr (ND == 3) {
                     S.p1[f] = make_double2(kFar, 1.0);
@@ -959,6 +981,11 @@ __global__ void __launch_bounds__(kBrickThreads)
                     S.p2[i] = make_double2(c.y, c.w);
                 }
                 S.vol[i] = F.vol[g].x;
+                if constexpr (ANT) {
+                    const double *gg = P.ddgrad + (size_t)g * ND;
+                    S.g0[i] = make_double2(gg[0], gg[1]);
+                    if constexpr (ND == 3) S.g1[i] = gg[2];
+                }
             }
             __syncthreads();
         }
@@ -993,6 +1020,14 @@ __global__ void __launch_bounds__(kBrickThreads)
             }
             const double inv_rho_a = 1.0 / rho_a;
             double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
+            double ga[3] = {0.0, 0.0, 0.0};  // DensityDiffusionAntuono: the target's gradient
+            if constexpr (ANT) {
+                if (active) {
+#pragma unroll
+                    for (int d = 0; d < ND; d++) ga[d] = P.ddgrad[(size_t)s * ND + d];
+                }
+            }
+            const double *gap = ANT ? ga : nullptr;
             const int m = max(n, 0);
             if (staged) {
                 // four LDS indices per 8-byte load, one group ahead of the pair terms
@@ -1009,10 +1044,21 @@ __global__ void __launch_bounds__(kBrickThreads)
 #pragma unroll 1
                     for (int h2 = 0; h2 < 2; h2++) {
                         // slots past the end of the list read the far record: exact zeros
-                        const FluidNbr na = lds_fluid<ND>(S, h2 ? cur4.z : cur4.x);
-                        const FluidNbr nb = lds_fluid<ND>(S, h2 ? cur4.w : cur4.y);
-                        pair_fluid_masked<ND, VISC>(P, na, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
-                        pair_fluid_masked<ND, VISC>(P, nb, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                        const int ea = h2 ? cur4.z : cur4.x, eb = h2 ? cur4.w : cur4.y;
+                        const FluidNbr na = lds_fluid<ND>(S, ea);
+                        const FluidNbr nb = lds_fluid<ND>(S, eb);
+                        if constexpr (ANT) {
+                            double gna[3], gnb[3];
+                            lds_grad<ND>(S, ea, gna);
+                            lds_grad<ND>(S, eb, gnb);
+                            pair_fluid_masked<ND, VISC, true>(P, na, true, xa, va, rho_a, p_a,
+                                                              inv_rho_a, acc, drho, ga, gna);
+                            pair_fluid_masked<ND, VISC, true>(P, nb, true, xa, va, rho_a, p_a,
+                                                              inv_rho_a, acc, drho, ga, gnb);
+                        } else {
+                            pair_fluid_masked<ND, VISC>(P, na, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                            pair_fluid_masked<ND, VISC>(P, nb, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                        }
                     }
                     cur4 = nxt4;
                 }
@@ -1023,12 +1069,12 @@ __global__ void __launch_bounds__(kBrickThreads)
                     const double p2 = (ND == 3) ? xa[2] - nb.a.z : 0.0;
                     const double d2 = dist2<ND>(p0, p1, p2);
                     if (d2 <= P.R2 && d2 >= P.az2)
-                        pair_fluid<ND>(P, nb, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                        pair_fluid<ND>(P, nb, xa, va, rho_a, p_a, inv_rho_a, acc, drho, gap);
                 }
                 if (__any(n < 0))
                     neighbour_sweep<ND, false, kFallbackCap, kBrickThreads>(
                         P, F.start, F.x, F.y, F.z, P.hs_fluid, n < 0 && half == 0, F, Wv, lst, xa,
-                        va, rho_a, p_a, inv_rho_a, acc, drho);
+                        va, rho_a, p_a, inv_rho_a, acc, drho, 0.0, gap);
             }
 #pragma unroll
             for (int o = 1; o < kL; o <<= 1) {
@@ -1573,14 +1619,14 @@ __global__ void k_eval_pairs(DevParams P, FluidView F, WallView Wv, EvalPairsArg
         const int n = a.cnt[r];
         int b = -1;
         if (a.bricks) b = bsearch_le(a.bstart, a.nbricks, r);
-        if (b >= 0 && a.bntot[b] <= kBrickCap - 1) {
+        if (b >= 0 && a.bntot[b] <= P.brick_cap - 1) {
             // the brick list: four LDS indices per group, the far record pads the last group
             const int *gb = a.brow_gb + (size_t)b * P.maxr;
             const int *lds = a.brow_lds + (size_t)b * (P.maxr + 1);
             const int ng = (max(n, 0) + 3) >> 2;
             for (int g = 0; g < 4 * ng; g++) {
                 const int e = a.nbr16[((size_t)(g >> 2) * a.ls + r) * 4 + (g & 3)];
-                if (e == kBrickCap - 1) continue;
+                if (e == P.brick_cap - 1) continue;
                 const int row_e = bsearch_le(lds, P.maxr + 1, e);
                 consider(gb[row_e] + (e - lds[row_e]));
             }
@@ -1818,9 +1864,14 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
                     h->P, h->fluid_view(), h->wall_view(), h->ord, h->wtargets, h->counters + 2,
                     h->wnbr, h->wcnt, h->ls, dv);
         } else if (h->search == TPSPH_SEARCH_LISTS && h->bricks) {
-            const auto brick = visc_mode(h->P) == kViscNone ? k_fluid_brick<ND, false, kViscNone>
-                             : visc_mode(h->P) == kViscAlpha ? k_fluid_brick<ND, false, kViscAlpha>
-                                                             : k_fluid_brick<ND, false, kViscAlphaBeta>;
+            const int vm = visc_mode(h->P);
+            const bool ant = h->P.ddgrad != nullptr;
+            const auto brick = ant ? (vm == kViscNone    ? k_fluid_brick<ND, false, kViscNone, true>
+                                      : vm == kViscAlpha ? k_fluid_brick<ND, false, kViscAlpha, true>
+                                                         : k_fluid_brick<ND, false, kViscAlphaBeta, true>)
+                                   : (vm == kViscNone    ? k_fluid_brick<ND, false, kViscNone>
+                                      : vm == kViscAlpha ? k_fluid_brick<ND, false, kViscAlpha>
+                                                         : k_fluid_brick<ND, false, kViscAlphaBeta>);
             hipLaunchKernelGGL(brick, dim3(std::max(1, std::min(h->nbricks, h->n_cu))),
                 dim3(kBrickThreads), 0, h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
                 h->ord, h->bstart, h->nel, h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
@@ -1954,11 +2005,17 @@ int tpsph::derive_params(const tpsph_params *params, int nf, int nw, const doubl
         // and a union of at most about 2020 particles at rest, inside the LDS records of the
         // brick kernel (full 3-span bricks: 512 targets, and the unions of 5-10 % of the
         // bricks exceed the records and take the slower global path)
+        // DensityDiffusionAntuono stages the density gradients too: in 3D its records hold
+        // kBrickCapGrad particles, so its bricks are one cell shorter along y and z (unions of
+        // about 5/6 the size, about 320 targets)
         const int per = nd == 3 ? 3 : 8;
+        const bool grad = params->use_density_diffusion == TPSPH_DIFFUSION_ANTUONO && nd == 3;
+        P.brick_cap = grad ? kBrickCapGrad : kBrickCap;
         int64_t nb = 1;
         for (int d = 0; d < 3; d++) {
             P.bc[d] = d < nd ? per * (d == 0 ? xdiv : div) : 1;
             if (nd == 3 && d == 0) P.bc[d] = std::max(1, P.bc[d] * 11 / 12);
+            if (grad && d >= 1) P.bc[d] = std::max(1, P.bc[d] - 1);
             P.nb[d] = (P.n[d] + P.bc[d] - 1) / P.bc[d];
             nb *= P.nb[d];
         }
@@ -2484,10 +2541,9 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
         CTRY(dalloc(h, &h->cnt, nf));
         CTRY(dalloc(h, &h->ord, nf));
         CTRY(dalloc(h, &h->counters, 4));
-        // the brick kernel's LDS records carry no density gradients: DensityDiffusionAntuono
-        // takes the list kernel
-        h->bricks = P.maxr <= kBrickMaxRows &&
-                    params->use_density_diffusion != TPSPH_DIFFUSION_ANTUONO;
+        // the brick kernel (with DensityDiffusionAntuono its records also carry the density
+        // gradients) whenever every brick's union rows fit one scan block
+        h->bricks = P.maxr <= kBrickMaxRows;
         if (h->bricks) {
             CTRY(dalloc(h, &h->brow_gb, (size_t)h->nbricks * P.maxr));
             CTRY(dalloc(h, &h->brow_lds, (size_t)h->nbricks * (P.maxr + 1)));

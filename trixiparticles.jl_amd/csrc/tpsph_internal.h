@@ -26,6 +26,8 @@ constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kBrickThreads = 1024;   // threads of the brick kernel (one block per CU), two
                                       // per target
 constexpr int kBrickCap = 2048;       // LDS records of a brick: the union plus one far record
+constexpr int kBrickCapGrad = 1664;   // the same with DensityDiffusionAntuono's density
+                                      // gradients in the records (3D: 96 instead of 72 bytes)
 constexpr double kFar = 1e100;        // coordinate of the far record (never within R)
 constexpr int kBrickMaxRows = 256;    // union rows per brick (one scan block)
 constexpr int kGateBlocks = 1024;     // grid-stride kernels behind the rebuild flag: a small
@@ -64,6 +66,9 @@ struct DevParams {
     // radius (list radius + margin) and cell growth (margin) of the union
     int ur[3], maxr;
     double rl_union2, hs_union;
+    // LDS records of a brick (kBrickCap, or kBrickCapGrad with DensityDiffusionAntuono in
+    // 3D): a union of up to brick_cap - 1 particles stages, the last record is the far record
+    int brick_cap;
     // StateEquationAdaptiveCole: {c, B, av_c, dd2} of the current kick in device memory
     // (k_sound_speed writes them), or nullptr for the constants above
     const double *dyn;
@@ -480,11 +485,16 @@ inline int visc_mode(const DevParams &P)
 // of a zero).  Every intermediate of an out-of-range candidate stays finite or is selected
 // away: the far record (coordinates 1e100, zero volume) gives d2 about 1e200, whose
 // reciprocals are normal numbers.
-template <int ND, int VISC>
+//
+// ga, gb (DensityDiffusionAntuono only): the renormalised density gradients of the target and
+// the candidate, whose term is pair_fluid's.
+template <int ND, int VISC, bool ANT = false>
 __device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const FluidNbr &n,
                                                   bool valid, const double *xa,
                                                   const double *va, double rho_a, double p_a,
-                                                  double inv_rho_a, double *acc, double &drho)
+                                                  double inv_rho_a, double *acc, double &drho,
+                                                  const double *ga = nullptr,
+                                                  const double *gb = nullptr)
 {
     const double p0 = xa[0] - n.a.x, p1 = xa[1] - n.a.y, p2 = (ND == 3) ? xa[2] - n.a.z : 0.0;
     const double d2 = dist2<ND>(p0, p1, p2);
@@ -511,7 +521,13 @@ __device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const Flui
     acc[0] = fma(fac, p0, acc[0]);
     acc[1] = fma(fac, p1, acc[1]);
     if constexpr (ND == 3) acc[2] = fma(fac, p2, acc[2]);
-    drho = fma(n.v.x * kdr, fma(P.dd2, rho_a - rho_b, rho_a * vr), drho);
+    double drr = rho_a - rho_b;
+    if constexpr (ANT) {
+        double gr = (ga[0] + gb[0]) * p0 + (ga[1] + gb[1]) * p1;
+        if constexpr (ND == 3) gr += (ga[2] + gb[2]) * p2;
+        drr = fma(-0.5, gr, drr);
+    }
+    drho = fma(n.v.x * kdr, fma(P.dd2, drr, rho_a * vr), drho);
 }
 
 // Two-phase sweep over the neighbours of one target: phase 1 runs the exact radius test over
