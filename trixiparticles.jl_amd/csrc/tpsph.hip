@@ -106,7 +106,7 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
     P = with_sound(P);
     const int f = *flag;
     // MODE 0 runs first in every kick: it resets the work counter of the brick kernel
-    if (MODE == 0 && work && blockIdx.x == 0 && threadIdx.x == 0) work[0] = work[1] = 0;
+    if (MODE == 0 && work && blockIdx.x == 0 && threadIdx.x == 0) work[0] = work[1] = work[2] = 0;
     if ((MODE == 0) == (f != 0)) return;
     const size_t nf = (size_t)P.nf;
     // with Verlet lists the SoA coordinates are read between rebuilds only by the cell walk
@@ -1155,8 +1155,8 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// DensityDiffusionAntuono's update! (density_diffusion.jl:176-211), one lane per fluid
-// particle a in cell-sorted order, over its fluid neighbours b:
+// DensityDiffusionAntuono's update! (density_diffusion.jl:176-211), over the fluid
+// neighbours b of fluid particle a:
 //   M_a = -sum_b V_b grad W_ab (x) r_ab        (compute_gradient_correction_matrix!,
 //                                               corrections.jl:329-354)
 //   L_a = M_a^-1, or the identity where |det M_a| < 1f-9
@@ -1167,88 +1167,68 @@ __global__ void __launch_bounds__(kBlock)
 //                                               multiplies the sum once)
 // with V_b = m_b / rho_b and grad W zero at distance >= R and at distance^2 < eps(h^2)
 // (smoothing_kernel_grad, abstract_system.jl:142-155).  M_a is symmetric: its upper
-// triangle is summed.  Four lanes share a particle and take every fourth candidate; their
-// partial sums are added with shuffles.  With Verlet lists (nbr != nullptr) group t takes
-// the particle ord[t] and its list; a particle whose list overflowed, and every particle of
-// the cell search, walks the cells over the SoA coordinates, which are current whenever
-// they are walked.
-template <int ND>
-__global__ void __launch_bounds__(kBlock)
-    k_antuono_gradient(DevParams P, FluidView F, const int *__restrict__ ord,
-                       const int *__restrict__ nbr, const int *__restrict__ cnt, int ls)
-{
-    constexpr int kLanes = 4;
-    const int q = threadIdx.x % kLanes;
-    const int t = (blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
-    const bool active = t < P.nf;
-    const int tc = active ? t : P.nf - 1;  // an in-bounds particle for the idle lanes
-    const int s = nbr ? ord[tc] : tc;
-    const double4 a = F.rec[2 * (size_t)s];
-    const double xa[3] = {a.x, a.y, a.z};
-    const double rho_a = a.w;
+// triangle is summed.  Four lanes share a particle; their partial sums are added with
+// shuffles.
+struct GradSums {
     double m[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // M_00 M_01 M_02 M_11 M_12 M_22
     double sv[3] = {0.0, 0.0, 0.0};
-    auto term = [&](int j, double p0, double p1, double p2, double d2, double rho_b) {
-        if (!(d2 < P.R2 && d2 >= P.eps_hh)) return;
-        const double r = sqrt_pos(d2);
-        const double w = fma(-0.5 * P.h_inv, r, 1.0);
-        const double kdr = P.kgrad_c * (w * w * w);
-        const double vk = F.vol[j].x * kdr;  // V_b |grad W| / r
-        const double f = (rho_a - rho_b) * vk;
-        m[0] -= vk * p0 * p0;
-        m[1] -= vk * p0 * p1;
-        m[3] -= vk * p1 * p1;
-        sv[0] += f * p0;
-        sv[1] += f * p1;
-        if constexpr (ND == 3) {
-            m[2] -= vk * p0 * p2;
-            m[4] -= vk * p1 * p2;
-            m[5] -= vk * p2 * p2;
-            sv[2] += f * p2;
-        }
-    };
-    const int n = !active ? 0 : (nbr ? cnt[t] : -1);
-    if (n >= 0) {
-        for (int k = q; k < n; k += kLanes) {
-            const int j = nbr[(size_t)k * ls + t];
-            const double4 b = F.rec[2 * (size_t)j];
-            const double p0 = xa[0] - b.x, p1 = xa[1] - b.y;
-            const double p2 = (ND == 3) ? xa[2] - b.z : 0.0;
-            term(j, p0, p1, p2, dist2<ND>(p0, p1, p2), b.w);
-        }
-    } else {
-        Pencils<ND> it;
-        it.init(P, xa[0], xa[1], xa[2], P.hs_fluid);
-        int jb, je;
-        while (it.next(P, F.start, jb, je)) {
-            for (int j = jb + q; j < je; j += kLanes) {
-                const double p0 = xa[0] - F.x[j], p1 = xa[1] - F.y[j];
-                const double p2 = (ND == 3) ? xa[2] - F.z[j] : 0.0;
-                const double d2 = dist2<ND>(p0, p1, p2);
-                if (d2 < P.R2) term(j, p0, p1, p2, d2, F.rec[2 * (size_t)j].w);
-            }
-        }
+};
+
+// One pair's terms; vk = V_b |grad W| / r (zero for a pair outside the kernel's support).
+template <int ND>
+__device__ __forceinline__ void grad_add(GradSums &g, double vk, double drr, double p0,
+                                         double p1, double p2)
+{
+    const double f = drr * vk;
+    g.m[0] -= vk * p0 * p0;
+    g.m[1] -= vk * p0 * p1;
+    g.m[3] -= vk * p1 * p1;
+    g.sv[0] += f * p0;
+    g.sv[1] += f * p1;
+    if constexpr (ND == 3) {
+        g.m[2] -= vk * p0 * p2;
+        g.m[4] -= vk * p1 * p2;
+        g.m[5] -= vk * p2 * p2;
+        g.sv[2] += f * p2;
     }
+}
+
+// V_b |grad W| / r of a pair at squared distance d2, zero outside [eps(h^2), R^2) without a
+// branch (the square root of an excluded d2 may be NaN: it is selected away).
+__device__ __forceinline__ double grad_vk(const DevParams &P, double V_b, double d2)
+{
+    const bool in = (d2 < P.R2) & (d2 >= P.eps_hh);
+    const double r = sqrt_pos(d2);
+    const double w = fma(-0.5 * P.h_inv, r, 1.0);
+    return in ? V_b * (P.kgrad_c * (w * w * w)) : 0.0;
+}
+
+// Sums of the four lanes of a quad (a fixed butterfly), then the inversion and the gradient
+// of particle s (lane 0 of an active quad writes it).
+template <int ND>
+__device__ __forceinline__ void grad_finish(const DevParams &P, GradSums &g, bool write, int s)
+{
 #pragma unroll
-    for (int o = 1; o < kLanes; o <<= 1) {
+    for (int o = 1; o < 4; o <<= 1) {
 #pragma unroll
         for (int i = 0; i < 6; i++)
-            if (ND == 3 || i == 0 || i == 1 || i == 3) m[i] += __shfl_xor(m[i], o);
+            if (ND == 3 || i == 0 || i == 1 || i == 3) g.m[i] += __shfl_xor(g.m[i], o);
 #pragma unroll
-        for (int i = 0; i < ND; i++) sv[i] += __shfl_xor(sv[i], o);
+        for (int i = 0; i < ND; i++) g.sv[i] += __shfl_xor(g.sv[i], o);
     }
-    if (!active || q != 0) return;
+    if (!write) return;
+    const double *m = g.m, *sv = g.sv;
     constexpr double kSingular = (double)1.0e-9f;  // the Float32 literal 1.0f-9
-    double g[3];
+    double gr[3];
     if constexpr (ND == 2) {
         const double det = m[0] * m[3] - m[1] * m[1];
         if (fabs(det) < kSingular) {
-            g[0] = -sv[0];
-            g[1] = -sv[1];
+            gr[0] = -sv[0];
+            gr[1] = -sv[1];
         } else {
             const double id = 1.0 / det;
-            g[0] = -(m[3] * sv[0] - m[1] * sv[1]) * id;
-            g[1] = -(m[0] * sv[1] - m[1] * sv[0]) * id;
+            gr[0] = -(m[3] * sv[0] - m[1] * sv[1]) * id;
+            gr[1] = -(m[0] * sv[1] - m[1] * sv[0]) * id;
         }
     } else {
         // cofactors of the symmetric M (its adjugate is symmetric too)
@@ -1260,18 +1240,170 @@ __global__ void __launch_bounds__(kBlock)
         const double c22 = m[0] * m[3] - m[1] * m[1];
         const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
         if (fabs(det) < kSingular) {
-            g[0] = -sv[0];
-            g[1] = -sv[1];
-            g[2] = -sv[2];
+            gr[0] = -sv[0];
+            gr[1] = -sv[1];
+            gr[2] = -sv[2];
         } else {
             const double id = 1.0 / det;
-            g[0] = -(c00 * sv[0] + c01 * sv[1] + c02 * sv[2]) * id;
-            g[1] = -(c01 * sv[0] + c11 * sv[1] + c12 * sv[2]) * id;
-            g[2] = -(c02 * sv[0] + c12 * sv[1] + c22 * sv[2]) * id;
+            gr[0] = -(c00 * sv[0] + c01 * sv[1] + c02 * sv[2]) * id;
+            gr[1] = -(c01 * sv[0] + c11 * sv[1] + c12 * sv[2]) * id;
+            gr[2] = -(c02 * sv[0] + c12 * sv[1] + c22 * sv[2]) * id;
         }
     }
 #pragma unroll
-    for (int d = 0; d < ND; d++) P.ddgrad[(size_t)s * ND + d] = g[d];
+    for (int d = 0; d < ND; d++) P.ddgrad[(size_t)s * ND + d] = gr[d];
+}
+
+// The sums of particle s from global memory: its Verlet list (n >= 0 entries in column t,
+// lane q takes entries q, q + 4, ...) or, for an overflowed list or the cell search
+// (n < 0), the cell walk over the SoA coordinates, which are current whenever they are
+// walked.
+template <int ND>
+__device__ __forceinline__ void grad_sums_global(const DevParams &P, const FluidView &F,
+                                                 GradSums &g, const double *xa, double rho_a,
+                                                 int n, const int *__restrict__ nbr, int t,
+                                                 int ls, int q)
+{
+    if (n >= 0) {
+        for (int k = q; k < n; k += 4) {
+            const int j = nbr[(size_t)k * ls + t];
+            const double4 b = F.rec[2 * (size_t)j];
+            const double p0 = xa[0] - b.x, p1 = xa[1] - b.y;
+            const double p2 = (ND == 3) ? xa[2] - b.z : 0.0;
+            const double d2 = dist2<ND>(p0, p1, p2);
+            if (d2 < P.R2 && d2 >= P.eps_hh)
+                grad_add<ND>(g, grad_vk(P, F.vol[j].x, d2), rho_a - b.w, p0, p1, p2);
+        }
+    } else {
+        Pencils<ND> it;
+        it.init(P, xa[0], xa[1], xa[2], P.hs_fluid);
+        int jb, je;
+        while (it.next(P, F.start, jb, je)) {
+            for (int j = jb + q; j < je; j += 4) {
+                const double p0 = xa[0] - F.x[j], p1 = xa[1] - F.y[j];
+                const double p2 = (ND == 3) ? xa[2] - F.z[j] : 0.0;
+                const double d2 = dist2<ND>(p0, p1, p2);
+                if (d2 < P.R2 && d2 >= P.eps_hh)
+                    grad_add<ND>(g, grad_vk(P, F.vol[j].x, d2), rho_a - F.rec[2 * (size_t)j].w,
+                                 p0, p1, p2);
+            }
+        }
+    }
+}
+
+// The gradients over the Verlet lists (group t takes the particle ord[t] and its list) or,
+// without lists (nbr == nullptr), the cell walk of every particle.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_antuono_gradient(DevParams P, FluidView F, const int *__restrict__ ord,
+                       const int *__restrict__ nbr, const int *__restrict__ cnt, int ls)
+{
+    const int q = threadIdx.x % 4;
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) / 4;
+    const bool active = t < P.nf;
+    const int tc = active ? t : P.nf - 1;  // an in-bounds particle for the idle lanes
+    const int s = nbr ? ord[tc] : tc;
+    const double4 a = F.rec[2 * (size_t)s];
+    const double xa[3] = {a.x, a.y, a.z};
+    GradSums g;
+    const int n = !active ? 0 : (nbr ? cnt[t] : -1);
+    grad_sums_global<ND>(P, F, g, xa, a.w, n, nbr, t, ls, q);
+    grad_finish<ND>(P, g, active && q == 0, s);
+}
+
+// The gradients brick by brick (the lists' LDS indices, the union staged in LDS as
+// {x, y} and {z, rho} fields and V, the far record padding the list groups), as
+// k_fluid_brick takes the fluid pass; bricks that do not stage read global memory.  Blocks
+// take bricks from work[2].
+template <int ND>
+struct GradSmem {
+    double2 p0[kBrickCap], p1[kBrickCap];  // {x, y}, {z, rho} (2D: {0, rho})
+    double vol[kBrickCap];
+    int row_gb[kBrickMaxRows], row_lds[kBrickMaxRows + 1];
+    int item;
+};
+
+template <int ND>
+__global__ void __launch_bounds__(kBrickThreads)
+    k_antuono_gradient_brick(DevParams P, FluidView F, const int *__restrict__ ord,
+                             const int *__restrict__ bstart, const int *__restrict__ nel,
+                             const int *__restrict__ nne, int *__restrict__ work,
+                             const int *__restrict__ brow_gb, const int *__restrict__ brow_lds,
+                             const int *__restrict__ bntot,
+                             const unsigned short *__restrict__ nbr16,
+                             const int *__restrict__ nbr, const int *__restrict__ cnt, int ls)
+{
+    __shared__ GradSmem<ND> S;
+    const int tid = threadIdx.x;
+    const int ne = *nne;
+    const int cap = P.brick_cap;
+    while (true) {
+        if (tid == 0) S.item = atomicAdd(&work[2], 1);
+        __syncthreads();
+        const int item = S.item;
+        if (item >= ne) break;
+        const int b = nel[item];
+        const int t0 = bstart[b], t1 = bstart[b + 1];
+        const int ntot = bntot[b];
+        const bool staged = ntot <= cap - 1;
+        if (staged) {
+            for (int r = tid; r <= P.maxr; r += kBrickThreads) {
+                if (r < P.maxr) S.row_gb[r] = brow_gb[(size_t)b * P.maxr + r];
+                S.row_lds[r] = brow_lds[(size_t)b * (P.maxr + 1) + r];
+            }
+            if (tid == 0) {  // the far record: far away, zero volume
+                S.p0[cap - 1] = make_double2(kFar, kFar);
+                S.p1[cap - 1] = make_double2(kFar, 1.0);
+                S.vol[cap - 1] = 0.0;
+            }
+            __syncthreads();
+            for (int i = tid; i < ntot; i += kBrickThreads) {
+                const int r = bsearch_le(S.row_lds, P.maxr + 1, i);
+                const int g = S.row_gb[r] + (i - S.row_lds[r]);
+                const double4 a = F.rec[2 * (size_t)g];
+                S.p0[i] = make_double2(a.x, a.y);
+                S.p1[i] = make_double2(ND == 3 ? a.z : 0.0, a.w);
+                S.vol[i] = F.vol[g].x;
+            }
+            __syncthreads();
+        }
+        const int q = tid & 3;
+        for (int base = t0; base < t1; base += kBrickThreads / 4) {
+            const int t = base + tid / 4;
+            const bool active = t < t1;
+            const int col = active ? t : t0;
+            const int s = ord[col];
+            const double4 a = F.rec[2 * (size_t)s];
+            const double xa[3] = {a.x, a.y, ND == 3 ? a.z : 0.0};
+            const int n = active ? cnt[col] : 0;
+            GradSums gs;
+            if (staged) {
+                const ushort4 *l4 = reinterpret_cast<const ushort4 *>(nbr16);
+                const int ng = (max(n, 0) + 3) >> 2;
+                // rows clamped into the list rather than a select between a load and a
+                // register (which made the compiler issue flat loads)
+                ushort4 cur4 = l4[(size_t)min(q, max(ng - 1, 0)) * ls + col];
+                for (int g = q; g < ng; g += 4) {
+                    const ushort4 nxt4 = l4[(size_t)min(g + 4, ng - 1) * ls + col];
+                    const unsigned short e4[4] = {cur4.x, cur4.y, cur4.z, cur4.w};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int e = e4[k];
+                        const double2 b0 = S.p0[e], b1 = S.p1[e];
+                        const double p0 = xa[0] - b0.x, p1 = xa[1] - b0.y;
+                        const double p2 = (ND == 3) ? xa[2] - b1.x : 0.0;
+                        const double d2 = dist2<ND>(p0, p1, p2);
+                        grad_add<ND>(gs, grad_vk(P, S.vol[e], d2), a.w - b1.y, p0, p1, p2);
+                    }
+                    cur4 = nxt4;
+                }
+            } else {
+                grad_sums_global<ND>(P, F, gs, xa, a.w, n, nbr, col, ls, q);
+            }
+            grad_finish<ND>(P, gs, active && q == 0, s);
+        }
+        __syncthreads();  // the LDS is staged again for the next brick
+    }
 }
 
 // drift!: du = v (semidiscretization.jl:561-571; delta_v = 0 without shifting)
@@ -1323,7 +1455,7 @@ __global__ void __launch_bounds__(kBlock)
         v[k] = fma(b, t, v[k]);
     }
     __syncthreads();  // the block's new u and v are complete
-    if (work && blockIdx.x == 0 && threadIdx.x == 0) work[0] = work[1] = 0;
+    if (work && blockIdx.x == 0 && threadIdx.x == 0) work[0] = work[1] = work[2] = 0;
     const int i = i0 + (int)threadIdx.x;
     bool moved = false;  // left its skin
     if (i < nf) {
@@ -1848,9 +1980,15 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
         TimedScope t(h, TPSPH_TIME_FLUID);
         if (h->P.ddgrad) {
             const bool lists = h->search == TPSPH_SEARCH_LISTS;
-            hipLaunchKernelGGL((k_antuono_gradient<ND>), dim3(nblocks(4 * (int64_t)nf)), dim3(kBlock), 0,
-                               h->stream, h->P, h->fluid_view(), lists ? h->ord : nullptr,
-                               lists ? h->nbr : nullptr, lists ? h->cnt : nullptr, h->ls);
+            if (lists && h->bricks)
+                hipLaunchKernelGGL((k_antuono_gradient_brick<ND>),
+                    dim3(std::max(1, std::min(h->nbricks, h->n_cu))), dim3(kBrickThreads), 0,
+                    h->stream, h->P, h->fluid_view(), h->ord, h->bstart, h->nel, h->counters,
+                    h->bwork, h->brow_gb, h->brow_lds, h->bntot, h->nbr16, h->nbr, h->cnt, h->ls);
+            else
+                hipLaunchKernelGGL((k_antuono_gradient<ND>), dim3(nblocks(4 * (int64_t)nf)), dim3(kBlock), 0,
+                                   h->stream, h->P, h->fluid_view(), lists ? h->ord : nullptr,
+                                   lists ? h->nbr : nullptr, lists ? h->cnt : nullptr, h->ls);
             TRY(launch_check(h, "k_antuono_gradient"));
         }
         if (h->search == TPSPH_SEARCH_LISTS && h->bricks && h->mixed) {
@@ -2550,8 +2688,10 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
             CTRY(dalloc(h, &h->bntot, h->nbricks));
             CTRY(dalloc(h, &h->nel, h->nbricks));
             // [0] the brick kernels' work counter, [1] the bricks the mixed-precision pass
-            // leaves to the FP64 pass (listed in bfall); both cleared by the kick's gather
-            CTRY(dalloc(h, &h->bwork, 2));
+            // leaves to the FP64 pass (listed in bfall), [2] the work counter of the brick
+            // gradient pass (DensityDiffusionAntuono); all cleared by the kick's gather or
+            // the stage kernel
+            CTRY(dalloc(h, &h->bwork, 3));
             if (h->mixed) CTRY(dalloc(h, &h->bfall, (size_t)h->nbricks));
             CTRY(dalloc(h, &h->nbr16, (size_t)h->cap_f * h->ls));
         }
