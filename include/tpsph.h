@@ -120,8 +120,8 @@ enum tpsph_timed_kernel {
  *                                      (density_diffusion.jl:41-55) or DensityDiffusionAntuono
  *                                      (density_diffusion.jl:89-211), whose renormalised
  *                                      density gradients every kick computes in a pass of its
- *                                      own; Antuono runs with the Verlet lists (without the
- *                                      brick kernel) or the cell walk
+ *                                      own; Antuono runs with the Verlet lists (the brick
+ *                                      kernel stages the gradients) or the cell walk
  *   acceleration                       `acceleration` of the fluid system (gravity)
  *   clip_wall_pressure                 BoundaryModelDummyParticles(; clip_negative_pressure)
  *   pressure_offset                    AdamiPressureExtrapolation(; pressure_offset)
@@ -169,8 +169,10 @@ enum tpsph_timed_kernel {
  *                                      -m_b (p_a/rho_a^2 + p_b/rho_b^2) grad W
  *                                      (pressure_acceleration.jl:8-15, 139-145) and has no
  *                                      continuity or density-diffusion term (fluid.jl:
- *                                      133-139); it runs the cell-walk pair search
- *                                      (TPSPH_SEARCH_CELLS; AUTO selects it) in FP64
+ *                                      133-139); it runs in FP64, with the Verlet lists
+ *                                      (summation pass over the lists, fluid pass on the
+ *                                      brick kernel; AUTO selects them when skin > 0) or the
+ *                                      cell walk
  */
 typedef struct tpsph_params {
     int32_t ndims;                 /* 2 or 3 */

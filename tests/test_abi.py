@@ -124,10 +124,9 @@ def test_create_rejects_mixed_precision_with_a_quadratic_viscosity(tp):
     assert "beta = 0" in err
 
 
-@pytest.mark.parametrize("fields", [{"pair_search": 1}, {"precision": 1}])
-def test_create_rejects_summation_density_off_the_cell_walk(tp, fields):
-    """SummationDensity runs the cell-walk pair search in FP64."""
-    rc, err = _create(tp, density_calculator=1, **fields)
+def test_create_rejects_summation_density_in_mixed_precision(tp):
+    """SummationDensity runs in FP64 (on the cell walk or the brick kernel)."""
+    rc, err = _create(tp, density_calculator=1, precision=1)
     assert rc == tp.native.TPSPH_ERR_ARGUMENT
     assert "SummationDensity" in err
 

@@ -206,14 +206,17 @@ def _state(db, seed):
 
 @pytest.mark.gpu
 @needs_gpu
+@pytest.mark.parametrize("search", ["lists", "cells"])
 @pytest.mark.parametrize("nd,dx,walls,backend", [(2, 0.01, True, "hip"), (2, 0.02, False, "hip"),
                                                   (3, 0.02, True, "hip"),
                                                   (2, 0.02, True, "hip_host")])
-def test_summation_kick_matches_oracle(tp, orc, nd, dx, walls, backend):
+def test_summation_kick_matches_oracle(tp, orc, nd, dx, walls, backend, search):
+    """The summation pass and the fluid pass over the Verlet lists (the brick kernel) or the
+    cell walk, against the oracle."""
     import torch
     db = tp.dam_break(nd, dx, density_calculator=tp.SummationDensity())
     systems = (db.fluid_system, db.boundary_system) if walls else (db.fluid_system,)
-    semi = tp.Semidiscretization(*systems, parallelization_backend=backend)
+    semi = tp.Semidiscretization(*systems, parallelization_backend=backend, pair_search=search)
     v, u = _state(db, 3)
     P = orc.params_from_semi(semi)
     xw = db.boundary_system.coordinates.T if walls else None
@@ -247,9 +250,10 @@ def test_summation_kick_matches_oracle(tp, orc, nd, dx, walls, backend):
 
 @pytest.mark.gpu
 @needs_gpu
-def test_summation_trajectory_matches_oracle(tp, orc):
+@pytest.mark.parametrize("search", ["lists", "cells"])
+def test_summation_trajectory_matches_oracle(tp, orc, search):
     db = tp.dam_break(2, 0.02, density_calculator=tp.SummationDensity())
-    semi = tp.Semidiscretization(db.fluid_system, db.boundary_system)
+    semi = tp.Semidiscretization(db.fluid_system, db.boundary_system, pair_search=search)
     ode = tp.semidiscretize(semi, (0.0, 20e-5))
     v, u = tp.solve(ode, dt=1e-5, nsteps=20)
     P = orc.params_from_semi(semi)

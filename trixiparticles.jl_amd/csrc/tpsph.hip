@@ -344,6 +344,67 @@ __global__ void __launch_bounds__(kBlock)
     rho[s] = sum;
 }
 
+// The same sums over the Verlet lists (group t of four lanes takes the particle ord[t]): the
+// particle itself (m W(0)), its fluid list and its wall list, each within the compact
+// support; an overflowed list walks the cells instead (the SoA coordinates are current
+// while some list overflows; the walls are static).  The four lanes' partial sums are added
+// in a fixed order.
+template <int ND>
+__global__ void __launch_bounds__(kBlock)
+    k_sum_density_lists(DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
+                        const int *__restrict__ nbr, const int *__restrict__ cnt,
+                        const int *__restrict__ wnbr, const int *__restrict__ wcnt, int ls,
+                        double *__restrict__ rho)
+{
+    const int q = threadIdx.x % 4;
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) / 4;
+    const bool active = t < P.nf;
+    const int tc = active ? t : P.nf - 1;  // an in-bounds particle for the idle lanes
+    const int s = ord[tc];
+    const double4 a = F.rec[2 * (size_t)s];
+    const double x0 = a.x, x1 = a.y, x2 = (ND == 3) ? a.z : 0.0;
+    double sum = (q == 0) ? F.vol[s].y * wendland_c2(P, 0.0) : 0.0;
+    const int n = active ? cnt[tc] : 0;
+    if (n >= 0) {
+        for (int k = q; k < n; k += 4) {
+            const int j = nbr[(size_t)k * ls + tc];
+            const double4 b = F.rec[2 * (size_t)j];
+            const double d2 = dist2<ND>(x0 - b.x, x1 - b.y, (ND == 3) ? x2 - b.z : 0.0);
+            if (d2 <= P.R2) sum += F.vol[j].y * wendland_c2(P, sqrt(d2));
+        }
+    } else {
+        Pencils<ND> it;
+        it.init(P, x0, x1, x2, P.hs_fluid);
+        int jb, je;
+        while (it.next(P, F.start, jb, je))
+            for (int j = jb + q; j < je; j += 4) {
+                const double d2 = dist2<ND>(x0 - F.x[j], x1 - F.y[j], (ND == 3) ? x2 - F.z[j] : 0.0);
+                if (d2 <= P.R2 && j != s) sum += F.vol[j].y * wendland_c2(P, sqrt(d2));
+            }
+    }
+    const int nw = (active && P.nw > 0) ? wcnt[tc] : 0;
+    if (nw > 0) {
+        for (int k = q; k < nw; k += 4) {
+            const double4 xw = Wv.pos[wnbr[(size_t)k * ls + tc]];
+            const double d2 = dist2<ND>(x0 - xw.x, x1 - xw.y, (ND == 3) ? x2 - xw.z : 0.0);
+            if (d2 <= P.R2) sum += xw.w * wendland_c2(P, sqrt(d2));
+        }
+    } else if (nw < 0) {
+        Pencils<ND> iw;
+        iw.init(P, x0, x1, x2, P.hs_wall);
+        int jb, je;
+        while (iw.next(P, Wv.start, jb, je))
+            for (int w = jb + q; w < je; w += 4) {
+                const double4 xw = Wv.pos[w];
+                const double d2 = dist2<ND>(x0 - xw.x, x1 - xw.y, (ND == 3) ? x2 - xw.z : 0.0);
+                if (d2 <= P.R2) sum += xw.w * wendland_c2(P, sqrt(d2));
+            }
+    }
+    sum += __shfl_xor(sum, 1);
+    sum += __shfl_xor(sum, 2);
+    if (active && q == 0) rho[s] = sum;
+}
+
 // The records of the summed densities: rho, the state equation's pressure and V = m / rho.
 __global__ void __launch_bounds__(kBlock)
     k_sum_finish(DevParams P, int nf, const double *__restrict__ rho, double4 *__restrict__ rec,
@@ -905,8 +966,10 @@ __device__ __forceinline__ void lds_grad(const BrickSmem<ND, ANT> &S, int e, dou
 // GLOBAL_ONLY: only the bricks that do not stage, which the mixed-precision pass
 // (tpsph_mixed.hip) leaves to this kernel: the work[1] bricks it listed in bfall, taken by
 // blocks in turn.  VISC: visc_mode(P), the viscosity form of the staged pair loop.  ANT:
-// DensityDiffusionAntuono (P.ddgrad), whose gradients are staged beside the records.
-template <int ND, bool GLOBAL_ONLY = false, int VISC = kViscAlpha, bool ANT = false>
+// DensityDiffusionAntuono (P.ddgrad), whose gradients are staged beside the records.  SUM:
+// SummationDensity (pressure slot staged as p / rho^2, no density rate, dv of ND rows).
+template <int ND, bool GLOBAL_ONLY = false, int VISC = kViscAlpha, bool ANT = false,
+          bool SUM = false>
 __global__ void __launch_bounds__(kBrickThreads)
     k_fluid_brick(int *flag, DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
                   const int *__restrict__ bstart, const int *__restrict__ nel,
@@ -970,15 +1033,16 @@ __global__ void __launch_bounds__(kBrickThreads)
                 const int g = S.row_gb[r] + (i - S.row_lds[r]);
                 const double4 a = F.rec[2 * (size_t)g];
                 const double4 c = F.rec[2 * (size_t)g + 1];
+                const double pslot = SUM ? c.w / (a.w * a.w) : c.w;
                 if constexpr (ND == 3) {
                     S.p0[i] = make_double2(a.x, a.y);
                     S.p1[i] = make_double2(a.z, a.w);
                     S.p2[i] = make_double2(c.x, c.y);
-                    S.p3[i] = make_double2(c.z, c.w);
+                    S.p3[i] = make_double2(c.z, pslot);
                 } else {
                     S.p0[i] = make_double2(a.x, a.y);
                     S.p1[i] = make_double2(a.w, c.x);
-                    S.p2[i] = make_double2(c.y, c.w);
+                    S.p2[i] = make_double2(c.y, pslot);
                 }
                 S.vol[i] = F.vol[g].x;
                 if constexpr (ANT) {
@@ -1019,6 +1083,7 @@ __global__ void __launch_bounds__(kBrickThreads)
                 if (P.nw > 0) nw = wcnt[col];
             }
             const double inv_rho_a = 1.0 / rho_a;
+            if constexpr (SUM) p_a = p_a / (rho_a * rho_a);  // the target's p / rho^2
             double acc[3] = {0.0, 0.0, 0.0}, drho = 0.0;
             double ga[3] = {0.0, 0.0, 0.0};  // DensityDiffusionAntuono: the target's gradient
             if constexpr (ANT) {
@@ -1056,8 +1121,10 @@ __global__ void __launch_bounds__(kBrickThreads)
                             pair_fluid_masked<ND, VISC, true>(P, nb, true, xa, va, rho_a, p_a,
                                                               inv_rho_a, acc, drho, ga, gnb);
                         } else {
-                            pair_fluid_masked<ND, VISC>(P, na, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
-                            pair_fluid_masked<ND, VISC>(P, nb, true, xa, va, rho_a, p_a, inv_rho_a, acc, drho);
+                            pair_fluid_masked<ND, VISC, false, SUM>(P, na, true, xa, va, rho_a, p_a,
+                                                                    inv_rho_a, acc, drho);
+                            pair_fluid_masked<ND, VISC, false, SUM>(P, nb, true, xa, va, rho_a, p_a,
+                                                                    inv_rho_a, acc, drho);
                         }
                     }
                     cur4 = nxt4;
@@ -1069,10 +1136,10 @@ __global__ void __launch_bounds__(kBrickThreads)
                     const double p2 = (ND == 3) ? xa[2] - nb.a.z : 0.0;
                     const double d2 = dist2<ND>(p0, p1, p2);
                     if (d2 <= P.R2 && d2 >= P.az2)
-                        pair_fluid<ND>(P, nb, xa, va, rho_a, p_a, inv_rho_a, acc, drho, gap);
+                        pair_fluid<ND, SUM>(P, nb, xa, va, rho_a, p_a, inv_rho_a, acc, drho, gap);
                 }
                 if (__any(n < 0))
-                    neighbour_sweep<ND, false, kFallbackCap, kBrickThreads>(
+                    neighbour_sweep<ND, false, kFallbackCap, kBrickThreads, SUM>(
                         P, F.start, F.x, F.y, F.z, P.hs_fluid, n < 0 && half == 0, F, Wv, lst, xa,
                         va, rho_a, p_a, inv_rho_a, acc, drho, 0.0, gap);
             }
@@ -1084,12 +1151,13 @@ __global__ void __launch_bounds__(kBrickThreads)
             }
             if (active && half == 0) {
                 // with wall candidates k_fluid_walls adds the wall sum and then gravity
+                constexpr int rows = SUM ? ND : ND + 1;
                 const int i = F.perm[s];
                 const bool walls = nw != 0;
 #pragma unroll
                 for (int d = 0; d < ND; d++)
-                    dv[(size_t)i * (ND + 1) + d] = walls ? acc[d] : acc[d] + P.g[d];
-                dv[(size_t)i * (ND + 1) + ND] = drho;
+                    dv[(size_t)i * rows + d] = walls ? acc[d] : acc[d] + P.g[d];
+                if constexpr (!SUM) dv[(size_t)i * rows + ND] = drho;
             }
         }
         __syncthreads();  // the LDS is staged again for the next brick
@@ -1100,7 +1168,8 @@ __global__ void __launch_bounds__(kBrickThreads)
 // k_fluid_brick: dv = (dv + wall sum) + gravity, the reference's order of the sums.
 // Four lanes share a target: lane q takes list entries q, q + 4, ...; the sums are added
 // across the four lanes before the write.
-template <int ND>
+// SUM: SummationDensity (the target's p / rho^2 in the pressure slot, dv of ND rows).
+template <int ND, bool SUM = false>
 __global__ void __launch_bounds__(kBlock)
     k_fluid_walls(DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
                   const int *__restrict__ wtargets, const int *__restrict__ nwtargets,
@@ -1121,7 +1190,8 @@ __global__ void __launch_bounds__(kBlock)
         const double4 a = F.rec[2 * (size_t)s];
         const double4 c = F.rec[2 * (size_t)s + 1];
         const double xa[3] = {a.x, a.y, a.z}, va[3] = {c.x, c.y, c.z};
-        const double rho_a = a.w, p_a = c.w, inv_rho_a = 1.0 / rho_a;
+        const double rho_a = a.w, inv_rho_a = 1.0 / rho_a;
+        const double p_a = SUM ? c.w / (rho_a * rho_a) : c.w;
         const double m_a = P.visc_w == 2 ? F.vol[s].y : 1.0;
         const int nw = active ? wcnt[t] : 0;
         double accw[3] = {0.0, 0.0, 0.0}, drhow = 0.0;
@@ -1135,11 +1205,12 @@ __global__ void __launch_bounds__(kBlock)
             const double p2 = (ND == 3) ? xa[2] - cur.a.z : 0.0;
             const double d2 = dist2<ND>(p0, p1, p2);
             if (d2 <= P.R2 && d2 >= P.az2)
-                pair_wall<ND>(P, cur, xa, va, rho_a, p_a, inv_rho_a, accw, drhow, m_a);
+                pair_wall<ND, SUM>(P, cur, xa, va, rho_a, p_a, inv_rho_a, accw, drhow, m_a);
         }
         if (__any(nw < 0))
-            neighbour_sweep<ND, true>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall, nw < 0 && q == 0, F,
-                                      Wv, lst, xa, va, rho_a, p_a, inv_rho_a, accw, drhow, m_a);
+            neighbour_sweep<ND, true, kCap, kBlock, SUM>(P, Wv.start, Wv.x, Wv.y, Wv.z, P.hs_wall,
+                                                         nw < 0 && q == 0, F, Wv, lst, xa, va,
+                                                         rho_a, p_a, inv_rho_a, accw, drhow, m_a);
 #pragma unroll
         for (int o = 1; o < kLanes; o <<= 1) {
 #pragma unroll
@@ -1147,10 +1218,10 @@ __global__ void __launch_bounds__(kBlock)
             drhow += __shfl_xor(drhow, o);
         }
         if (active && q == 0) {
-            double *o = dv + (size_t)F.perm[s] * (ND + 1);
+            double *o = dv + (size_t)F.perm[s] * (SUM ? ND : ND + 1);
 #pragma unroll
             for (int d = 0; d < ND; d++) o[d] = (o[d] + accw[d]) + P.g[d];
-            o[ND] = o[ND] + drhow;
+            if constexpr (!SUM) o[ND] = o[ND] + drhow;
         }
     }
 }
@@ -1950,8 +2021,13 @@ int prepare_impl(tpsph_handle *h, const double *v, const double *u, bool gathere
     if (h->P.summation && nf > 0) {
         // compute_density! before the state equation (wcsph/system.jl:291-301, 323-326)
         TimedScope t(h, TPSPH_TIME_GATHER);
-        hipLaunchKernelGGL((k_sum_density<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
-                           h->P, h->fluid_view(), h->wall_view(), h->wall_near, h->sum_rho);
+        if (h->search == TPSPH_SEARCH_LISTS)
+            hipLaunchKernelGGL((k_sum_density_lists<ND>), dim3(nblocks(4 * (int64_t)nf)), dim3(kBlock),
+                               0, h->stream, h->P, h->fluid_view(), h->wall_view(), h->ord, h->nbr,
+                               h->cnt, h->wnbr, h->wcnt, h->ls, h->sum_rho);
+        else
+            hipLaunchKernelGGL((k_sum_density<ND>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
+                               h->P, h->fluid_view(), h->wall_view(), h->wall_near, h->sum_rho);
         hipLaunchKernelGGL((k_sum_finish), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream, h->P,
                            nf, h->sum_rho, h->rec, h->vol);
         TRY(launch_check(h, "k_sum_density"));
@@ -2003,19 +2079,24 @@ int kick_impl(tpsph_handle *h, double *dv, const double *v, const double *u, boo
                     h->wnbr, h->wcnt, h->ls, dv);
         } else if (h->search == TPSPH_SEARCH_LISTS && h->bricks) {
             const int vm = visc_mode(h->P);
-            const bool ant = h->P.ddgrad != nullptr;
-            const auto brick = ant ? (vm == kViscNone    ? k_fluid_brick<ND, false, kViscNone, true>
-                                      : vm == kViscAlpha ? k_fluid_brick<ND, false, kViscAlpha, true>
-                                                         : k_fluid_brick<ND, false, kViscAlphaBeta, true>)
-                                   : (vm == kViscNone    ? k_fluid_brick<ND, false, kViscNone>
-                                      : vm == kViscAlpha ? k_fluid_brick<ND, false, kViscAlpha>
-                                                         : k_fluid_brick<ND, false, kViscAlphaBeta>);
+            const bool ant = h->P.ddgrad != nullptr, sum = h->P.summation != 0;
+            const auto brick =
+                ant   ? (vm == kViscNone    ? k_fluid_brick<ND, false, kViscNone, true>
+                         : vm == kViscAlpha ? k_fluid_brick<ND, false, kViscAlpha, true>
+                                            : k_fluid_brick<ND, false, kViscAlphaBeta, true>)
+                : sum ? (vm == kViscNone    ? k_fluid_brick<ND, false, kViscNone, false, true>
+                         : vm == kViscAlpha ? k_fluid_brick<ND, false, kViscAlpha, false, true>
+                                            : k_fluid_brick<ND, false, kViscAlphaBeta, false, true>)
+                      : (vm == kViscNone    ? k_fluid_brick<ND, false, kViscNone>
+                         : vm == kViscAlpha ? k_fluid_brick<ND, false, kViscAlpha>
+                                            : k_fluid_brick<ND, false, kViscAlphaBeta>);
             hipLaunchKernelGGL(brick, dim3(std::max(1, std::min(h->nbricks, h->n_cu))),
                 dim3(kBrickThreads), 0, h->stream, h->flag, h->P, h->fluid_view(), h->wall_view(),
                 h->ord, h->bstart, h->nel, h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot,
                 h->nbr16, h->nbr, h->cnt, h->wnbr, h->wcnt, h->ls, dv, nullptr);
+            const auto walls = sum ? k_fluid_walls<ND, true> : k_fluid_walls<ND, false>;
             if (nw > 0)
-                hipLaunchKernelGGL((k_fluid_walls<ND>), dim3(gate_blocks(4 * (int64_t)nf)), dim3(kBlock), 0, h->stream,
+                hipLaunchKernelGGL(walls, dim3(gate_blocks(4 * (int64_t)nf)), dim3(kBlock), 0, h->stream,
                     h->P, h->fluid_view(), h->wall_view(), h->ord, h->wtargets, h->counters + 2,
                     h->wnbr, h->wcnt, h->ls, dv);
         } else if (h->search == TPSPH_SEARCH_LISTS) {
@@ -2217,10 +2298,8 @@ int tpsph::validate(const tpsph_params *p, int64_t nf, int64_t nw, const double 
     if (p->density_calculator != TPSPH_DENSITY_CONTINUITY &&
         p->density_calculator != TPSPH_DENSITY_SUMMATION)
         return bad("density_calculator must be one of enum tpsph_density_calculator");
-    if (p->density_calculator == TPSPH_DENSITY_SUMMATION &&
-        (p->pair_search == TPSPH_SEARCH_LISTS || p->precision != TPSPH_PRECISION_FP64))
-        return bad("SummationDensity runs the cell-walk pair search (TPSPH_SEARCH_CELLS or "
-                   "AUTO) in FP64");
+    if (p->density_calculator == TPSPH_DENSITY_SUMMATION && p->precision != TPSPH_PRECISION_FP64)
+        return bad("SummationDensity runs in FP64 (TPSPH_PRECISION_FP64)");
     if (p->precision == TPSPH_PRECISION_MIXED && p->use_viscosity && p->beta != 0.0)
         return bad("precision TPSPH_PRECISION_MIXED evaluates ArtificialViscosityMonaghan with "
                    "beta = 0 only");
@@ -2572,8 +2651,17 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     DevParams &P = h->P;
     const int64_t ncells = P.ncells;
     h->search = params->pair_search;
+    // SummationDensity on the Verlet lists needs the brick kernel (P.maxr within one scan
+    // block); the automatic choice takes the cell walk otherwise
+    const bool sum_lists_ok = !P.summation || P.maxr <= kBrickMaxRows;
     if (h->search == TPSPH_SEARCH_AUTO)
-        h->search = (P.half_skin > 0.0 && !P.summation) ? TPSPH_SEARCH_LISTS : TPSPH_SEARCH_CELLS;
+        h->search = (P.half_skin > 0.0 && sum_lists_ok) ? TPSPH_SEARCH_LISTS : TPSPH_SEARCH_CELLS;
+    if (h->search == TPSPH_SEARCH_LISTS && !sum_lists_ok) {
+        delete h;
+        return fail(nullptr, TPSPH_ERR_ARGUMENT,
+                    "SummationDensity with TPSPH_SEARCH_LISTS needs the brick kernel: the "
+                    "default cell divisions, or TPSPH_SEARCH_CELLS");
+    }
     if (h->search == TPSPH_SEARCH_LISTS) {
         // capacities: 1.5 x the expected count in the list ball (particle volume m / rho0)
         // plus 16; walls fill at most 7/8 (3D) or 3/4 (2D) of the ball around a corner

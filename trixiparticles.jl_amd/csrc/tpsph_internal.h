@@ -488,7 +488,10 @@ inline int visc_mode(const DevParams &P)
 //
 // ga, gb (DensityDiffusionAntuono only): the renormalised density gradients of the target and
 // the candidate, whose term is pair_fluid's.
-template <int ND, int VISC, bool ANT = false>
+// SUM (SummationDensity): the pressure term -m_b (p_a/rho_a^2 + p_b/rho_b^2) grad W with the
+// target's p/rho^2 in p_a and the candidate's in n.b.w (the brick kernel stages p/rho^2 in
+// the pressure slot), and no density rate.
+template <int ND, int VISC, bool ANT = false, bool SUM = false>
 __device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const FluidNbr &n,
                                                   bool valid, const double *xa,
                                                   const double *va, double rho_a, double p_a,
@@ -506,7 +509,7 @@ __device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const Flui
     double vr = v0 * p0 + v1 * p1;
     if constexpr (ND == 3) vr += v2 * p2;
     const double rho_b = n.a.w;
-    double fac = -n.v.x * inv_rho_a * (p_a + n.b.w);
+    double fac = SUM ? -n.v.y * (p_a + n.b.w) : -n.v.x * inv_rho_a * (p_a + n.b.w);
     if constexpr (VISC != kViscNone) {
         const double q = d2 + P.eps_h2;
         const double sum_rho = rho_a + rho_b;
@@ -521,13 +524,15 @@ __device__ __forceinline__ void pair_fluid_masked(const DevParams &P, const Flui
     acc[0] = fma(fac, p0, acc[0]);
     acc[1] = fma(fac, p1, acc[1]);
     if constexpr (ND == 3) acc[2] = fma(fac, p2, acc[2]);
-    double drr = rho_a - rho_b;
-    if constexpr (ANT) {
-        double gr = (ga[0] + gb[0]) * p0 + (ga[1] + gb[1]) * p1;
-        if constexpr (ND == 3) gr += (ga[2] + gb[2]) * p2;
-        drr = fma(-0.5, gr, drr);
+    if constexpr (!SUM) {
+        double drr = rho_a - rho_b;
+        if constexpr (ANT) {
+            double gr = (ga[0] + gb[0]) * p0 + (ga[1] + gb[1]) * p1;
+            if constexpr (ND == 3) gr += (ga[2] + gb[2]) * p2;
+            drr = fma(-0.5, gr, drr);
+        }
+        drho = fma(n.v.x * kdr, fma(P.dd2, drr, rho_a * vr), drho);
     }
-    drho = fma(n.v.x * kdr, fma(P.dd2, drr, rho_a * vr), drho);
 }
 
 // Two-phase sweep over the neighbours of one target: phase 1 runs the exact radius test over
