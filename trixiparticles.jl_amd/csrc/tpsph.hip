@@ -405,6 +405,132 @@ __global__ void __launch_bounds__(kBlock)
     if (active && q == 0) rho[s] = sum;
 }
 
+// The fluid part of the sums brick by brick: the union staged in LDS as {x, y} and {z, m}
+// fields (the far record: far away, zero mass), each target summing m W over its list's LDS
+// indices, four lanes per target; the particle itself (m W(0)) and the wall part as in
+// k_sum_density_lists.  Bricks that do not stage take the global lists.  Blocks take bricks
+// from work[2].
+template <int ND>
+struct SumSmem {
+    double2 p0[kBrickCap], p1[kBrickCap];  // {x, y}, {z, m} (2D: {0, m})
+    int row_gb[kBrickMaxRows], row_lds[kBrickMaxRows + 1];
+    int item;
+};
+
+template <int ND>
+__global__ void __launch_bounds__(kBrickThreads)
+    k_sum_density_brick(DevParams P, FluidView F, WallView Wv, const int *__restrict__ ord,
+                        const int *__restrict__ bstart, const int *__restrict__ nel,
+                        const int *__restrict__ nne, int *__restrict__ work,
+                        const int *__restrict__ brow_gb, const int *__restrict__ brow_lds,
+                        const int *__restrict__ bntot, const unsigned short *__restrict__ nbr16,
+                        const int *__restrict__ nbr, const int *__restrict__ cnt,
+                        const int *__restrict__ wnbr, const int *__restrict__ wcnt, int ls,
+                        double *__restrict__ rho)
+{
+    __shared__ SumSmem<ND> S;
+    const int tid = threadIdx.x;
+    const int ne = *nne;
+    const int cap = P.brick_cap;
+    while (true) {
+        if (tid == 0) S.item = atomicAdd(&work[2], 1);
+        __syncthreads();
+        const int item = S.item;
+        if (item >= ne) break;
+        const int b = nel[item];
+        const int t0 = bstart[b], t1 = bstart[b + 1];
+        const int ntot = bntot[b];
+        const bool staged = ntot <= cap - 1;
+        if (staged) {
+            for (int r = tid; r <= P.maxr; r += kBrickThreads) {
+                if (r < P.maxr) S.row_gb[r] = brow_gb[(size_t)b * P.maxr + r];
+                S.row_lds[r] = brow_lds[(size_t)b * (P.maxr + 1) + r];
+            }
+            if (tid == 0) {  // the far record
+                S.p0[cap - 1] = make_double2(kFar, kFar);
+                S.p1[cap - 1] = make_double2(kFar, 0.0);
+            }
+            __syncthreads();
+            for (int i = tid; i < ntot; i += kBrickThreads) {
+                const int r = bsearch_le(S.row_lds, P.maxr + 1, i);
+                const int g = S.row_gb[r] + (i - S.row_lds[r]);
+                const double4 a = F.rec[2 * (size_t)g];
+                S.p0[i] = make_double2(a.x, a.y);
+                S.p1[i] = make_double2(ND == 3 ? a.z : 0.0, F.vol[g].y);
+            }
+            __syncthreads();
+        }
+        const int q = tid & 3;
+        for (int base = t0; base < t1; base += kBrickThreads / 4) {
+            const int t = base + tid / 4;
+            const bool active = t < t1;
+            const int col = active ? t : t0;
+            const int s = ord[col];
+            const double4 a = F.rec[2 * (size_t)s];
+            const double x0 = a.x, x1 = a.y, x2 = (ND == 3) ? a.z : 0.0;
+            double sum = (q == 0) ? F.vol[s].y * wendland_c2(P, 0.0) : 0.0;
+            const int n = active ? cnt[col] : 0;
+            if (staged) {
+                const ushort4 *l4 = reinterpret_cast<const ushort4 *>(nbr16);
+                const int ng = (max(n, 0) + 3) >> 2;
+                // rows clamped into the list rather than a select between a load and a
+                // register (which made the compiler issue flat loads)
+                ushort4 cur4 = l4[(size_t)min(q, max(ng - 1, 0)) * ls + col];
+                for (int g = q; g < ng; g += 4) {
+                    const ushort4 nxt4 = l4[(size_t)min(g + 4, ng - 1) * ls + col];
+                    const unsigned short e4[4] = {cur4.x, cur4.y, cur4.z, cur4.w};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const double2 b0 = S.p0[e4[k]], b1 = S.p1[e4[k]];
+                        const double d2 = dist2<ND>(x0 - b0.x, x1 - b0.y, (ND == 3) ? x2 - b1.x : 0.0);
+                        // the far record (and the slots past the list) add zero mass
+                        if (d2 <= P.R2) sum += b1.y * wendland_c2(P, sqrt(d2));
+                    }
+                    cur4 = nxt4;
+                }
+            } else if (n >= 0) {
+                for (int k = q; k < n; k += 4) {
+                    const int j = nbr[(size_t)k * ls + col];
+                    const double4 c = F.rec[2 * (size_t)j];
+                    const double d2 = dist2<ND>(x0 - c.x, x1 - c.y, (ND == 3) ? x2 - c.z : 0.0);
+                    if (d2 <= P.R2) sum += F.vol[j].y * wendland_c2(P, sqrt(d2));
+                }
+            } else {
+                Pencils<ND> it;
+                it.init(P, x0, x1, x2, P.hs_fluid);
+                int jb, je;
+                while (it.next(P, F.start, jb, je))
+                    for (int j = jb + q; j < je; j += 4) {
+                        const double d2 = dist2<ND>(x0 - F.x[j], x1 - F.y[j], (ND == 3) ? x2 - F.z[j] : 0.0);
+                        if (d2 <= P.R2 && j != s) sum += F.vol[j].y * wendland_c2(P, sqrt(d2));
+                    }
+            }
+            const int nw = (active && P.nw > 0) ? wcnt[col] : 0;
+            if (nw > 0) {
+                for (int k = q; k < nw; k += 4) {
+                    const double4 xw = Wv.pos[wnbr[(size_t)k * ls + col]];
+                    const double d2 = dist2<ND>(x0 - xw.x, x1 - xw.y, (ND == 3) ? x2 - xw.z : 0.0);
+                    if (d2 <= P.R2) sum += xw.w * wendland_c2(P, sqrt(d2));
+                }
+            } else if (nw < 0) {
+                Pencils<ND> iw;
+                iw.init(P, x0, x1, x2, P.hs_wall);
+                int jb, je;
+                while (iw.next(P, Wv.start, jb, je))
+                    for (int w = jb + q; w < je; w += 4) {
+                        const double4 xw = Wv.pos[w];
+                        const double d2 = dist2<ND>(x0 - xw.x, x1 - xw.y, (ND == 3) ? x2 - xw.z : 0.0);
+                        if (d2 <= P.R2) sum += xw.w * wendland_c2(P, sqrt(d2));
+                    }
+            }
+            sum += __shfl_xor(sum, 1);
+            sum += __shfl_xor(sum, 2);
+            if (active && q == 0) rho[s] = sum;
+        }
+        __syncthreads();  // the LDS is staged again for the next brick
+    }
+}
+
 // The records of the summed densities: rho, the state equation's pressure and V = m / rho.
 __global__ void __launch_bounds__(kBlock)
     k_sum_finish(DevParams P, int nf, const double *__restrict__ rho, double4 *__restrict__ rec,
@@ -2021,7 +2147,13 @@ int prepare_impl(tpsph_handle *h, const double *v, const double *u, bool gathere
     if (h->P.summation && nf > 0) {
         // compute_density! before the state equation (wcsph/system.jl:291-301, 323-326)
         TimedScope t(h, TPSPH_TIME_GATHER);
-        if (h->search == TPSPH_SEARCH_LISTS)
+        if (h->search == TPSPH_SEARCH_LISTS && h->bricks)
+            hipLaunchKernelGGL((k_sum_density_brick<ND>),
+                dim3(std::max(1, std::min(h->nbricks, h->n_cu))), dim3(kBrickThreads), 0,
+                h->stream, h->P, h->fluid_view(), h->wall_view(), h->ord, h->bstart, h->nel,
+                h->counters, h->bwork, h->brow_gb, h->brow_lds, h->bntot, h->nbr16, h->nbr,
+                h->cnt, h->wnbr, h->wcnt, h->ls, h->sum_rho);
+        else if (h->search == TPSPH_SEARCH_LISTS)
             hipLaunchKernelGGL((k_sum_density_lists<ND>), dim3(nblocks(4 * (int64_t)nf)), dim3(kBlock),
                                0, h->stream, h->P, h->fluid_view(), h->wall_view(), h->ord, h->nbr,
                                h->cnt, h->wnbr, h->wcnt, h->ls, h->sum_rho);
