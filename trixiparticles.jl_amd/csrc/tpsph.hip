@@ -940,6 +940,9 @@ __device__ __forceinline__ void
     }
     const int nuy = P.bc[1] + 2 * P.ur[1];
     for (int b = bx; b < nbricks; b += gx) {
+        // only the bricks with targets are staged or looked up (most of a tank's bricks hold
+        // no fluid); the condition is uniform over the block
+        if (bstart[b + 1] == bstart[b]) continue;
         const int bx = b % P.nb[0], by = (b / P.nb[0]) % P.nb[1], bz = b / (P.nb[0] * P.nb[1]);
         const int X0 = bx * P.bc[0], X1 = min(X0 + P.bc[0], P.n[0]) - 1;
         const int Y0 = by * P.bc[1], Y1 = min(Y0 + P.bc[1], P.n[1]) - 1;
