@@ -160,8 +160,7 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
                 }
             }
         }
-        // one atomic per wavefront that has a particle out of its skin
-        if (MODE == 0 && __any(moved) && ((int)threadIdx.x & 63) == 0) atomicOr(flag, 1);
+        if constexpr (MODE == 0) raise_rebuild_flag(flag, moved);
         if (MODE == 1 && bcount) {
             // a wavefront holds consecutive cell-sorted particles, so equal bricks come in runs:
             // the first lane of a run takes ranks for the whole run with one atomic, and lane l
@@ -1685,8 +1684,7 @@ __global__ void __launch_bounds__(kBlock)
         }
         moved = !(e2 <= P.half_skin2);
     }
-    // one atomic per wavefront that has a particle out of its skin
-    if (__any(moved) && ((int)threadIdx.x & 63) == 0) atomicOr(flag, 1);
+    raise_rebuild_flag(flag, moved);
 }
 
 // ------------------------------------------------------------------------------------------

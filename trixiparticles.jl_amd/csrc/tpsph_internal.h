@@ -138,6 +138,16 @@ __device__ __forceinline__ double dist2(double a, double b, double c)
     return s;
 }
 
+// The rebuild flag: one atomic per wavefront that has a particle out of its skin, skipped
+// once the flag reads as raised (when most particles have moved, thousands of atomics on
+// the one address would serialise)
+__device__ __forceinline__ void raise_rebuild_flag(int *flag, bool moved)
+{
+    if (__any(moved) && ((int)threadIdx.x & 63) == 0 &&
+        __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        atomicOr(flag, 1);
+}
+
 __device__ __forceinline__ int cell_1d(double x, double lo, double inv, int n)
 {
     double f = floor((x - lo) * inv);
