@@ -101,7 +101,8 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
                          double *__restrict__ snap, int *__restrict__ iperm,
                          int *__restrict__ bid, int *__restrict__ rank,
                          int *__restrict__ bcount, int *__restrict__ zero, int nzero,
-                         const int *__restrict__ noverflow, int *__restrict__ nrebuild)
+                         const int *__restrict__ noverflow, int *__restrict__ nrebuild,
+                         float4 *__restrict__ xf)
 {
     P = with_sound(P);
     const int f = *flag;
@@ -152,6 +153,20 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
 #pragma unroll
                 for (int d = 0; d < ND; d++) snap[d * nf + s] = x[d];
                 iperm[i] = s;
+                if (xf) {
+                    // relative to the grid origin, NaN outside the grid box (the FP32
+                    // pre-test of the list build then leaves the particle to FP64)
+                    bool inbox = true;
+                    float c[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int d = 0; d < ND; d++) {
+                        const double r = x[d] - P.lo[d];
+                        inbox = inbox && r >= 0.0 && r <= P.box[d];
+                        c[d] = (float)r;
+                    }
+                    const float nan = __builtin_nanf("");
+                    xf[s] = inbox ? make_float4(c[0], c[1], c[2], 0.f) : make_float4(nan, nan, nan, nan);
+                }
                 if (bcount) {
                     const int c0 = cell_1d(x[0], P.lo[0], P.inv_cs[0], P.n[0]);
                     const int c1 = cell_1d(x[1], P.lo[1], P.inv_cs[1], P.n[1]);
@@ -604,6 +619,11 @@ __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &
     const double *cy = WALL ? Wv.y : F.y;
     const double *cz = WALL ? Wv.z : F.z;
     const double rl2 = WALL ? P.rl2_wall : P.rl2_fluid;
+    // fluid candidates: an FP32 test on F.xf first; only a distance within a few FP32 ulps
+    // of the box size of the list radius (or a particle outside the grid box, NaN there)
+    // takes the FP64 test, so the lists are those of the FP64 test
+    float4 xaf = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (!WALL) xaf = F.xf[s];
     Pencils<ND> it;
     it.init(P, xa, ya, za, WALL ? P.hs_list_wall : P.hs_list_fluid);
     int n = 0, j, je;
@@ -627,14 +647,33 @@ __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &
         // kPad entries past their end, so the loads stay in bounds, and j + k < je masks the
         // candidates past the range (the list order is the candidates' order)
         for (; j < je; j += 4) {
-            double d2[4];
+            bool hit[4];
+            if constexpr (WALL) {
+                double d2[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                d2[k] = dist2<ND>(xa - cx[j + k], ya - cy[j + k],
-                                  (ND == 3) ? za - cz[j + k] : 0.0);
+                for (int k = 0; k < 4; k++)
+                    d2[k] = dist2<ND>(xa - cx[j + k], ya - cy[j + k],
+                                      (ND == 3) ? za - cz[j + k] : 0.0);
+#pragma unroll
+                for (int k = 0; k < 4; k++) hit[k] = d2[k] <= rl2;
+            } else {
+                float4 q[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) q[k] = F.xf[j + k];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float ex = xaf.x - q[k].x, ey = xaf.y - q[k].y;
+                    const float ez = (ND == 3) ? xaf.z - q[k].z : 0.f;
+                    const float d2f = ex * ex + ey * ey + ez * ez;
+                    hit[k] = d2f <= P.list_in2;
+                    if (!hit[k] && !(d2f > P.list_out2) && j + k < je)
+                        hit[k] = dist2<ND>(xa - cx[j + k], ya - cy[j + k],
+                                           (ND == 3) ? za - cz[j + k] : 0.0) <= rl2;
+                }
+            }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                if (j + k < je && d2[k] <= rl2 && (WALL || j + k != s)) {
+                if (j + k < je && hit[k] && (WALL || j + k != s)) {
                     if (n < cap) {
                         nbr[(size_t)n * ls + col] = j + k;
                         if constexpr (BRICK) {
@@ -2075,7 +2114,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
         hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, h->bwork, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
             h->snap, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
-            h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr, nullptr);
+            h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr, nullptr, nullptr);
         return launch_check(h, "k_gather<0>");
     }
     if (!gathered) {  // else the stage kernel wrote the records of (v, u)
@@ -2083,7 +2122,7 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
         hipLaunchKernelGGL((k_gather<ND, 0>), dim3(nblocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, h->bwork, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
             h->snap, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
-            h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr, nullptr);
+            h->search == TPSPH_SEARCH_LISTS ? h->counters + 3 : nullptr, nullptr, nullptr);
         TRY(launch_check(h, "k_gather<0>"));
     }
     {
@@ -2096,7 +2135,8 @@ int update_fluid(tpsph_handle *h, const double *v, const double *u, bool gathere
         hipLaunchKernelGGL((k_gather<ND, 1>), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
             h->flag, nullptr, h->P, h->f_perm, u, v, h->f_mass, h->xs, h->ys, h->zs, h->rec, h->vol,
             h->snap, h->iperm, lists ? h->f_cell : nullptr, lists ? h->f_rank : nullptr,
-            lists ? h->bcount : nullptr, lists ? h->counters : nullptr, 4, nullptr, h->nrebuild);
+            lists ? h->bcount : nullptr, lists ? h->counters : nullptr, 4, nullptr, h->nrebuild,
+            lists ? h->xf : nullptr);
         if (lists) {
             TRY(scan(h, h->flag, h->bcount, h->bstart, h->nbricks + 1));
             hipLaunchKernelGGL((k_brick_scatter), dim3(gate_blocks(nf)), dim3(kBlock), 0, h->stream,
@@ -2351,6 +2391,27 @@ int tpsph::derive_params(const tpsph_params *params, int nf, int nw, const doubl
     if (ncells > (int64_t)INT32_MAX / 4)
         return fail(nullptr, TPSPH_ERR_ARGUMENT, "cell grid too large (domain / smoothing_length)");
     P.ncells = (int)ncells;
+    // FP32 pre-test of the fluid list build.  A position inside the grid box is stored as
+    // fl32(x - lo), off by at most 2^-24 L (L the box's longest side), so an FP32 distance d'
+    // is off by at most 3.5 * 2^-24 L + 2^-24 d from the FP64 one d, and its square by a
+    // further 3.1 * 2^-24 relative: with e = 4 * 2^-24 (L + rl) the bounds leave a margin of
+    // at least 2^-24 rl on either side of the list radius rl
+    {
+        const double u = 0x1p-24;
+        double L = 0.0;
+        for (int d = 0; d < 3; d++) {
+            P.box[d] = d < nd ? P.n[d] * P.cs[d] : 0.0;
+            L = std::fmax(L, P.box[d]);
+        }
+        const double rl = std::sqrt(P.rl2_fluid), e = 4.0 * u * (L + rl);
+        const double in2 = rl > e ? (rl - e) * (rl - e) * (1.0 - 4.0 * u) : -1.0;
+        const double out2 = (rl + e) * (rl + e) * (1.0 + 4.0 * u);
+        float fi = (float)in2, fo = (float)out2;
+        if ((double)fi > in2) fi = std::nextafter(fi, -INFINITY);
+        if ((double)fo < out2) fo = std::nextafter(fo, INFINITY);
+        P.list_in2 = fi;
+        P.list_out2 = fo;
+    }
     {
         // bricks of 3 (3D) or 8 (2D) cell spans per side, about 3.1 R (8.3 R) wide, in 3D
         // shortened along x to 2.75 spans: at h = 1.2 dx a 3D brick holds up to 448 targets
@@ -2885,6 +2946,7 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     CTRY(dalloc(h, &h->xs, (size_t)nf + kPad));
     CTRY(dalloc(h, &h->ys, (size_t)nf + kPad));
     CTRY(dalloc(h, &h->zs, (size_t)nf + kPad));
+    if (h->search == TPSPH_SEARCH_LISTS) CTRY(dalloc(h, &h->xf, (size_t)nf + kPad));
     CTRY(dalloc(h, &h->snap, (size_t)nd * nf));
     CTRY(dalloc(h, &h->iperm, nf));
     CTRY(dalloc(h, &h->rec, 2 * (size_t)nf));
@@ -2935,6 +2997,10 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     // padded entry can not pass the radius test
     for (double *a : {h->xs, h->ys, h->zs})
         CTRY(hip_fill(h, a, (size_t)nf + kPad));
+    if (h->xf && hipMemset(h->xf, 0xff, ((size_t)nf + kPad) * sizeof(float4)) != hipSuccess) {
+        h->last_error = "clearing the FP32 positions failed";
+        return cleanup_fail(TPSPH_ERR_HIP);
+    }
     for (double *a : {h->wx, h->wy, h->wz})
         CTRY(hip_fill(h, a, (size_t)nw + kPad));
     if (hipMemset(h->f_count, 0, ((size_t)ncells + 1) * sizeof(int)) != hipSuccess) {
@@ -3464,7 +3530,7 @@ void tpsph_destroy(tpsph_handle *h)
     if (!h) return;
     (void)hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->flag, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm,
-                    h->scan_tiles, h->f_mass, h->xs, h->ys, h->zs, h->snap, h->rec, h->vol,
+                    h->scan_tiles, h->f_mass, h->xs, h->ys, h->zs, h->xf, h->snap, h->rec, h->vol,
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
                     h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,

@@ -60,6 +60,11 @@ struct DevParams {
     double half_skin, half_skin2;
     // Verlet lists: squared list radii and the cell growth that makes the pencils reach them
     double rl2_fluid, rl2_wall, hs_list_fluid, hs_list_wall;
+    // FP32 pre-test of the fluid list build (positions relative to lo inside the grid box
+    // box[d]): a squared FP32 distance at most list_in2 is surely within the list radius,
+    // one above list_out2 surely beyond it
+    double box[3];
+    float list_in2, list_out2;
     // bricks of cells (processing order of the list kernels): cells per brick, bricks per axis
     int bc[3], nb[3];
     // union rows of a brick: cells beyond the brick along y and z, rows per brick, and the
@@ -114,6 +119,7 @@ struct FluidView {
     const double *x, *y, *z;
     const double4 *rec;
     const double2 *vol;
+    const float4 *xf = nullptr;  // the FP32 positions of the list build (Verlet lists only)
 };
 
 struct WallView {
@@ -784,6 +790,7 @@ struct tpsph_handle {
     int *scan_tiles = nullptr;
     double *f_mass = nullptr;  // caller order
     double *xs = nullptr, *ys = nullptr, *zs = nullptr, *snap = nullptr;
+    float4 *xf = nullptr;  // Verlet lists: FP32 positions at the last rebuild (+ kPad NaN)
     int *iperm = nullptr;  // sorted index of every caller particle (last rebuild)
     double4 *rec = nullptr;
     double2 *vol = nullptr;
@@ -846,7 +853,7 @@ struct tpsph_handle {
     size_t events_used = 0;
     std::vector<Rec> records;
 
-    tpsph::FluidView fluid_view() const { return {f_start, f_perm, xs, ys, zs, rec, vol}; }
+    tpsph::FluidView fluid_view() const { return {f_start, f_perm, xs, ys, zs, rec, vol, xf}; }
     tpsph::WallView wall_view() const { return {w_start, wx, wy, wz, wpos, wpv, wvel}; }
 };
 
