@@ -92,6 +92,34 @@ __global__ void k_brick_scatter(const int *flag, int n, const int *__restrict__ 
 // MODE 1 with bcount also counts the particles of every brick of cells (the processing
 // order of the list kernels: see k_brick_scatter), recording each particle's brick and rank,
 // and zeroes the `nzero` counters of the list build.
+// FP32 position for the pre-test of the list build: relative to the grid origin, NaN
+// outside the grid box (the list build then leaves the particle to the FP64 test)
+template <int ND>
+__device__ __forceinline__ float4 list_f32(const DevParams &P, double x0, double x1, double x2)
+{
+    const double x[3] = {x0, x1, x2};
+    bool inbox = true;
+    float c[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+        const double r = x[d] - P.lo[d];
+        inbox = inbox && r >= 0.0 && r <= P.box[d];
+        c[d] = (float)r;
+    }
+    const float nan = __builtin_nanf("");
+    return inbox ? make_float4(c[0], c[1], c[2], 0.f) : make_float4(nan, nan, nan, nan);
+}
+
+template <int ND>
+__global__ void k_wall_f32(DevParams P, int nw, const double4 *__restrict__ wpos,
+                           float4 *__restrict__ wxf)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nw) return;
+    const double4 x = wpos[w];
+    wxf[w] = list_f32<ND>(P, x.x, x.y, x.z);
+}
+
 template <int ND, int MODE>
 __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restrict__ perm,
                          const double *__restrict__ u, const double *__restrict__ v,
@@ -153,20 +181,7 @@ __global__ void k_gather(int *flag, int *work, DevParams P, const int *__restric
 #pragma unroll
                 for (int d = 0; d < ND; d++) snap[d * nf + s] = x[d];
                 iperm[i] = s;
-                if (xf) {
-                    // relative to the grid origin, NaN outside the grid box (the FP32
-                    // pre-test of the list build then leaves the particle to FP64)
-                    bool inbox = true;
-                    float c[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int d = 0; d < ND; d++) {
-                        const double r = x[d] - P.lo[d];
-                        inbox = inbox && r >= 0.0 && r <= P.box[d];
-                        c[d] = (float)r;
-                    }
-                    const float nan = __builtin_nanf("");
-                    xf[s] = inbox ? make_float4(c[0], c[1], c[2], 0.f) : make_float4(nan, nan, nan, nan);
-                }
+                if (xf) xf[s] = list_f32<ND>(P, x[0], x[1], x[2]);
                 if (bcount) {
                     const int c0 = cell_1d(x[0], P.lo[0], P.inv_cs[0], P.n[0]);
                     const int c1 = cell_1d(x[1], P.lo[1], P.inv_cs[1], P.n[1]);
@@ -619,11 +634,12 @@ __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &
     const double *cy = WALL ? Wv.y : F.y;
     const double *cz = WALL ? Wv.z : F.z;
     const double rl2 = WALL ? P.rl2_wall : P.rl2_fluid;
-    // fluid candidates: an FP32 test on F.xf first; only a distance within a few FP32 ulps
-    // of the box size of the list radius (or a particle outside the grid box, NaN there)
-    // takes the FP64 test, so the lists are those of the FP64 test
-    float4 xaf = make_float4(0.f, 0.f, 0.f, 0.f);
-    if constexpr (!WALL) xaf = F.xf[s];
+    // an FP32 test on the FP32 positions first; only a distance within a few FP32 ulps of
+    // the box size of the list radius (or a particle outside the grid box, NaN there) takes
+    // the FP64 test, so the lists are those of the FP64 test
+    const float4 xaf = F.xf[s];
+    const float4 *cf = WALL ? Wv.xf : F.xf;
+    const float in2 = WALL ? P.wlist_in2 : P.list_in2, out2 = WALL ? P.wlist_out2 : P.list_out2;
     Pencils<ND> it;
     it.init(P, xa, ya, za, WALL ? P.hs_list_wall : P.hs_list_fluid);
     int n = 0, j, je;
@@ -648,28 +664,18 @@ __device__ __forceinline__ void build_list(const DevParams &P, const FluidView &
         // candidates past the range (the list order is the candidates' order)
         for (; j < je; j += 4) {
             bool hit[4];
-            if constexpr (WALL) {
-                double d2[4];
+            float4 q[4];
 #pragma unroll
-                for (int k = 0; k < 4; k++)
-                    d2[k] = dist2<ND>(xa - cx[j + k], ya - cy[j + k],
-                                      (ND == 3) ? za - cz[j + k] : 0.0);
+            for (int k = 0; k < 4; k++) q[k] = cf[j + k];
 #pragma unroll
-                for (int k = 0; k < 4; k++) hit[k] = d2[k] <= rl2;
-            } else {
-                float4 q[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) q[k] = F.xf[j + k];
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const float ex = xaf.x - q[k].x, ey = xaf.y - q[k].y;
-                    const float ez = (ND == 3) ? xaf.z - q[k].z : 0.f;
-                    const float d2f = ex * ex + ey * ey + ez * ez;
-                    hit[k] = d2f <= P.list_in2;
-                    if (!hit[k] && !(d2f > P.list_out2) && j + k < je)
-                        hit[k] = dist2<ND>(xa - cx[j + k], ya - cy[j + k],
-                                           (ND == 3) ? za - cz[j + k] : 0.0) <= rl2;
-                }
+            for (int k = 0; k < 4; k++) {
+                const float ex = xaf.x - q[k].x, ey = xaf.y - q[k].y;
+                const float ez = (ND == 3) ? xaf.z - q[k].z : 0.f;
+                const float d2f = ex * ex + ey * ey + ez * ez;
+                hit[k] = d2f <= in2;
+                if (!hit[k] && !(d2f > out2) && j + k < je)
+                    hit[k] = dist2<ND>(xa - cx[j + k], ya - cy[j + k],
+                                       (ND == 3) ? za - cz[j + k] : 0.0) <= rl2;
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -767,13 +773,20 @@ __device__ __forceinline__ void
     if (skip(flag)) return;
     for (int w = bx * kBlock + threadIdx.x; w < P.nw; w += gx * kBlock) {
         const double4 xw = wpos[w];
+        const float4 xwf = list_f32<ND>(P, xw.x, xw.y, xw.z);
         Pencils<ND> it;
         it.init(P, xw.x, xw.y, xw.z, P.hs_list_wall);
         int n = 0, j, je;
         while (it.next(P, F.start, j, je)) {
             for (; j < je; j++) {
-                const double d2 = dist2<ND>(xw.x - F.x[j], xw.y - F.y[j], (ND == 3) ? xw.z - F.z[j] : 0.0);
-                if (d2 <= P.rl2_wall) {
+                // the FP32 pre-test of the list build, FP64 in its band (and for NaN)
+                const float4 q = F.xf[j];
+                const float ex = xwf.x - q.x, ey = xwf.y - q.y, ez = (ND == 3) ? xwf.z - q.z : 0.f;
+                const float d2f = ex * ex + ey * ey + ez * ez;
+                bool hit = d2f <= P.wlist_in2;
+                if (!hit && !(d2f > P.wlist_out2))
+                    hit = dist2<ND>(xw.x - F.x[j], xw.y - F.y[j], (ND == 3) ? xw.z - F.z[j] : 0.0) <= P.rl2_wall;
+                if (hit) {
                     if (n < cap) nbr[(size_t)n * lw + w] = j;
                     n++;
                 }
@@ -2403,14 +2416,17 @@ int tpsph::derive_params(const tpsph_params *params, int nf, int nw, const doubl
             P.box[d] = d < nd ? P.n[d] * P.cs[d] : 0.0;
             L = std::fmax(L, P.box[d]);
         }
-        const double rl = std::sqrt(P.rl2_fluid), e = 4.0 * u * (L + rl);
-        const double in2 = rl > e ? (rl - e) * (rl - e) * (1.0 - 4.0 * u) : -1.0;
-        const double out2 = (rl + e) * (rl + e) * (1.0 + 4.0 * u);
-        float fi = (float)in2, fo = (float)out2;
-        if ((double)fi > in2) fi = std::nextafter(fi, -INFINITY);
-        if ((double)fo < out2) fo = std::nextafter(fo, INFINITY);
-        P.list_in2 = fi;
-        P.list_out2 = fo;
+        auto bounds = [&](double rl2, float &fi, float &fo) {
+            const double rl = std::sqrt(rl2), e = 4.0 * u * (L + rl);
+            const double in2 = rl > e ? (rl - e) * (rl - e) * (1.0 - 4.0 * u) : -1.0;
+            const double out2 = (rl + e) * (rl + e) * (1.0 + 4.0 * u);
+            fi = (float)in2;
+            fo = (float)out2;
+            if ((double)fi > in2) fi = std::nextafter(fi, -INFINITY);
+            if ((double)fo < out2) fo = std::nextafter(fo, INFINITY);
+        };
+        bounds(P.rl2_fluid, P.list_in2, P.list_out2);
+        bounds(P.rl2_wall, P.wlist_in2, P.wlist_out2);
     }
     {
         // bricks of 3 (3D) or 8 (2D) cell spans per side, about 3.1 R (8.3 R) wide, in 3D
@@ -2947,6 +2963,7 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     CTRY(dalloc(h, &h->ys, (size_t)nf + kPad));
     CTRY(dalloc(h, &h->zs, (size_t)nf + kPad));
     if (h->search == TPSPH_SEARCH_LISTS) CTRY(dalloc(h, &h->xf, (size_t)nf + kPad));
+    if (h->search == TPSPH_SEARCH_LISTS && nw > 0) CTRY(dalloc(h, &h->wxf, (size_t)nw + kPad));
     CTRY(dalloc(h, &h->snap, (size_t)nd * nf));
     CTRY(dalloc(h, &h->iperm, nf));
     CTRY(dalloc(h, &h->rec, 2 * (size_t)nf));
@@ -2997,7 +3014,8 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
     // padded entry can not pass the radius test
     for (double *a : {h->xs, h->ys, h->zs})
         CTRY(hip_fill(h, a, (size_t)nf + kPad));
-    if (h->xf && hipMemset(h->xf, 0xff, ((size_t)nf + kPad) * sizeof(float4)) != hipSuccess) {
+    if ((h->xf && hipMemset(h->xf, 0xff, ((size_t)nf + kPad) * sizeof(float4)) != hipSuccess) ||
+        (h->wxf && hipMemset(h->wxf, 0xff, ((size_t)nw + kPad) * sizeof(float4)) != hipSuccess)) {
         h->last_error = "clearing the FP32 positions failed";
         return cleanup_fail(TPSPH_ERR_HIP);
     }
@@ -3052,6 +3070,10 @@ int tpsph_create(const tpsph_params *params, int64_t n_fluid, int64_t n_wall,
                 hipLaunchKernelGGL((k_wall_near<2>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P, nw, h->wpos, h->wall_near);
             else
                 hipLaunchKernelGGL((k_wall_near<3>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P, nw, h->wpos, h->wall_near);
+            if (h->wxf && nd == 2)
+                hipLaunchKernelGGL((k_wall_f32<2>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P, nw, h->wpos, h->wxf);
+            else if (h->wxf)
+                hipLaunchKernelGGL((k_wall_f32<3>), dim3(nblocks(nw)), dim3(kBlock), 0, h->stream, h->P, nw, h->wpos, h->wxf);
             rc = launch_check(h, "wall setup");
         }
         hipError_t es = hipStreamSynchronize(h->stream);
@@ -3530,7 +3552,7 @@ void tpsph_destroy(tpsph_handle *h)
     if (!h) return;
     (void)hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->flag, h->f_cell, h->f_rank, h->f_count, h->f_start, h->f_perm,
-                    h->scan_tiles, h->f_mass, h->xs, h->ys, h->zs, h->xf, h->snap, h->rec, h->vol,
+                    h->scan_tiles, h->f_mass, h->xs, h->ys, h->zs, h->xf, h->wxf, h->snap, h->rec, h->vol,
                     h->w_start, h->w_perm, h->wx, h->wy, h->wz, h->wpos, h->wpv, h->wall_near,
                     h->pw, h->nbr, h->cnt, h->wnbr, h->wcnt, h->anbr, h->acnt, h->ord, h->bcount, h->bstart,
                     h->brow_gb, h->brow_lds, h->bntot, h->nel, h->counters, h->wactive, h->bwork, h->nbr16,

@@ -65,6 +65,7 @@ struct DevParams {
     // one above list_out2 surely beyond it
     double box[3];
     float list_in2, list_out2;
+    float wlist_in2, wlist_out2;  // the same bounds for the wall list radius
     // bricks of cells (processing order of the list kernels): cells per brick, bricks per axis
     int bc[3], nb[3];
     // union rows of a brick: cells beyond the brick along y and z, rows per brick, and the
@@ -128,6 +129,7 @@ struct WallView {
     const double4 *pos;  // {x, y, z, m}
     const double2 *pv;   // {p_w, m/rho_w} of the current kick
     const double *vel;   // wall velocity for the viscosity (3 per particle), or nullptr
+    const float4 *xf = nullptr;  // the FP32 positions of the list build (Verlet lists only)
 };
 // ------------------------------------------------------------------------------------------
 // device helpers
@@ -791,6 +793,7 @@ struct tpsph_handle {
     double *f_mass = nullptr;  // caller order
     double *xs = nullptr, *ys = nullptr, *zs = nullptr, *snap = nullptr;
     float4 *xf = nullptr;  // Verlet lists: FP32 positions at the last rebuild (+ kPad NaN)
+    float4 *wxf = nullptr;  // Verlet lists: FP32 positions of the sorted walls (+ kPad NaN)
     int *iperm = nullptr;  // sorted index of every caller particle (last rebuild)
     double4 *rec = nullptr;
     double2 *vol = nullptr;
@@ -854,7 +857,7 @@ struct tpsph_handle {
     std::vector<Rec> records;
 
     tpsph::FluidView fluid_view() const { return {f_start, f_perm, xs, ys, zs, rec, vol, xf}; }
-    tpsph::WallView wall_view() const { return {w_start, wx, wy, wz, wpos, wpv, wvel}; }
+    tpsph::WallView wall_view() const { return {w_start, wx, wy, wz, wpos, wpv, wvel, wxf}; }
 };
 
 
