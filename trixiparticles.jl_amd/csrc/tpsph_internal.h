@@ -661,20 +661,62 @@ namespace tpsph {
 // ------------------------------------------------------------------------------------------
 namespace {
 
+// Exclusive scan over the block: an inclusive scan within each wavefront by shuffles, then
+// the wavefront totals through LDS (s holds at least kBlock / 64 ints)
 __device__ __forceinline__ int block_exclusive_scan(int value, int *s, int &total)
 {
-    s[threadIdx.x] = value;
-    __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-        int t = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    int x = value;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(x, off);
+        if (lane >= off) x += t;
     }
-    total = s[kBlock - 1];
-    int excl = s[threadIdx.x] - value;
+    if (lane == 63) s[wave] = x;
     __syncthreads();
-    return excl;
+    int below = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) {
+        const int t = s[w];
+        below += w < wave ? t : 0;
+        total += t;
+    }
+    __syncthreads();  // s is free again for the next call
+    return below + x - value;
+}
+
+// kScanItems consecutive ints from p + base (16-byte loads where p is 16-byte aligned and
+// the items lie within n), zeros past n
+__device__ __forceinline__ void scan_load(const int *p, int base, int n, bool vec, int *v)
+{
+    static_assert(kScanItems == 8, "two int4 per thread");
+    if (vec && base + kScanItems <= n) {
+        const int4 a = *reinterpret_cast<const int4 *>(p + base);
+        const int4 b = *reinterpret_cast<const int4 *>(p + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++) v[k] = (base + k < n) ? p[base + k] : 0;
+    }
+}
+
+__device__ __forceinline__ void scan_store(int *p, int base, int n, bool vec, const int *r)
+{
+    if (vec && base + kScanItems <= n) {
+        *reinterpret_cast<int4 *>(p + base) = make_int4(r[0], r[1], r[2], r[3]);
+        *reinterpret_cast<int4 *>(p + base + 4) = make_int4(r[4], r[5], r[6], r[7]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; k++)
+            if (base + k < n) p[base + k] = r[k];
+    }
+}
+
+__device__ __forceinline__ bool aligned16(const void *p)
+{
+    return ((uintptr_t)p & 15u) == 0;
 }
 
 [[maybe_unused]] __global__ void __launch_bounds__(kBlock)
@@ -682,21 +724,22 @@ __device__ __forceinline__ int block_exclusive_scan(int value, int *s, int &tota
 {
     if (skip(flag)) return;
     __shared__ int s[kBlock];
-    int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    const bool vec = aligned16(in) && aligned16(out);
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
+    scan_load(in, base, n, vec, v);
     int sum = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
-        v[k] = (base + k < n) ? in[base + k] : 0;
-        sum += v[k];
-    }
+    for (int k = 0; k < kScanItems; k++) sum += v[k];
     int total;
     int run = block_exclusive_scan(sum, s, total);
+    int r[kScanItems];
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
-        if (base + k < n) out[base + k] = run;
+        r[k] = run;
         run += v[k];
     }
+    scan_store(out, base, n, vec, r);
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
 }
 
