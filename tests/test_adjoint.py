@@ -228,7 +228,7 @@ def test_autograd_functions_give_the_direct_gradients(tp, ad):
     v, u = ad.Trajectory.apply(v0r, u0r, params, semi, DT, steps, 5)
     L = ad.mse_loss(u, target)
     L.backward()
-    assert float(L) == loss
+    assert float(L.detach()) == loss
     assert torch.equal(v0r.grad, v0b) and torch.equal(u0r.grad, u0b)
     assert np.array_equal(params.grad.cpu().numpy(), pb)
     # Kick: the pullback of one kick through autograd
